@@ -1,0 +1,170 @@
+"""HIP backends over ``libgsw_hip.so`` (AMD Instinct MI355X, gfx950).
+
+:class:`HipBackend` implements the reference's ``Backend`` contract
+(``src/gpusw/backends/__init__.py:31-87``) the way ``CudaBackend`` does
+(``src/gpusw/backends/cuda.py:29-112``), but every call goes through the C ABI in
+``include/gsw_hip.h``: uploads are ``gsw_malloc`` + copy, ``run_cross``/``run_pairs``
+call the batched wavefront kernels (``gsw_score_cross``/``gsw_score_pairs``), ``topk``
+is the device top-k (``gsw_topk_rows``).  ``dt`` is device time from HIP events.
+:class:`HipPlainBackend` routes the same calls to the one-thread-per-pair comparator.
+"""
+from __future__ import annotations
+
+import ctypes
+from contextlib import contextmanager
+
+import numpy as np
+
+from .. import _hip
+from . import Backend
+
+
+class DeviceScores:
+    """Native scores: a device int32 matrix ``(nq, nr)`` owned by Python."""
+
+    def __init__(self, buf: _hip.DeviceBuffer, shape):
+        self.buf = buf
+        self.shape = tuple(shape)
+
+    def to_host(self) -> np.ndarray:
+        return self.buf.to_host().reshape(self.shape)
+
+
+def _policy(scheme) -> int:
+    # 'int32' forced: 32-bit kernels.  Otherwise the library picks int16 per pair
+    # where the score range allows it (exact either way).
+    return _hip.DTYPE_INT32 if scheme.dtype == "int32" else _hip.DTYPE_AUTO
+
+
+def _describe(kind: str, scheme, stats: _hip.GswStats, dtype: str) -> str:
+    return (
+        f"// libgsw_hip.so ({_hip.load().gsw_version().decode()}) {kind}\n"
+        f"// MODE={scheme.mode} GOPEN={scheme.gap_open} GEXT={scheme.gap_extend} "
+        f"FREE_Q={int(scheme.free_end_gaps_query)} FREE_R={int(scheme.free_end_gaps_ref)} "
+        f"NS={scheme.ns} dtype={dtype}\n"
+        f"// launches={stats.launches} variant={stats.variant} rescored={stats.rescored} "
+        f"plain_pairs={stats.plain_pairs} kernel_ms={stats.kernel_ms:.3f}\n"
+    )
+
+
+class HipBackend(Backend):
+    """Batched wavefront kernels: packed-int16 lanes, int32 fallback, device top-k."""
+
+    name = "hip"
+
+    def available(self) -> bool:
+        return _hip.gpu_available()
+
+    @contextmanager
+    def device_scope(self, device: int):
+        prev = _hip.get_device()
+        if prev != device:
+            _hip.set_device(device)
+        try:
+            yield
+        finally:
+            if prev != device:
+                _hip.set_device(prev)
+
+    def upload(self, arr):
+        return _hip.DeviceBuffer.from_array(np.ascontiguousarray(arr))
+
+    # ------------------------------------------------------------------ scoring
+    def run_cross(self, scheme, maxq, dtype, threads, qbuf, qoff, rbuf, roff, qlist_host, nq, nr):
+        qset = _hip.seqset_for(qbuf, qoff)
+        rset = _hip.seqset_for(rbuf, roff)
+        qlist = np.ascontiguousarray(qlist_host, dtype=np.int32)
+        out = _hip.DeviceBuffer(4 * nq * nr, np.int32, (nq * nr,))
+        s, table = scheme_struct_cached(scheme)
+        stats = _hip.GswStats()
+        _hip.call("gsw_score_cross", ctypes.byref(s), _policy(scheme), qset.handle, _hip.ptr(qlist),
+                  int(nq), rset.handle, out.ptr, ctypes.byref(stats))
+        self.last_stats = stats
+        return DeviceScores(out, (nq, nr)), stats.kernel_ms / 1e3, _describe("cross", scheme, stats, dtype)
+
+    def run_pairs(self, scheme, maxq, dtype, threads, qbuf, qoff, rbuf, roff, qi_host, rj_host, n):
+        qset = _hip.seqset_for(qbuf, qoff)
+        rset = _hip.seqset_for(rbuf, roff)
+        qi = np.ascontiguousarray(qi_host, dtype=np.int32)
+        rj = np.ascontiguousarray(rj_host, dtype=np.int32)
+        _check_pair_indices(qi, rj, qset.n, rset.n)
+        out = _hip.DeviceBuffer(4 * n, np.int32, (n,))
+        s, table = scheme_struct_cached(scheme)
+        stats = _hip.GswStats()
+        _hip.call("gsw_score_pairs", ctypes.byref(s), _policy(scheme), qset.handle, rset.handle,
+                  _hip.ptr(qi), _hip.ptr(rj), int(n), out.ptr, ctypes.byref(stats))
+        self.last_stats = stats
+        return out.to_host(), stats.kernel_ms / 1e3, _describe("pairs", scheme, stats, dtype)
+
+    def as_host(self, scores_native):
+        return scores_native.to_host().astype(np.int32, copy=False)
+
+    def topk(self, scores_native, kk):
+        nq, nr = scores_native.shape
+        idx = np.empty((nq, kk), dtype=np.int32)
+        sc = np.empty((nq, kk), dtype=np.int32)
+        stats = _hip.GswStats()
+        _hip.call("gsw_topk_rows", scores_native.buf.ptr, int(nq), int(nr), int(kk), _hip.ptr(idx),
+                  _hip.ptr(sc), ctypes.byref(stats))
+        return idx, sc
+
+    def render(self, scheme, maxq, dtype) -> str:
+        return (
+            "// libgsw_hip.so batched wavefront kernels (gfx950): cross_kernel / pairs_kernel\n"
+            f"// MODE={scheme.mode} GOPEN={scheme.gap_open} GEXT={scheme.gap_extend} "
+            f"FREE_Q={int(scheme.free_end_gaps_query)} FREE_R={int(scheme.free_end_gaps_ref)} "
+            f"NS={scheme.ns} MAXQ={maxq} dtype={dtype}\n"
+        )
+
+
+class HipPlainBackend(HipBackend):
+    """The comparator: one thread per pair (reference kernel design), same library."""
+
+    name = "hip-plain"
+
+    def run_cross(self, scheme, maxq, dtype, threads, qbuf, qoff, rbuf, roff, qlist_host, nq, nr):
+        qlist = np.ascontiguousarray(qlist_host, dtype=np.int32)
+        out = _hip.DeviceBuffer(4 * nq * nr, np.int32, (nq * nr,))
+        s, table = scheme_struct_cached(scheme)
+        stats = _hip.GswStats()
+        store = _hip.DTYPE_INT16 if dtype == "int16" else _hip.DTYPE_INT32
+        _hip.call("gsw_score_cross_plain", ctypes.byref(s), store, int(threads), qbuf.ptr, qoff.ptr,
+                  rbuf.ptr, roff.ptr, _hip.ptr(qlist), int(nq), int(nr), max(int(maxq), 1), out.ptr,
+                  ctypes.byref(stats))
+        self.last_stats = stats
+        return DeviceScores(out, (nq, nr)), stats.kernel_ms / 1e3, _describe("cross-plain", scheme, stats, dtype)
+
+    def run_pairs(self, scheme, maxq, dtype, threads, qbuf, qoff, rbuf, roff, qi_host, rj_host, n):
+        qi = np.ascontiguousarray(qi_host, dtype=np.int32)
+        rj = np.ascontiguousarray(rj_host, dtype=np.int32)
+        _check_pair_indices(qi, rj, qoff.shape[0] - 1, roff.shape[0] - 1)
+        out = _hip.DeviceBuffer(4 * n, np.int32, (n,))
+        s, table = scheme_struct_cached(scheme)
+        stats = _hip.GswStats()
+        store = _hip.DTYPE_INT16 if dtype == "int16" else _hip.DTYPE_INT32
+        _hip.call("gsw_score_pairs_plain", ctypes.byref(s), store, int(threads), qbuf.ptr, qoff.ptr,
+                  rbuf.ptr, roff.ptr, _hip.ptr(qi), _hip.ptr(rj), int(n), max(int(maxq), 1), out.ptr,
+                  ctypes.byref(stats))
+        self.last_stats = stats
+        return out.to_host(), stats.kernel_ms / 1e3, _describe("pairs-plain", scheme, stats, dtype)
+
+
+def _check_pair_indices(qi, rj, nq, nr) -> None:
+    if qi.size and (qi.min() < 0 or qi.max() >= nq):
+        raise IndexError("query index out of range")
+    if rj.size and (rj.min() < 0 or rj.max() >= nr):
+        raise IndexError("reference index out of range")
+
+
+_SCHEME_CACHE: dict = {}
+
+
+def scheme_struct_cached(scheme):
+    """C scheme struct for ``scheme`` (cached by value; the table stays alive)."""
+    key = (scheme.mode, scheme.gap_open, scheme.gap_extend, bool(scheme.free_end_gaps_query),
+           bool(scheme.free_end_gaps_ref), scheme.table_hash())
+    hit = _SCHEME_CACHE.get(key)
+    if hit is None:
+        hit = _hip.scheme_struct(scheme)
+        _SCHEME_CACHE[key] = hit
+    return hit

@@ -1,0 +1,115 @@
+// Internal declarations shared by the libgsw_hip.so translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "gsw_hip.h"
+
+namespace gsw {
+
+// ---------------------------------------------------------------- errors
+void set_error(const char *fmt, ...);
+int fail(int code, const char *fmt, ...);
+
+#define GSW_HIP_TRY(expr)                                                              \
+    do {                                                                               \
+        hipError_t _e = (expr);                                                        \
+        if (_e != hipSuccess)                                                          \
+            return ::gsw::fail(GSW_ERR_HIP, "%s failed: %s (%s:%d)", #expr,           \
+                               hipGetErrorString(_e), __FILE__, __LINE__);             \
+    } while (0)
+
+#define GSW_TRY(expr)                                                                  \
+    do {                                                                               \
+        int _rc = (expr);                                                              \
+        if (_rc != GSW_OK) return _rc;                                                 \
+    } while (0)
+
+// ---------------------------------------------------------------- device context
+// One library stream and a set of grow-only scratch buffers per device.
+struct DeviceCtx {
+    int device = -1;
+    int num_cus = 0;
+    int max_smem = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // grow-only scratch buffers owned by the library
+    void *plain_ws = nullptr;  size_t plain_ws_bytes = 0;
+    void *meta = nullptr;      size_t meta_bytes = 0;      // task lists, lists of ids
+    void *meta2 = nullptr;     size_t meta2_bytes = 0;
+    int32_t *counters = nullptr;                            // 64 int32 counters
+    std::mutex mu;                                          // serialises calls on this device
+};
+
+int ctx_get(DeviceCtx **out);                    // context of the current device
+int ensure_buffer(void **buf, size_t *cap, size_t need);
+
+// Scheme validated and translated for the kernels.
+struct SchemeDev {
+    int mode, go, ge, free_q, free_r, ns;
+    int sub_min, sub_max;                       // table extremes
+    const int32_t *d_sub;                       // device copy (cached by content)
+    std::vector<int32_t> h_sub;
+};
+int scheme_prepare(const gsw_scheme *s, DeviceCtx *ctx, SchemeDev *out);
+
+// Static bound check used to choose int16 per pair (sound for every DP cell).
+bool fits_int16(const SchemeDev &s, int lq, int lr);
+bool fits_int32(const SchemeDev &s, int lq, int lr);
+
+// Closed-form score when one side is empty (src/gpusw/kernel.py:63-80).
+int32_t empty_edge_score(const SchemeDev &s, int lq, int lr);
+
+// ---------------------------------------------------------------- seqset
+struct SeqSet {
+    const uint8_t *d_codes = nullptr;
+    const int32_t *d_off = nullptr;
+    std::vector<int32_t> h_off;
+    std::vector<int32_t> len;
+    int32_t n = 0;
+    int32_t max_len = 0;
+    int64_t total = 0;
+    std::vector<int32_t> order_desc;   // indices sorted by length, longest first
+    int32_t *d_order_desc = nullptr;   // device copy of order_desc (non-empty only)
+    int32_t n_nonempty = 0;
+    int device = -1;
+};
+
+// ---------------------------------------------------------------- plain kernel
+// Score pairs with the one-thread-per-pair kernel.  Either a cross product of
+// qrows (query ids) x all nr refs, writing out[out_row[a]*ldo + b], or explicit
+// pairs writing out[out_idx[k]] (out_idx may be null for identity).
+struct PlainJob {
+    const uint8_t *qcodes; const int32_t *qoff;
+    const uint8_t *rcodes; const int32_t *roff;
+    int dtype;            // GSW_DTYPE_INT16 / INT32 (store type)
+    int threads;
+    int maxq;
+    // cross mode
+    const int32_t *h_qlist = nullptr; const int32_t *h_outrow = nullptr; int32_t nq = 0;
+    int32_t nr = 0; int64_t ldo = 0;
+    // pairs mode
+    const int32_t *h_qi = nullptr; const int32_t *h_rj = nullptr; const int64_t *h_outidx = nullptr;
+    int64_t npairs = 0;
+    int32_t *d_out;
+};
+int plain_run(DeviceCtx *ctx, const SchemeDev &s, const PlainJob &job, float *ms);
+
+// ---------------------------------------------------------------- batched kernels
+struct BatchResult { double kernel_ms = 0; int64_t rescored = 0; int64_t plain_pairs = 0;
+                     int launches = 0; int variant = -1; };
+
+int batch_cross(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqSet &qs,
+                const int32_t *h_qlist, int32_t nq, const SeqSet &rs, int32_t *d_out,
+                BatchResult *res);
+int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqSet &qs,
+                const SeqSet &rs, const int32_t *h_qi, const int32_t *h_rj, int64_t n,
+                int32_t *d_out, BatchResult *res);
+
+}  // namespace gsw

@@ -1,0 +1,265 @@
+// Wavefront-per-pair Gotoh kernels for gfx950 (device code, included by gsw_sweep.hip).
+//
+// Layout.  A "group" of G lanes scores one reference against one or two queries.  Lane t
+// of the group owns query columns [t*R, t*R+R) and keeps their DP state in VGPRs:
+//   hup[k] = H(i-1, j)  (row above)          fn[k] = F(i, j) (vertical gap for this row)
+// The reference streams down the rows: at step s lane t processes row i = s - t, so the
+// group forms a diagonal pipeline along the query.  After each step the H and E values
+// of every lane's last column move one lane to the right with a DPP wave_shr:1; lane 0
+// of a group receives the row's left-boundary values instead.  The diagonal input of a
+// lane is the H it received one step earlier.
+//
+// Element type V:
+//   short2 (packed): two independent alignments per lane, query A in the low half and
+//                    query B in the high half, both against the group's reference; the
+//                    recurrence runs on v_pk_{add,sub,max}_{i16,u16} with saturation.
+//   int:             one alignment per lane in 32-bit.
+//
+// Local mode uses the clamped form: E, F and H-gap are kept >= 0 with unsigned
+// saturating subtraction.  Since local H = max(diag+s, E, F, 0), clamping E and F at 0
+// leaves every H unchanged and removes the separate max(H, 0).  In int16, a result equal
+// to 32767 may have saturated and is rescored in int32 by the host.
+//
+// Query profile.  prof(c, col) = sub[c][query[col]] (packed A|B for short2), stored in
+// LDS as 16-byte chunks ordered [c][chunk][lane]: lane t's four values of chunk kc of
+// row c sit at ((c*KC + kc)*G + t)*16, so the lanes of a group read consecutive chunks.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace gsw {
+
+typedef short s2 __attribute__((ext_vector_type(2)));
+typedef unsigned short u2 __attribute__((ext_vector_type(2)));
+
+template <typename V> struct VOps;
+
+template <> struct VOps<s2> {
+    static constexpr int NEG = -32768;
+    static constexpr int TOP = 32767;
+    static constexpr bool PACKED = true;
+    __device__ static __forceinline__ s2 add(s2 a, s2 b) { return __builtin_elementwise_add_sat(a, b); }
+    __device__ static __forceinline__ s2 subu(s2 a, s2 b) {
+        return __builtin_bit_cast(s2, __builtin_elementwise_sub_sat(__builtin_bit_cast(u2, a),
+                                                                    __builtin_bit_cast(u2, b)));
+    }
+    __device__ static __forceinline__ s2 mx(s2 a, s2 b) { return __builtin_elementwise_max(a, b); }
+    __device__ static __forceinline__ short clamp16(int v) { return (short)min(max(v, -32768), 32767); }
+    __device__ static __forceinline__ s2 splat(int v) { short x = clamp16(v); return s2{x, x}; }
+    __device__ static __forceinline__ int bits(s2 v) { return __builtin_bit_cast(int, v); }
+    __device__ static __forceinline__ s2 from_bits(int b) { return __builtin_bit_cast(s2, b); }
+    __device__ static __forceinline__ int lo(s2 v) { return (int)v.x; }
+    __device__ static __forceinline__ int hi(s2 v) { return (int)v.y; }
+    __device__ static __forceinline__ uint32_t pack(int a, int b) {
+        return (uint32_t)(uint16_t)clamp16(a) | ((uint32_t)(uint16_t)clamp16(b) << 16);
+    }
+};
+
+template <> struct VOps<int> {
+    static constexpr int NEG = -(1 << 30);   // the oracle's sentinel
+    static constexpr int TOP = 0x7fffffff;
+    static constexpr bool PACKED = false;
+    __device__ static __forceinline__ int add(int a, int b) { return __builtin_elementwise_add_sat(a, b); }
+    __device__ static __forceinline__ int subu(int a, int b) {
+        return (int)__builtin_elementwise_sub_sat((unsigned)a, (unsigned)b);
+    }
+    __device__ static __forceinline__ int mx(int a, int b) { return max(a, b); }
+    __device__ static __forceinline__ int splat(int v) { return v; }
+    __device__ static __forceinline__ int bits(int v) { return v; }
+    __device__ static __forceinline__ int from_bits(int b) { return b; }
+    __device__ static __forceinline__ int lo(int v) { return v; }
+    __device__ static __forceinline__ int hi(int v) { return v; }
+    __device__ static __forceinline__ uint32_t pack(int a, int) { return (uint32_t)a; }
+};
+
+// DPP wave_shr:1 -- lane l receives lane l-1's value, lane 0 receives 0.
+__device__ __forceinline__ int wave_shr1(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, true);
+}
+
+__device__ __forceinline__ int wave_max_i32(int v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+template <int G>
+__device__ __forceinline__ int group_max_i32(int v) {
+#pragma unroll
+    for (int off = G / 2; off >= 1; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// Scalar scheme constants of one launch.
+struct SweepConst {
+    int mode;          // GSW_MODE_*
+    int go, ge;        // gap scores (<= 0)
+    int free_q, free_r;
+    int ns;            // substitution table side; codes >= ns are read as ns-1
+};
+
+// Row-0 value H(0, col) and column-0 value H(row, 0), 1-based (kernel.py:85-111).
+__device__ __forceinline__ int row0_value(const SweepConst &k, int col) {
+    if (col <= 0) return 0;
+    if (k.mode == 1 || (k.mode == 2 && !k.free_r)) return k.go + (col - 1) * k.ge;
+    return 0;
+}
+__device__ __forceinline__ int col0_value(const SweepConst &k, int row) {
+    if (row <= 0) return 0;
+    if (k.mode == 1 || (k.mode == 2 && !k.free_q)) return k.go + (row - 1) * k.ge;
+    return 0;
+}
+
+constexpr int RES_NONE = -0x40000000 - 7;   // below every reachable result
+
+// Per-group results after a sweep.
+struct SweepOut {
+    int a, b;   // scores of query A and query B (B unused for V=int)
+};
+
+// Sweep one group's reference through the group's query profile.
+//   rseq/lr    : this group's reference codes and length (lr = 0: idle group)
+//   nsteps     : wave-uniform step count (max lr over the wave's groups + G - 1)
+//   prof_lane  : this lane's profile base (group profile base + t*16)
+//   lqa/lqb    : query lengths (for padding masks and result positions)
+//   kstar      : wave-uniform (lq-1) % R of both queries, used when tracking the last column
+template <typename V, int G, int R, bool LOCAL>
+__device__ __forceinline__ SweepOut sweep_group(const uint8_t *__restrict__ rseq, int lr, int nsteps,
+                                                const char *prof_lane, int t, const SweepConst &k,
+                                                int lqa, int lqb, int kstar) {
+    using O = VOps<V>;
+    constexpr int KC = (R + 3) / 4;
+    constexpr int ROWB = KC * G * 16;   // bytes per profile row
+    const bool semi = k.mode == 2;
+    const bool track_endcol = semi && k.free_q;
+
+    V hup[R], fn[R];
+    const V GOV = O::splat(k.go), GEV = O::splat(k.ge);
+    const V GOP = O::splat(-k.go), GEP = O::splat(-k.ge);
+    const V ZERO = O::splat(0);
+#pragma unroll
+    for (int kk = 0; kk < R; ++kk) {
+        if (LOCAL) {
+            hup[kk] = ZERO;
+            fn[kk] = ZERO;
+        } else {
+            const int h0 = row0_value(k, t * R + kk + 1);
+            hup[kk] = O::splat(h0);
+            fn[kk] = O::splat(max(h0 + k.go, O::NEG + k.ge));   // F(1, col)
+        }
+    }
+    V hout = LOCAL ? ZERO : O::splat(row0_value(k, (t + 1) * R));
+    V hin = LOCAL ? ZERO : O::splat(t == 0 ? col0_value(k, 1) : row0_value(k, t * R));
+    V din = ZERO;
+    V ein = LOCAL ? ZERO : O::splat(O::NEG);
+    V eout = ein;
+    V best = ZERO;
+    V endcol = O::splat(O::NEG);
+    int corner_a = RES_NONE, corner_b = RES_NONE, endrow_a = RES_NONE, endrow_b = RES_NONE;
+
+    for (int s = 0; s < nsteps; ++s) {
+        const int row = s - t;
+        if ((unsigned)row < (unsigned)lr) {
+            const int c = min((int)rseq[row], k.ns - 1);   // out-of-range codes score as the sentinel
+            const char *pc = prof_lane + c * ROWB;
+            V pv[KC * 4];
+#pragma unroll
+            for (int kc = 0; kc < KC; ++kc) {
+                const int4 x = *reinterpret_cast<const int4 *>(pc + kc * G * 16);
+                pv[4 * kc + 0] = O::from_bits(x.x);
+                pv[4 * kc + 1] = O::from_bits(x.y);
+                pv[4 * kc + 2] = O::from_bits(x.z);
+                pv[4 * kc + 3] = O::from_bits(x.w);
+            }
+            V e = ein;
+            V hgo = LOCAL ? O::subu(hin, GOP) : O::add(hin, GOV);
+            V h = hin;
+            // xd = H(i-1, j-1) + s(i, j).  The sum for column kk+1 is taken before
+            // hup[kk] is overwritten, so H(i, j) can replace H(i-1, j) in place.
+            V xd = O::add(din, pv[0]);
+#pragma unroll
+            for (int kk = 0; kk < R; ++kk) {
+                const V xn = (kk + 1 < R) ? O::add(hup[kk], pv[kk + 1]) : xd;
+                if (LOCAL) {
+                    e = O::mx(O::subu(e, GEP), hgo);
+                    h = O::mx(O::mx(xd, e), fn[kk]);
+                    best = O::mx(best, h);
+                    hup[kk] = h;
+                    hgo = O::subu(h, GOP);
+                    fn[kk] = O::mx(O::subu(fn[kk], GEP), hgo);
+                } else {
+                    e = O::mx(O::add(e, GEV), hgo);
+                    h = O::mx(O::mx(xd, e), fn[kk]);
+                    hup[kk] = h;
+                    hgo = O::add(h, GOV);
+                    fn[kk] = O::mx(O::add(fn[kk], GEV), hgo);
+                }
+                xd = xn;
+            }
+            hout = h;
+            eout = e;
+            if (!LOCAL) {
+                if (track_endcol) {
+#pragma unroll
+                    for (int kk = 0; kk < R; ++kk)
+                        if (kk == kstar) endcol = O::mx(endcol, hup[kk]);
+                }
+                if (row == lr - 1) {
+                    // last row: corner and (free_r) best of the last row over real columns
+#pragma unroll
+                    for (int kk = 0; kk < R; ++kk) {
+                        const int col = t * R + kk;
+                        const int va = O::lo(hup[kk]);
+                        const int vb = O::hi(hup[kk]);
+                        if (col == lqa - 1) corner_a = va;
+                        if (col == lqb - 1) corner_b = vb;
+                        if (col < lqa) endrow_a = max(endrow_a, va);
+                        if (col < lqb) endrow_b = max(endrow_b, vb);
+                    }
+                }
+            }
+        }
+        din = hin;
+        hin = O::from_bits(wave_shr1(O::bits(hout)));
+        ein = O::from_bits(wave_shr1(O::bits(eout)));
+        if (t == 0) {
+            hin = LOCAL ? ZERO : O::splat(col0_value(k, s + 2));
+            ein = LOCAL ? ZERO : O::splat(O::NEG);
+        }
+    }
+
+    SweepOut out;
+    if (LOCAL) {
+        int ba = O::lo(best), bb = O::hi(best);
+        ba = group_max_i32<G>(ba);
+        bb = O::PACKED ? group_max_i32<G>(bb) : ba;
+        out.a = ba;
+        out.b = bb;
+        return out;
+    }
+    // non-local: each lane offers what it holds; the group maximum is the result
+    int ca = corner_a, cb = corner_b;
+    if (semi) {
+        if (track_endcol) {
+            if (t == (lqa - 1) / R) ca = max(ca, O::lo(endcol));
+            if (t == (lqb - 1) / R) cb = max(cb, O::hi(endcol));
+        }
+        if (k.free_r) {
+            ca = max(ca, endrow_a);
+            cb = max(cb, endrow_b);
+        }
+    }
+    ca = group_max_i32<G>(ca);
+    cb = O::PACKED ? group_max_i32<G>(cb) : ca;
+    if (semi && k.free_q && k.free_r) {
+        ca = max(ca, 0);
+        cb = max(cb, 0);
+    }
+    out.a = ca;
+    out.b = cb;
+    return out;
+}
+
+}  // namespace gsw

@@ -1,0 +1,816 @@
+// Batched scoring on gfx950: persistent wavefront kernels fed from atomic work queues,
+// plus their host-side scheduling (length binning, query pairing, int16/int32 routing,
+// saturation rescoring).  Replaces the reference's one-launch-per-batch path
+// (src/gpusw/backends/cuda.py:47-85, src/gpusw/aligner.py:95-167).
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <numeric>
+
+#include "gsw_internal.h"
+#include "gsw_sweep.h"
+
+namespace gsw {
+
+// Work item of the cross kernel: one query pair (or single query) against the
+// length-sorted references [rbeg, rend).
+struct CrossItem {
+    int32_t qa, qb;          // query ids (qb < 0: single query)
+    int32_t slota, slotb;    // output rows
+    int32_t rbeg, rend;      // range in the sorted reference order
+    int32_t kstar, pad;
+};
+
+// Task of the pairs kernel: queries qa and qb (qb < 0: single) against reference ref.
+struct PairTask {
+    int32_t qa, qb, ref, kstar;
+    int64_t oa, ob;          // output indices (ob < 0: none)
+};
+
+struct SatEntry {
+    int32_t q, r;
+    int64_t o;
+};
+
+struct KArgs {
+    const uint8_t *qcodes;
+    const int32_t *qoff;
+    const uint8_t *rcodes;
+    const int32_t *roff;
+    const int32_t *rorder;
+    const int32_t *sub;
+    int32_t ns;
+    SweepConst k;
+    int32_t *out;
+    int64_t ldo;
+    SatEntry *sat;
+    int32_t *sat_count;
+    int32_t sat_cap;
+    int32_t sat_check;
+    const void *items;
+    int32_t nitems;
+    int32_t *counter;
+};
+
+__device__ __forceinline__ void emit(const KArgs &a, int64_t o, int q, int r, int score) {
+    a.out[o] = score;
+    if (a.sat_check && score >= 32767) {
+        const int i = atomicAdd(a.sat_count, 1);
+        if (i < a.sat_cap) a.sat[i] = SatEntry{q, r, o};
+    }
+}
+
+// ---------------------------------------------------------------- cross kernel
+// A workgroup takes (query pair, reference range) items from a global counter, builds
+// the pair's packed profile in LDS once, then its waves take blocks of 64/G references
+// from an LDS counter; each group of G lanes sweeps one reference.
+template <typename V, int G, int R, bool LOCAL, int WAVES>
+__global__ void __launch_bounds__(WAVES * 64) cross_kernel(KArgs a) {
+    using O = VOps<V>;
+    constexpr int KC = (R + 3) / 4;
+    constexpr int LW = KC * 4;   // padded columns per lane
+    constexpr int GPW = 64 / G;
+    extern __shared__ __align__(16) char lds[];
+    __shared__ int s_item, s_blk;
+    const int lane = threadIdx.x & 63;
+    const int t = lane % G, g = lane / G;
+    const CrossItem *items = static_cast<const CrossItem *>(a.items);
+    for (;;) {
+        if (threadIdx.x == 0) {
+            s_item = atomicAdd(a.counter, 1);
+            s_blk = 0;
+        }
+        __syncthreads();
+        const int it = __builtin_amdgcn_readfirstlane(s_item);
+        if (it >= a.nitems) break;
+        const CrossItem item = items[it];
+        const int qa = item.qa, qb = item.qb >= 0 ? item.qb : item.qa;
+        const int qsa = a.qoff[qa], lqa = a.qoff[qa + 1] - qsa;
+        const int qsb = a.qoff[qb], lqb = a.qoff[qb + 1] - qsb;
+        const int total = a.ns * G * LW;
+        for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+            const int c = idx / (G * LW);
+            const int rem = idx - c * (G * LW);
+            const int lt = rem / LW, kk = rem - lt * LW;
+            const int col = lt * R + kk;
+            const int va = (kk < R && col < lqa) ? a.sub[c * a.ns + min((int)a.qcodes[qsa + col], a.ns - 1)] : O::NEG;
+            const int vb = (kk < R && col < lqb) ? a.sub[c * a.ns + min((int)a.qcodes[qsb + col], a.ns - 1)] : O::NEG;
+            *reinterpret_cast<uint32_t *>(lds + ((c * KC + (kk >> 2)) * G + lt) * 16 + (kk & 3) * 4) =
+                O::pack(va, vb);
+        }
+        __syncthreads();
+        const char *prof_lane = lds + t * 16;
+        for (;;) {
+            int blk = 0;
+            if (lane == 0) blk = atomicAdd(&s_blk, 1);
+            blk = __builtin_amdgcn_readfirstlane(blk);
+            const int r0 = item.rbeg + blk * GPW;
+            if (r0 >= item.rend) break;
+            const int rpos = r0 + g;
+            const int ref = rpos < item.rend ? a.rorder[rpos] : -1;
+            const int rs = ref >= 0 ? a.roff[ref] : 0;
+            const int lr = ref >= 0 ? a.roff[ref + 1] - rs : 0;
+            const int nsteps = __builtin_amdgcn_readfirstlane(wave_max_i32(lr)) + G - 1;
+            const SweepOut res = sweep_group<V, G, R, LOCAL>(a.rcodes + rs, lr, nsteps, prof_lane, t,
+                                                             a.k, lqa, lqb, item.kstar);
+            if (t == 0 && ref >= 0) {
+                emit(a, (int64_t)item.slota * a.ldo + ref, qa, ref, res.a);
+                if (item.slotb >= 0) emit(a, (int64_t)item.slotb * a.ldo + ref, qb, ref, res.b);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------- pairs kernel
+// Waves take blocks of 64/G tasks from a global counter.  Every lane writes the
+// profile entries of its own columns into the wave's LDS region and reads only those,
+// so no barrier is needed.  Tasks of one block share kstar (the host pads blocks).
+template <typename V, int G, int R, bool LOCAL, int WAVES>
+__global__ void __launch_bounds__(WAVES * 64) pairs_kernel(KArgs a) {
+    using O = VOps<V>;
+    constexpr int KC = (R + 3) / 4;
+    constexpr int GPW = 64 / G;
+    extern __shared__ __align__(16) char lds[];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int t = lane % G, g = lane / G;
+    const int group_bytes = a.ns * KC * G * 16;
+    char *glds = lds + (size_t)wave * GPW * group_bytes + (size_t)g * group_bytes;
+    const PairTask *tasks = static_cast<const PairTask *>(a.items);
+    const int nblocks = (a.nitems + GPW - 1) / GPW;
+    for (;;) {
+        int blk = 0;
+        if (lane == 0) blk = atomicAdd(a.counter, 1);
+        blk = __builtin_amdgcn_readfirstlane(blk);
+        if (blk >= nblocks) break;
+        const int ti = blk * GPW + g;
+        PairTask task{-1, -1, -1, 0, -1, -1};
+        if (ti < a.nitems) task = tasks[ti];
+        const int kstar = __builtin_amdgcn_readfirstlane(tasks[blk * GPW].kstar);
+        const bool valid = task.ref >= 0;
+        const int qa = valid ? task.qa : 0;
+        const int qb = (valid && task.qb >= 0) ? task.qb : qa;
+        const int qsa = valid ? a.qoff[qa] : 0, lqa = valid ? a.qoff[qa + 1] - qsa : 0;
+        const int qsb = valid ? a.qoff[qb] : 0, lqb = valid ? a.qoff[qb + 1] - qsb : 0;
+        const int rs = valid ? a.roff[task.ref] : 0;
+        const int lr = valid ? a.roff[task.ref + 1] - rs : 0;
+        // this lane's profile columns
+        for (int c = 0; c < a.ns; ++c) {
+            const int32_t *srow = a.sub + c * a.ns;
+#pragma unroll
+            for (int kc = 0; kc < KC; ++kc) {
+                uint32_t w[4];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int kk = kc * 4 + e;
+                    const int col = t * R + kk;
+                    const int va = (kk < R && col < lqa) ? srow[min((int)a.qcodes[qsa + col], a.ns - 1)] : O::NEG;
+                    const int vb = (kk < R && col < lqb) ? srow[min((int)a.qcodes[qsb + col], a.ns - 1)] : O::NEG;
+                    w[e] = O::pack(va, vb);
+                }
+                *reinterpret_cast<int4 *>(glds + ((c * KC + kc) * G + t) * 16) =
+                    int4{(int)w[0], (int)w[1], (int)w[2], (int)w[3]};
+            }
+        }
+        const int nsteps = __builtin_amdgcn_readfirstlane(wave_max_i32(lr)) + G - 1;
+        const SweepOut res = sweep_group<V, G, R, LOCAL>(a.rcodes + rs, lr, nsteps, glds + t * 16, t,
+                                                         a.k, lqa, lqb, kstar);
+        if (t == 0 && valid) {
+            emit(a, task.oa, qa, task.ref, res.a);
+            if (task.ob >= 0) emit(a, task.ob, qb, task.ref, res.b);
+        }
+    }
+}
+
+// ---------------------------------------------------------------- edge cells
+// Closed-form scores of pairs with an empty side, written to out[o[k]].
+__global__ void scatter_kernel(int32_t *out, const int64_t *idx, const int32_t *val, int64_t n) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) out[idx[k]] = val[k];
+}
+
+// ---------------------------------------------------------------- variants
+constexpr int CROSS_WAVES = 8;
+constexpr int PAIRS_WAVES = 4;
+
+typedef void (*LaunchFn)(const KArgs &, int grid, size_t lds, hipStream_t);
+
+struct Variant {
+    int id, G, R;
+    bool packed, local;
+    const void *cross_fn, *pairs_fn;
+    LaunchFn cross, pairs;
+};
+
+template <typename V, int G, int R, bool L>
+static void launch_cross(const KArgs &a, int grid, size_t lds, hipStream_t st) {
+    hipLaunchKernelGGL((cross_kernel<V, G, R, L, CROSS_WAVES>), dim3(grid), dim3(CROSS_WAVES * 64), lds, st, a);
+}
+template <typename V, int G, int R, bool L>
+static void launch_pairs(const KArgs &a, int grid, size_t lds, hipStream_t st) {
+    hipLaunchKernelGGL((pairs_kernel<V, G, R, L, PAIRS_WAVES>), dim3(grid), dim3(PAIRS_WAVES * 64), lds, st, a);
+}
+
+#define GSW_VARIANT(V, PK, G, R, L)                                                               \
+    Variant{0, G, R, PK, L, reinterpret_cast<const void *>(&cross_kernel<V, G, R, L, CROSS_WAVES>), \
+            reinterpret_cast<const void *>(&pairs_kernel<V, G, R, L, PAIRS_WAVES>),               \
+            &launch_cross<V, G, R, L>, &launch_pairs<V, G, R, L>}
+#define GSW_VARIANT4(G, R)                                                                        \
+    GSW_VARIANT(s2, true, G, R, true), GSW_VARIANT(s2, true, G, R, false),                        \
+        GSW_VARIANT(int, false, G, R, true), GSW_VARIANT(int, false, G, R, false)
+
+static std::vector<Variant> &variant_table() {
+    static std::vector<Variant> table = [] {
+        std::vector<Variant> v = {
+            GSW_VARIANT4(8, 8),   GSW_VARIANT4(8, 12),  GSW_VARIANT4(8, 16),  GSW_VARIANT4(8, 24),
+            GSW_VARIANT4(8, 32),  GSW_VARIANT4(8, 36),  GSW_VARIANT4(16, 24), GSW_VARIANT4(16, 32),
+            GSW_VARIANT4(16, 48), GSW_VARIANT4(32, 32), GSW_VARIANT4(32, 48), GSW_VARIANT4(64, 32),
+        };
+        for (size_t i = 0; i < v.size(); ++i) v[i].id = (int)i;
+        return v;
+    }();
+    return table;
+}
+
+// Forced variant (G, R) for tuning; 0 = automatic.
+static int g_force_g = 0, g_force_r = 0;
+
+// Estimated relative cost per cell of a (G, R) tile for a query of length lq against
+// references of mean length lr: padded columns, per-step overhead and pipeline fill.
+static double tile_cost(int G, int R, int lq, double lr) {
+    const double width = (double)G * R;
+    const double pad = width / std::max(lq, 1);
+    const double step = (9.0 * R + 14.0) / (9.0 * R);
+    const double fill = (lr + G - 1) / std::max(lr, 1.0);
+    double regs = 1.0;
+    if (R > 40) regs = 1.08;   // fewer waves per SIMD
+    return pad * step * fill * regs;
+}
+
+static size_t cross_lds_bytes(const Variant &v, int ns);
+static size_t pairs_lds_bytes(const Variant &v, int ns);
+
+// Cheapest compiled tile covering a query of length lq whose LDS profile fits
+// (cross: one profile per workgroup; pairs: one per lane group).
+static const Variant *pick_variant(int lq, double mean_lr, bool packed, bool local, int ns,
+                                   size_t lds_limit, bool for_pairs) {
+    const Variant *best = nullptr;
+    double best_cost = 1e30;
+    for (const Variant &v : variant_table()) {
+        if (v.packed != packed || v.local != local) continue;
+        if (v.G * v.R < lq) continue;
+        if ((for_pairs ? pairs_lds_bytes(v, ns) : cross_lds_bytes(v, ns)) > lds_limit) continue;
+        if (g_force_g && (v.G != g_force_g || v.R != g_force_r)) continue;
+        const double c = tile_cost(v.G, v.R, lq, mean_lr);
+        if (c < best_cost) {
+            best_cost = c;
+            best = &v;
+        }
+    }
+    return best;
+}
+
+static int max_width() {
+    int w = 0;
+    for (const Variant &v : variant_table()) w = std::max(w, v.G * v.R);
+    return w;
+}
+
+static size_t cross_lds_bytes(const Variant &v, int ns) {
+    return (size_t)ns * ((v.R + 3) / 4) * v.G * 16;
+}
+static size_t pairs_lds_bytes(const Variant &v, int ns) {
+    return (size_t)PAIRS_WAVES * ns * ((v.R + 3) / 4) * 64 * 16;
+}
+
+static int occupancy_grid(DeviceCtx *ctx, const void *fn, int threads, size_t lds, int *grid) {
+    int per_cu = 0;
+    GSW_HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    GSW_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, lds));
+    if (per_cu <= 0) return fail(GSW_ERR_LIMIT, "kernel does not fit on a CU (lds %zu bytes)", lds);
+    *grid = per_cu * ctx->num_cus;
+    return GSW_OK;
+}
+
+// ---------------------------------------------------------------- shared helpers
+struct SatBuffers {
+    SatEntry *d_sat = nullptr;
+    int32_t cap = 0;
+};
+
+static int prepare_sat(DeviceCtx *ctx, SatBuffers *sb, int32_t cap) {
+    static thread_local void *buf = nullptr;
+    static thread_local size_t bytes = 0;
+    static thread_local int dev = -1;
+    if (dev != ctx->device) {
+        buf = nullptr;
+        bytes = 0;
+        dev = ctx->device;
+    }
+    GSW_TRY(ensure_buffer(&buf, &bytes, sizeof(SatEntry) * (size_t)cap));
+    sb->d_sat = static_cast<SatEntry *>(buf);
+    sb->cap = cap;
+    GSW_HIP_TRY(hipMemsetAsync(ctx->counters + 1, 0, sizeof(int32_t), ctx->stream));
+    return GSW_OK;
+}
+
+// Rescore the listed saturated int16 results in int32 with the plain kernel.  Sets
+// *overflowed (and rescored nothing) when more pairs saturated than the list holds.
+static int rescore_saturated(DeviceCtx *ctx, const SchemeDev &s, const SatBuffers &sb,
+                             const SeqSet &qs, const SeqSet &rs, int32_t *d_out, BatchResult *res,
+                             bool *overflowed) {
+    int32_t count = 0;
+    GSW_HIP_TRY(hipMemcpyAsync(&count, ctx->counters + 1, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    GSW_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    *overflowed = count > sb.cap;
+    if (count == 0 || *overflowed) return GSW_OK;
+    std::vector<SatEntry> h(count);
+    GSW_HIP_TRY(hipMemcpy(h.data(), sb.d_sat, sizeof(SatEntry) * count, hipMemcpyDeviceToHost));
+    std::vector<int32_t> qi(count), rj(count);
+    std::vector<int64_t> oi(count);
+    int maxq = 1;
+    for (int i = 0; i < count; ++i) {
+        qi[i] = h[i].q;
+        rj[i] = h[i].r;
+        oi[i] = h[i].o;
+        maxq = std::max(maxq, qs.len[h[i].q]);
+    }
+    PlainJob job{};
+    job.qcodes = qs.d_codes; job.qoff = qs.d_off; job.rcodes = rs.d_codes; job.roff = rs.d_off;
+    job.dtype = GSW_DTYPE_INT32; job.threads = 256; job.maxq = maxq;
+    job.h_qi = qi.data(); job.h_rj = rj.data(); job.h_outidx = oi.data(); job.npairs = count;
+    job.d_out = d_out;
+    float pms = 0.f;
+    GSW_TRY(plain_run(ctx, s, job, &pms));
+    res->kernel_ms += pms;
+    res->rescored += count;
+    res->launches++;
+    return GSW_OK;
+}
+
+static int scatter_values(DeviceCtx *ctx, int32_t *d_out, const std::vector<int64_t> &idx,
+                          const std::vector<int32_t> &val) {
+    const int64_t n = (int64_t)idx.size();
+    if (n == 0) return GSW_OK;
+    GSW_TRY(ensure_buffer(&ctx->meta, &ctx->meta_bytes, n * 12 + 64));
+    int64_t *d_idx = static_cast<int64_t *>(ctx->meta);
+    int32_t *d_val = reinterpret_cast<int32_t *>(d_idx + n);
+    GSW_HIP_TRY(hipMemcpyAsync(d_idx, idx.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice, ctx->stream));
+    GSW_HIP_TRY(hipMemcpyAsync(d_val, val.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, ctx->stream));
+    const int threads = 256;
+    hipLaunchKernelGGL(scatter_kernel, dim3((unsigned)((n + threads - 1) / threads)), dim3(threads), 0,
+                       ctx->stream, d_out, d_idx, d_val, n);
+    GSW_HIP_TRY(hipGetLastError());
+    GSW_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return GSW_OK;
+}
+
+static KArgs base_args(const SchemeDev &s, const SeqSet &qs, const SeqSet &rs, int32_t *d_out) {
+    KArgs a{};
+    a.qcodes = qs.d_codes;
+    a.qoff = qs.d_off;
+    a.rcodes = rs.d_codes;
+    a.roff = rs.d_off;
+    a.rorder = rs.d_order_desc;
+    a.sub = s.d_sub;
+    a.ns = s.ns;
+    a.k = SweepConst{s.mode, s.go, s.ge, s.free_q, s.free_r, s.ns};
+    a.out = d_out;
+    return a;
+}
+
+// ---------------------------------------------------------------- cross
+int batch_cross(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqSet &qs,
+                const int32_t *h_qlist, int32_t nq, const SeqSet &rs, int32_t *d_out,
+                BatchResult *res) {
+    const int32_t nr = rs.n;
+    if (nq == 0 || nr == 0) return GSW_OK;
+    const bool local = s.mode == GSW_MODE_LOCAL;
+    const int wmax = max_width();
+    double mean_lr = rs.n_nonempty ? (double)rs.total / rs.n_nonempty : 1.0;
+
+    // -- empty rows/columns: closed forms
+    std::vector<int64_t> e_idx;
+    std::vector<int32_t> e_val;
+    for (int a = 0; a < nq; ++a) {
+        const int q = h_qlist[a];
+        if (q < 0 || q >= qs.n) return fail(GSW_ERR_INVALID, "query index %d out of range", q);
+        const int lq = qs.len[q];
+        if (lq == 0) {
+            for (int b = 0; b < nr; ++b) {
+                e_idx.push_back((int64_t)a * nr + b);
+                e_val.push_back(empty_edge_score(s, 0, rs.len[b]));
+            }
+        } else {
+            for (int p = rs.n_nonempty; p < nr; ++p) {
+                const int b = rs.order_desc[p];
+                e_idx.push_back((int64_t)a * nr + b);
+                e_val.push_back(empty_edge_score(s, lq, 0));
+            }
+        }
+    }
+
+    // -- route each query: batched (int16 packed / int32) or plain
+    struct QInfo { int slot, q, lq; const Variant *v; int kstar; };
+    std::vector<QInfo> packed_q, single_q;
+    std::vector<int32_t> plain_rows, plain_q;
+    for (int a = 0; a < nq; ++a) {
+        const int q = h_qlist[a];
+        const int lq = qs.len[q];
+        if (lq == 0 || rs.n_nonempty == 0) continue;
+        if (!fits_int32(s, lq, rs.max_len))
+            return fail(GSW_ERR_OVERFLOW, "score range of query %d exceeds the int32 kernels", q);
+        bool use16 = dtype_policy != GSW_DTYPE_INT32 && fits_int16(s, lq, rs.max_len);
+        if (lq > wmax) {
+            plain_rows.push_back(a);
+            plain_q.push_back(q);
+            continue;
+        }
+        const Variant *v = pick_variant(lq, mean_lr, use16, local, s.ns, (size_t)ctx->max_smem, false);
+        if (!v) {
+            plain_rows.push_back(a);
+            plain_q.push_back(q);
+            continue;
+        }
+        const int kstar = (lq - 1) % v->R;
+        (use16 ? packed_q : single_q).push_back(QInfo{a, q, lq, v, kstar});
+    }
+
+    // -- pair packed queries of the same variant (and same kstar when the last column is tracked)
+    const bool need_kstar = s.mode == GSW_MODE_SEMIGLOBAL && s.free_q;
+    auto key = [&](const QInfo &x) { return std::make_tuple(x.v->id, need_kstar ? x.kstar : 0, x.lq); };
+    std::stable_sort(packed_q.begin(), packed_q.end(), [&](const QInfo &x, const QInfo &y) { return key(x) < key(y); });
+    std::stable_sort(single_q.begin(), single_q.end(), [&](const QInfo &x, const QInfo &y) { return key(x) < key(y); });
+    struct Unit { const Variant *v; int qa, qb, sa, sb, kstar; };
+    std::vector<Unit> units;
+    for (size_t i = 0; i < packed_q.size();) {
+        const QInfo &x = packed_q[i];
+        if (i + 1 < packed_q.size() && packed_q[i + 1].v == x.v &&
+            (!need_kstar || packed_q[i + 1].kstar == x.kstar)) {
+            const QInfo &y = packed_q[i + 1];
+            units.push_back(Unit{x.v, x.q, y.q, x.slot, y.slot, x.kstar});
+            i += 2;
+        } else {
+            units.push_back(Unit{x.v, x.q, -1, x.slot, -1, x.kstar});
+            i += 1;
+        }
+    }
+    for (const QInfo &x : single_q) units.push_back(Unit{x.v, x.q, -1, x.slot, -1, x.kstar});
+
+    // -- launch one persistent kernel per variant
+    SatBuffers sb;
+    const int32_t sat_cap = 1 << 20;
+    GSW_TRY(prepare_sat(ctx, &sb, sat_cap));
+    GSW_HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+    std::vector<CrossItem> items;
+    size_t u0 = 0;
+    int dominant = -1;
+    int64_t dominant_units = 0;
+    while (u0 < units.size()) {
+        size_t u1 = u0;
+        while (u1 < units.size() && units[u1].v == units[u0].v) ++u1;
+        const Variant &v = *units[u0].v;
+        if ((int64_t)(u1 - u0) > dominant_units) {
+            dominant_units = (int64_t)(u1 - u0);
+            dominant = v.id;
+        }
+        const size_t lds = cross_lds_bytes(v, s.ns);
+        int grid = 0;
+        GSW_TRY(occupancy_grid(ctx, v.cross_fn, CROSS_WAVES * 64, lds, &grid));
+        const int gpw = 64 / v.G;
+        const int64_t nref = rs.n_nonempty;
+        const int64_t units_here = (int64_t)(u1 - u0);
+        // chunk the reference range so that there are >= 4 items per workgroup
+        int64_t chunk = (nref * units_here + 4LL * grid - 1) / (4LL * grid);
+        chunk = std::max<int64_t>(chunk, (int64_t)gpw * CROSS_WAVES);
+        chunk = std::min<int64_t>(chunk, nref);
+        chunk = (chunk + gpw - 1) / gpw * gpw;
+        items.clear();
+        for (size_t u = u0; u < u1; ++u) {
+            for (int64_t rb = 0; rb < nref; rb += chunk) {
+                items.push_back(CrossItem{units[u].qa, units[u].qb, units[u].sa, units[u].sb, (int32_t)rb,
+                                          (int32_t)std::min(nref, rb + chunk), units[u].kstar, 0});
+            }
+        }
+        GSW_TRY(ensure_buffer(&ctx->meta2, &ctx->meta2_bytes, sizeof(CrossItem) * items.size()));
+        GSW_HIP_TRY(hipMemcpyAsync(ctx->meta2, items.data(), sizeof(CrossItem) * items.size(),
+                                   hipMemcpyHostToDevice, ctx->stream));
+        GSW_HIP_TRY(hipMemsetAsync(ctx->counters, 0, sizeof(int32_t), ctx->stream));
+        KArgs a = base_args(s, qs, rs, d_out);
+        a.ldo = nr;
+        a.sat = sb.d_sat;
+        a.sat_count = ctx->counters + 1;
+        a.sat_cap = sb.cap;
+        a.sat_check = (v.packed && local) ? 1 : 0;
+        a.items = ctx->meta2;
+        a.nitems = (int32_t)items.size();
+        a.counter = ctx->counters;
+        v.cross(a, std::min<int>(grid, (int)items.size()), lds, ctx->stream);
+        GSW_HIP_TRY(hipGetLastError());
+        res->launches++;
+        // the item list is reused by the next variant: wait before overwriting it
+        GSW_HIP_TRY(hipStreamSynchronize(ctx->stream));
+        u0 = u1;
+    }
+    GSW_HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
+    GSW_HIP_TRY(hipEventSynchronize(ctx->ev1));
+    float ms = 0.f;
+    GSW_HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    res->kernel_ms += ms;
+    res->variant = dominant;
+
+    // -- long queries: plain kernel
+    if (!plain_q.empty()) {
+        int maxq = 1;
+        for (int q : plain_q) maxq = std::max(maxq, qs.len[q]);
+        bool use16 = dtype_policy != GSW_DTYPE_INT32;
+        for (int q : plain_q) use16 = use16 && fits_int16(s, qs.len[q], rs.max_len) && !local;
+        PlainJob job{};
+        job.qcodes = qs.d_codes; job.qoff = qs.d_off; job.rcodes = rs.d_codes; job.roff = rs.d_off;
+        job.dtype = use16 ? GSW_DTYPE_INT16 : GSW_DTYPE_INT32;
+        job.threads = 256; job.maxq = maxq;
+        job.h_qlist = plain_q.data(); job.h_outrow = plain_rows.data(); job.nq = (int32_t)plain_q.size();
+        job.nr = nr; job.ldo = nr; job.d_out = d_out;
+        float pms = 0.f;
+        GSW_TRY(plain_run(ctx, s, job, &pms));
+        res->kernel_ms += pms;
+        res->plain_pairs += (int64_t)plain_q.size() * nr;
+        res->launches++;
+    }
+
+    // -- edges and saturation rescoring
+    GSW_TRY(scatter_values(ctx, d_out, e_idx, e_val));
+    bool overflowed = false;
+    GSW_TRY(rescore_saturated(ctx, s, sb, qs, rs, d_out, res, &overflowed));
+    if (overflowed) {
+        // too many saturated pairs to list: rescore every batched int16 row in int32
+        std::vector<int32_t> rows, ql;
+        int maxq = 1;
+        for (const Unit &u : units) {
+            if (!u.v->packed) continue;
+            rows.push_back(u.sa); ql.push_back(u.qa); maxq = std::max(maxq, qs.len[u.qa]);
+            if (u.qb >= 0) { rows.push_back(u.sb); ql.push_back(u.qb); maxq = std::max(maxq, qs.len[u.qb]); }
+        }
+        PlainJob job{};
+        job.qcodes = qs.d_codes; job.qoff = qs.d_off; job.rcodes = rs.d_codes; job.roff = rs.d_off;
+        job.dtype = GSW_DTYPE_INT32; job.threads = 256; job.maxq = maxq;
+        job.h_qlist = ql.data(); job.h_outrow = rows.data(); job.nq = (int32_t)ql.size();
+        job.nr = nr; job.ldo = nr; job.d_out = d_out;
+        float pms = 0.f;
+        GSW_TRY(plain_run(ctx, s, job, &pms));
+        res->kernel_ms += pms;
+        res->rescored += (int64_t)ql.size() * nr;
+        res->launches++;
+        // empty reference columns of those rows were just overwritten: restore them
+        GSW_TRY(scatter_values(ctx, d_out, e_idx, e_val));
+    }
+    return GSW_OK;
+}
+
+// ---------------------------------------------------------------- pairs
+int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqSet &qs,
+                const SeqSet &rs, const int32_t *h_qi, const int32_t *h_rj, int64_t n,
+                int32_t *d_out, BatchResult *res) {
+    if (n == 0) return GSW_OK;
+    const bool local = s.mode == GSW_MODE_LOCAL;
+    const bool need_kstar = s.mode == GSW_MODE_SEMIGLOBAL && s.free_q;
+    const int wmax = max_width();
+    const double mean_lr = rs.n_nonempty ? (double)rs.total / rs.n_nonempty : 1.0;
+
+    std::vector<int64_t> e_idx;
+    std::vector<int32_t> e_val;
+    std::vector<int32_t> p_qi, p_rj;
+    std::vector<int64_t> p_oi;
+    // key: variant id (8 bits) | kstar (8) | 65535-lr (16) | ref (32)
+    std::vector<std::pair<uint64_t, int64_t>> keyed;
+    keyed.reserve(n);
+    std::vector<const Variant *> vcache(2 * (wmax + 1), nullptr);
+    for (int64_t k = 0; k < n; ++k) {
+        const int q = h_qi[k], r = h_rj[k];
+        if (q < 0 || q >= qs.n || r < 0 || r >= rs.n)
+            return fail(GSW_ERR_INVALID, "pair %lld index out of range", (long long)k);
+        const int lq = qs.len[q], lr = rs.len[r];
+        if (lq == 0 || lr == 0) {
+            e_idx.push_back(k);
+            e_val.push_back(empty_edge_score(s, lq, lr));
+            continue;
+        }
+        if (!fits_int32(s, lq, lr))
+            return fail(GSW_ERR_OVERFLOW, "score range of pair %lld exceeds the int32 kernels", (long long)k);
+        const bool use16 = dtype_policy != GSW_DTYPE_INT32 && fits_int16(s, lq, lr);
+        const Variant *v = nullptr;
+        if (lq <= wmax) {
+            const Variant *&slot = vcache[(use16 ? 1 : 0) * (wmax + 1) + lq];
+            if (!slot) slot = pick_variant(lq, mean_lr, use16, local, s.ns, (size_t)ctx->max_smem, true);
+            v = slot;
+        }
+        if (!v) {
+            p_qi.push_back(q);
+            p_rj.push_back(r);
+            p_oi.push_back(k);
+            continue;
+        }
+        const int kstar = need_kstar ? (lq - 1) % v->R : 0;
+        const uint64_t key = ((uint64_t)v->id << 56) | ((uint64_t)kstar << 48) |
+                             ((uint64_t)(65535 - std::min(lr, 65535)) << 32) | (uint64_t)(uint32_t)r;
+        keyed.emplace_back(key, k);
+    }
+    std::sort(keyed.begin(), keyed.end());
+
+    SatBuffers sb;
+    GSW_TRY(prepare_sat(ctx, &sb, 1 << 20));
+    GSW_HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+    std::vector<PairTask> tasks;
+    size_t i0 = 0;
+    int dominant = -1;
+    size_t dominant_n = 0;
+    while (i0 < keyed.size()) {
+        const int vid = (int)(keyed[i0].first >> 56);
+        size_t i1 = i0;
+        while (i1 < keyed.size() && (int)(keyed[i1].first >> 56) == vid) ++i1;
+        const Variant &v = variant_table()[vid];
+        if (i1 - i0 > dominant_n) { dominant_n = i1 - i0; dominant = vid; }
+        const int gpw = 64 / v.G;
+        tasks.clear();
+        // build tasks: pack two pairs that share a reference (packed variants)
+        size_t i = i0;
+        int cur_kstar = -1;
+        while (i < i1) {
+            const uint64_t key = keyed[i].first;
+            const int kstar = (int)((key >> 48) & 0xff);
+            if (kstar != cur_kstar) {
+                while (tasks.size() % gpw) tasks.push_back(PairTask{-1, -1, -1, cur_kstar, -1, -1});
+                cur_kstar = kstar;
+            }
+            const int64_t ka = keyed[i].second;
+            if (v.packed && i + 1 < i1 && keyed[i + 1].first == key) {
+                const int64_t kb = keyed[i + 1].second;
+                tasks.push_back(PairTask{h_qi[ka], h_qi[kb], h_rj[ka], kstar, ka, kb});
+                i += 2;
+            } else {
+                tasks.push_back(PairTask{h_qi[ka], -1, h_rj[ka], kstar, ka, -1});
+                i += 1;
+            }
+        }
+        while (tasks.size() % gpw) tasks.push_back(PairTask{-1, -1, -1, cur_kstar, -1, -1});
+        const size_t lds = pairs_lds_bytes(v, s.ns);
+        int grid = 0;
+        GSW_TRY(occupancy_grid(ctx, v.pairs_fn, PAIRS_WAVES * 64, lds, &grid));
+        GSW_TRY(ensure_buffer(&ctx->meta2, &ctx->meta2_bytes, sizeof(PairTask) * tasks.size()));
+        GSW_HIP_TRY(hipMemcpyAsync(ctx->meta2, tasks.data(), sizeof(PairTask) * tasks.size(),
+                                   hipMemcpyHostToDevice, ctx->stream));
+        GSW_HIP_TRY(hipMemsetAsync(ctx->counters, 0, sizeof(int32_t), ctx->stream));
+        KArgs a = base_args(s, qs, rs, d_out);
+        a.ldo = 0;
+        a.sat = sb.d_sat;
+        a.sat_count = ctx->counters + 1;
+        a.sat_cap = sb.cap;
+        a.sat_check = (v.packed && local) ? 1 : 0;
+        a.items = ctx->meta2;
+        a.nitems = (int32_t)tasks.size();
+        a.counter = ctx->counters;
+        const int64_t blocks = (int64_t)tasks.size() / gpw;
+        const int wg = (int)std::min<int64_t>(grid, (blocks + PAIRS_WAVES - 1) / PAIRS_WAVES);
+        v.pairs(a, std::max(wg, 1), lds, ctx->stream);
+        GSW_HIP_TRY(hipGetLastError());
+        res->launches++;
+        GSW_HIP_TRY(hipStreamSynchronize(ctx->stream));
+        i0 = i1;
+    }
+    GSW_HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
+    GSW_HIP_TRY(hipEventSynchronize(ctx->ev1));
+    float ms = 0.f;
+    GSW_HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    res->kernel_ms += ms;
+    res->variant = dominant;
+
+    if (!p_qi.empty()) {
+        int maxq = 1;
+        bool use16 = dtype_policy != GSW_DTYPE_INT32 && !local;
+        for (size_t i = 0; i < p_qi.size(); ++i) {
+            maxq = std::max(maxq, qs.len[p_qi[i]]);
+            use16 = use16 && fits_int16(s, qs.len[p_qi[i]], rs.len[p_rj[i]]);
+        }
+        PlainJob job{};
+        job.qcodes = qs.d_codes; job.qoff = qs.d_off; job.rcodes = rs.d_codes; job.roff = rs.d_off;
+        job.dtype = use16 ? GSW_DTYPE_INT16 : GSW_DTYPE_INT32;
+        job.threads = 256; job.maxq = maxq;
+        job.h_qi = p_qi.data(); job.h_rj = p_rj.data(); job.h_outidx = p_oi.data();
+        job.npairs = (int64_t)p_qi.size(); job.d_out = d_out;
+        float pms = 0.f;
+        GSW_TRY(plain_run(ctx, s, job, &pms));
+        res->kernel_ms += pms;
+        res->plain_pairs += (int64_t)p_qi.size();
+        res->launches++;
+    }
+    GSW_TRY(scatter_values(ctx, d_out, e_idx, e_val));
+
+    bool overflowed = false;
+    GSW_TRY(rescore_saturated(ctx, s, sb, qs, rs, d_out, res, &overflowed));
+    if (overflowed)
+        return fail(GSW_ERR_LIMIT, "more than %d saturated int16 pairs; use dtype='int32'", sb.cap);
+    return GSW_OK;
+}
+
+}  // namespace gsw
+
+using namespace gsw;
+
+static int batch_common(const gsw_scheme *s, int32_t dtype_policy, DeviceCtx **ctx, SchemeDev *sd) {
+    if (dtype_policy < GSW_DTYPE_AUTO || dtype_policy > GSW_DTYPE_INT32)
+        return fail(GSW_ERR_INVALID, "dtype_policy %d out of range", dtype_policy);
+    GSW_TRY(ctx_get(ctx));
+    GSW_TRY(scheme_prepare(s, *ctx, sd));
+    return GSW_OK;
+}
+
+static void fill_stats(gsw_stats *st, const BatchResult &r, int64_t cells, int64_t pairs,
+                       std::chrono::steady_clock::time_point t0) {
+    if (!st) return;
+    *st = gsw_stats{};
+    st->kernel_ms = r.kernel_ms;
+    st->cells = cells;
+    st->pairs = pairs;
+    st->rescored = r.rescored;
+    st->plain_pairs = r.plain_pairs;
+    st->launches = r.launches;
+    st->variant = r.variant;
+    st->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+extern "C" int gsw_score_cross(const gsw_scheme *s, int32_t dtype_policy, gsw_seqset queries,
+                               const int32_t *h_qlist, int32_t nq, gsw_seqset refs, int32_t *d_out,
+                               gsw_stats *stats) {
+    auto t0 = std::chrono::steady_clock::now();
+    const SeqSet *qs = reinterpret_cast<const SeqSet *>(queries);
+    const SeqSet *rs = reinterpret_cast<const SeqSet *>(refs);
+    if (!qs || !rs) return fail(GSW_ERR_INVALID, "null sequence set");
+    if (nq < 0) return fail(GSW_ERR_INVALID, "negative query count");
+    DeviceCtx *ctx = nullptr;
+    SchemeDev sd;
+    GSW_TRY(batch_common(s, dtype_policy, &ctx, &sd));
+    if (qs->device != ctx->device || rs->device != ctx->device)
+        return fail(GSW_ERR_INVALID, "sequence set belongs to another device");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    BatchResult r;
+    GSW_TRY(batch_cross(ctx, sd, dtype_policy, *qs, h_qlist, nq, *rs, d_out, &r));
+    int64_t qsum = 0;
+    for (int a = 0; a < nq; ++a) qsum += qs->len[h_qlist[a]];
+    fill_stats(stats, r, qsum * rs->total, (int64_t)nq * rs->n, t0);
+    return GSW_OK;
+}
+
+extern "C" int gsw_score_pairs(const gsw_scheme *s, int32_t dtype_policy, gsw_seqset queries,
+                               gsw_seqset refs, const int32_t *h_qi, const int32_t *h_rj, int64_t n,
+                               int32_t *d_out, gsw_stats *stats) {
+    auto t0 = std::chrono::steady_clock::now();
+    const SeqSet *qs = reinterpret_cast<const SeqSet *>(queries);
+    const SeqSet *rs = reinterpret_cast<const SeqSet *>(refs);
+    if (!qs || !rs) return fail(GSW_ERR_INVALID, "null sequence set");
+    if (n < 0) return fail(GSW_ERR_INVALID, "negative pair count");
+    DeviceCtx *ctx = nullptr;
+    SchemeDev sd;
+    GSW_TRY(batch_common(s, dtype_policy, &ctx, &sd));
+    if (qs->device != ctx->device || rs->device != ctx->device)
+        return fail(GSW_ERR_INVALID, "sequence set belongs to another device");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    BatchResult r;
+    GSW_TRY(batch_pairs(ctx, sd, dtype_policy, *qs, *rs, h_qi, h_rj, n, d_out, &r));
+    int64_t cells = 0;
+    for (int64_t k = 0; k < n; ++k) cells += (int64_t)qs->len[h_qi[k]] * rs->len[h_rj[k]];
+    fill_stats(stats, r, cells, n, t0);
+    return GSW_OK;
+}
+
+// Tuning hook: force the (G, R) tile of the batched kernels (0, 0 = automatic).
+extern "C" int gsw_force_variant(int32_t G, int32_t R) {
+    if (G == 0 && R == 0) {
+        g_force_g = g_force_r = 0;
+        return GSW_OK;
+    }
+    for (const Variant &v : variant_table())
+        if (v.G == G && v.R == R) {
+            g_force_g = G;
+            g_force_r = R;
+            return GSW_OK;
+        }
+    return fail(GSW_ERR_INVALID, "no compiled variant G=%d R=%d", G, R);
+}
+
+// List compiled (G, R) tiles: writes up to cap (G, R) pairs, returns the count in *n.
+extern "C" int gsw_list_variants(int32_t *gr, int32_t cap, int32_t *n) {
+    int cnt = 0;
+    int last_g = -1, last_r = -1;
+    for (const Variant &v : variant_table()) {
+        if (v.G == last_g && v.R == last_r) continue;
+        last_g = v.G;
+        last_r = v.R;
+        if (cnt < cap) {
+            gr[2 * cnt] = v.G;
+            gr[2 * cnt + 1] = v.R;
+        }
+        ++cnt;
+    }
+    *n = cnt;
+    return GSW_OK;
+}

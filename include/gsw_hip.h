@@ -1,0 +1,139 @@
+/*
+ * gsw_hip.h -- C ABI of libgsw_hip.so, the MI355X (gfx950) affine-gap (Gotoh)
+ * alignment-scoring library.
+ *
+ * Plain C: no torch or HIP types cross this boundary.  Device buffers are opaque
+ * `void*` obtained from gsw_malloc; host buffers are plain pointers.  Every function
+ * returns GSW_OK (0) or an error code; gsw_last_error() gives a message for the
+ * calling thread.  Every scoring call is synchronous: when it returns, its outputs
+ * are complete.
+ *
+ * Reference interfaces replaced (freiburgermsu/gpuSW, gpusw 0.2.0):
+ *   - Backend contract      src/gpusw/backends/__init__.py:31-87
+ *   - CudaBackend           src/gpusw/backends/cuda.py:29-106
+ *   - module/SUB cache      src/gpusw/_compile.py:39-93
+ *   - kernels               src/gpusw/kernel.py:56-190 (sw_score, sw_cross, sw_pairs)
+ */
+#ifndef GSW_HIP_H
+#define GSW_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------- */
+#define GSW_OK            0
+#define GSW_ERR_NO_DEVICE 1  /* no AMD GPU / HIP runtime: NoGpuError            */
+#define GSW_ERR_INVALID   2  /* bad argument: ValueError / GpuSWError           */
+#define GSW_ERR_HIP       3  /* HIP runtime failure: GpuSWError                 */
+#define GSW_ERR_LIMIT     4  /* size limit exceeded: GpuSWError                 */
+#define GSW_ERR_OVERFLOW  5  /* score range exceeds int32: OverflowRiskError    */
+#define GSW_ERR_BUSY      6  /* server admission refused (memory budget)        */
+
+/* ---- scheme ------------------------------------------------------------- */
+#define GSW_MODE_LOCAL      0
+#define GSW_MODE_GLOBAL     1
+#define GSW_MODE_SEMIGLOBAL 2
+
+#define GSW_DTYPE_AUTO  0   /* int16 fast path, int32 where the range needs it   */
+#define GSW_DTYPE_INT16 1   /* caller checked the envelope (Scheme.resolve_dtype) */
+#define GSW_DTYPE_INT32 2   /* 32-bit DP everywhere                               */
+
+/* Scoring scheme passed by value.  Replaces the compile-time #defines of
+ * src/gpusw/kernel.py:44-54 (MODE, GOPEN, GEXT, FREE_Q, FREE_R, NS) and the SUB
+ * upload of src/gpusw/_compile.py:83-93.  sub[a*ns+b] scores reference code a
+ * against query code b (src/gpusw/scheme.py:149-162). */
+typedef struct gsw_scheme {
+    int32_t mode;        /* GSW_MODE_*                                  */
+    int32_t gap_open;    /* score of a gap's first residue, <= 0        */
+    int32_t gap_extend;  /* score of each further gap residue, <= 0     */
+    int32_t free_q;      /* semiglobal: query end gaps free             */
+    int32_t free_r;      /* semiglobal: reference end gaps free         */
+    int32_t ns;          /* side of the sentinel-padded table (<= 64)   */
+    const int32_t *sub;  /* host pointer, ns*ns int32                   */
+} gsw_scheme;
+
+/* Per-call measurements (kernel time from HIP events on the library stream). */
+typedef struct gsw_stats {
+    double  kernel_ms;    /* summed device time of the scoring kernels          */
+    double  wall_ms;      /* host wall time of the whole call                    */
+    int64_t cells;        /* sum over scored pairs of len(query) * len(ref)      */
+    int64_t pairs;        /* pairs scored                                        */
+    int64_t rescored;     /* int16 pairs that saturated and were rescored in int32 */
+    int64_t plain_pairs;  /* pairs routed to the one-thread-per-pair kernel      */
+    int32_t launches;     /* kernel launches issued                              */
+    int32_t variant;      /* dominant batched variant id (-1: none)              */
+} gsw_stats;
+
+/* ---- runtime (src/gpusw/_compile.py:30-46; backends/cuda.py:29-41) ----- */
+const char *gsw_last_error(void);
+const char *gsw_version(void);
+int gsw_device_count(int32_t *count);
+int gsw_set_device(int32_t device);            /* Backend.device_scope            */
+int gsw_get_device(int32_t *device);
+int gsw_device_name(int32_t device, char *buf, int32_t buflen);
+int gsw_malloc(void **dptr, size_t bytes);     /* Backend.upload (allocation)     */
+int gsw_free(void *dptr);
+int gsw_memcpy_h2d(void *dst, const void *src, size_t bytes);   /* Backend.upload   */
+int gsw_memcpy_d2h(void *dst, const void *src, size_t bytes);   /* Backend.as_host  */
+int gsw_memset(void *dptr, int32_t value, size_t bytes);
+int gsw_synchronize(void);
+
+/* ---- plain comparator: one thread per pair ------------------------------
+ * Replaces CudaBackend.run_cross / run_pairs (src/gpusw/backends/cuda.py:47-85)
+ * launching sw_cross / sw_pairs (src/gpusw/kernel.py:160-190).  dtype is the
+ * resolved store type (GSW_DTYPE_INT16 or GSW_DTYPE_INT32); threads is the block
+ * size.  Scores are written as int32:
+ *   cross: d_out[a*nr + b] = score(query h_qlist[a], reference b), a < nq, b < nr
+ *   pairs: d_out[k]        = score(query h_qi[k], reference h_rj[k])            */
+int gsw_score_cross_plain(const gsw_scheme *s, int32_t dtype, int32_t threads,
+                          const uint8_t *d_qcodes, const int32_t *d_qoff,
+                          const uint8_t *d_rcodes, const int32_t *d_roff,
+                          const int32_t *h_qlist, int32_t nq, int32_t nr, int32_t maxq,
+                          int32_t *d_out, gsw_stats *stats);
+int gsw_score_pairs_plain(const gsw_scheme *s, int32_t dtype, int32_t threads,
+                          const uint8_t *d_qcodes, const int32_t *d_qoff,
+                          const uint8_t *d_rcodes, const int32_t *d_roff,
+                          const int32_t *h_qi, const int32_t *h_rj, int64_t n, int32_t maxq,
+                          int32_t *d_out, gsw_stats *stats);
+
+/* ---- indexed sequence sets ------------------------------------------------
+ * A set of n sequences already resident on the device (codes + n+1 offsets, as
+ * produced by the reference's encode.funnel, src/gpusw/encode.py:164-220).  The
+ * set borrows the device buffers (caller keeps them alive) and copies the host
+ * offsets to build its length index.  Replaces the Aligner's per-set device state
+ * (src/gpusw/aligner.py:69-87). */
+typedef struct gsw_seqset_s *gsw_seqset;
+int gsw_seqset_create(const uint8_t *d_codes, const int32_t *d_off, const int32_t *h_off,
+                      int32_t n, gsw_seqset *out);
+int gsw_seqset_destroy(gsw_seqset set);
+
+/* ---- batched wavefront path (backend="hip") ------------------------------
+ * Same contract and outputs as the plain calls above (int32 scores, row-major
+ * (nq, nr) for cross), computed by the packed-int16 wavefront kernels with int32
+ * fallback.  dtype_policy is GSW_DTYPE_AUTO/INT16/INT32 (src/gpusw/scheme.py:116-141
+ * decides it on the host; AUTO lets the library pick per pair). */
+int gsw_score_cross(const gsw_scheme *s, int32_t dtype_policy,
+                    gsw_seqset queries, const int32_t *h_qlist, int32_t nq,
+                    gsw_seqset refs, int32_t *d_out, gsw_stats *stats);
+int gsw_score_pairs(const gsw_scheme *s, int32_t dtype_policy,
+                    gsw_seqset queries, gsw_seqset refs,
+                    const int32_t *h_qi, const int32_t *h_rj, int64_t n,
+                    int32_t *d_out, gsw_stats *stats);
+
+/* ---- device top-k ---------------------------------------------------------
+ * Per row of a device int32 (nrows, ncols) score matrix, the k best columns,
+ * best first, ties broken toward the lower column index.  Replaces
+ * CudaBackend.topk (src/gpusw/backends/cuda.py:93-106) whose tie order is
+ * unspecified; the order here is that of the dense result's stable sort
+ * (src/gpusw/result.py:37).  Host outputs h_idx/h_sc are (nrows, k). */
+int gsw_topk_rows(const int32_t *d_scores, int32_t nrows, int32_t ncols, int32_t k,
+                  int32_t *h_idx, int32_t *h_sc, gsw_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GSW_HIP_H */

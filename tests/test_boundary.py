@@ -1,0 +1,64 @@
+"""The C ABI boundary (CPU): the built library exports every symbol that
+include/gsw_hip.h declares, the ctypes declarations cover them, and backend resolution
+behaves as the reference's (unknown name -> ValueError; no GPU -> NoGpuError)."""
+import ctypes
+import re
+from pathlib import Path
+
+import pytest
+
+import gpusw_amd
+from gpusw_amd import _hip
+from gpusw_amd.backends import resolve_backend
+from gpusw_amd.errors import GpuSWError, NoGpuError, OverflowRiskError
+
+ROOT = Path(__file__).resolve().parent.parent
+HEADER = (ROOT / "include" / "gsw_hip.h").read_text()
+DECLARED = sorted(set(re.findall(r"^(?:int|const char \*)\s*(gsw_\w+)\(", HEADER, re.M)))
+
+
+def test_header_declares_core_entry_points():
+    for name in ("gsw_score_cross", "gsw_score_pairs", "gsw_score_cross_plain",
+                 "gsw_score_pairs_plain", "gsw_topk_rows", "gsw_last_error", "gsw_seqset_create"):
+        assert name in DECLARED
+
+
+def test_library_exports_every_declared_symbol():
+    if not _hip.LIB_PATH.exists():
+        pytest.fail("libgsw_hip.so is not built (run python -m gpusw_amd.build)")
+    so = ctypes.CDLL(str(_hip.LIB_PATH))
+    missing = [n for n in DECLARED if not hasattr(so, n)]
+    assert not missing, missing
+    assert set(DECLARED) <= set(_hip._SIGNATURES)
+
+
+def test_status_mapping():
+    _hip.load()
+    with pytest.raises(OverflowRiskError):
+        _hip.check(_hip.GSW_ERR_OVERFLOW)
+    with pytest.raises(ValueError):
+        _hip.check(_hip.GSW_ERR_INVALID)
+    with pytest.raises(NoGpuError):
+        _hip.check(_hip.GSW_ERR_NO_DEVICE)
+    with pytest.raises(GpuSWError):
+        _hip.check(_hip.GSW_ERR_HIP)
+
+
+def test_unknown_backend_name():
+    with pytest.raises(ValueError):
+        resolve_backend("cuda")
+
+
+@pytest.mark.skipif(Path("/dev/kfd").exists(), reason="a GPU is present")
+def test_no_gpu_raises_no_gpu_error():
+    assert not gpusw_amd.gpu_available()
+    assert gpusw_amd.available_backends() == []
+    with pytest.raises(NoGpuError):
+        resolve_backend("auto")
+    with pytest.raises(NoGpuError):
+        gpusw_amd.align_score(["ACGT"], ["ACGT"])
+
+
+def test_aligner_is_gpu_free_until_scoring():
+    al = gpusw_amd.Aligner("dna")
+    assert al.gcups == 0.0 and al.query_ids == []
