@@ -18,7 +18,8 @@ from .errors import GpuSWError, NoGpuError, OverflowRiskError
 
 LIB_PATH = Path(os.environ.get("GSW_HIP_LIB", Path(__file__).resolve().parent / "libgsw_hip.so"))
 
-GSW_OK, GSW_ERR_NO_DEVICE, GSW_ERR_INVALID, GSW_ERR_HIP, GSW_ERR_LIMIT, GSW_ERR_OVERFLOW, GSW_ERR_BUSY = range(7)
+(GSW_OK, GSW_ERR_NO_DEVICE, GSW_ERR_INVALID, GSW_ERR_HIP, GSW_ERR_LIMIT, GSW_ERR_OVERFLOW,
+ GSW_ERR_BUSY, GSW_ERR_TIMEOUT) = range(8)
 MODE = {"local": 0, "global": 1, "semiglobal": 2}
 DTYPE_AUTO, DTYPE_INT16, DTYPE_INT32 = 0, 1, 2
 
@@ -37,6 +38,24 @@ class GswStats(ctypes.Structure):
     _fields_ = [("kernel_ms", ctypes.c_double), ("wall_ms", ctypes.c_double), ("cells", _i64),
                 ("pairs", _i64), ("rescored", _i64), ("plain_pairs", _i64), ("launches", _i32),
                 ("variant", _i32)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class GswRequestInfo(ctypes.Structure):
+    _fields_ = [("submit_ms", ctypes.c_double), ("admit_ms", ctypes.c_double),
+                ("done_ms", ctypes.c_double), ("latency_ms", ctypes.c_double), ("cells", _i64),
+                ("nq", _i32), ("k", _i32), ("launches", _i32), ("status", _i32)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class GswServerStatus(ctypes.Structure):
+    _fields_ = [("submitted", _i64), ("completed", _i64), ("rejected", _i64), ("launches", _i64),
+                ("cells", _i64), ("rescored", _i64), ("queued", _i32), ("active", _i32),
+                ("memory_budget", _sz), ("memory_used", _sz), ("uptime_ms", ctypes.c_double)]
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -67,6 +86,13 @@ _SIGNATURES = {
     "gsw_score_pairs": [ctypes.POINTER(GswScheme), _i32, _vp, _vp, _vp, _vp, _i64, _vp,
                         ctypes.POINTER(GswStats)],
     "gsw_topk_rows": [_vp, _i32, _i32, _i32, _vp, _vp, ctypes.POINTER(GswStats)],
+    "gsw_memcpy_d2d": [_vp, _vp, _sz],
+    "gsw_server_create": [_sz, ctypes.c_double, ctypes.POINTER(_vp)],
+    "gsw_server_add_db": [_vp, ctypes.POINTER(GswScheme), _vp, ctypes.POINTER(_i32)],
+    "gsw_server_submit": [_vp, _i32, _vp, _vp, _i32, _i32, ctypes.POINTER(_i64)],
+    "gsw_server_wait": [_vp, _i64, _i32, _vp, _vp, ctypes.POINTER(GswRequestInfo)],
+    "gsw_server_stats": [_vp, ctypes.POINTER(GswServerStatus)],
+    "gsw_server_destroy": [_vp],
     "gsw_force_variant": [_i32, _i32],
     "gsw_list_variants": [_vp, _i32, ctypes.POINTER(_i32)],
 }
@@ -115,6 +141,8 @@ def check(rc: int) -> None:
         raise ValueError(msg)
     if rc == GSW_ERR_OVERFLOW:
         raise OverflowRiskError(msg)
+    if rc == GSW_ERR_TIMEOUT:
+        raise TimeoutError(msg)
     raise GpuSWError(msg)
 
 

@@ -1,0 +1,105 @@
+// Internal interface of the batched wavefront kernels (shared by the one-shot
+// scoring calls and the serving loop).
+#pragma once
+
+#include "gsw_internal.h"
+#include "gsw_sweep.h"
+
+namespace gsw {
+
+// Work item of the cross kernel: one query pair (or single query) against the
+// length-sorted references [rbeg, rend).
+struct CrossItem {
+    int32_t qsa, lqa;        // query A: start in the query code buffer, length
+    int32_t qsb, lqb;        // query B (lqb < 0: single query)
+    int64_t outa, outb;      // output offset of reference 0 for A / B (outb < 0: none)
+    int32_t rbeg, rend;      // range in the sorted reference order
+    int32_t kstar, pad;
+};
+
+// Task of the pairs kernel: queries qa and qb (qb < 0: single) against reference ref.
+struct PairTask {
+    int32_t qa, qb, ref, kstar;
+    int64_t oa, ob;          // output indices (ob < 0: none)
+};
+
+struct SatEntry {
+    int32_t qs, ql;          // query start and length
+    int32_t r, pad;          // reference id
+    int64_t o;               // output index
+};
+
+struct KArgs {
+    const uint8_t *qcodes;
+    const int32_t *qoff;
+    const uint8_t *rcodes;
+    const int32_t *roff;
+    const int32_t *rorder;
+    const int32_t *sub;
+    int32_t ns;
+    SweepConst k;
+    int32_t *out;
+    int64_t ldo;
+    SatEntry *sat;
+    int32_t *sat_count;
+    int32_t sat_cap;
+    int32_t sat_check;
+    const void *items;
+    int32_t nitems;
+    int32_t *counter;
+};
+
+
+constexpr int CROSS_WAVES = 8;
+constexpr int PAIRS_WAVES = 4;
+
+typedef void (*LaunchFn)(const KArgs &, int grid, size_t lds, hipStream_t);
+
+struct Variant {
+    int id, G, R;
+    bool packed, local;
+    const void *cross_fn, *pairs_fn;
+    LaunchFn cross, pairs;
+};
+
+std::vector<Variant> &variant_table();
+const Variant *pick_variant(int lq, double mean_lr, bool packed, bool local, int ns, size_t lds_limit,
+                            bool for_pairs);
+int max_width();
+size_t cross_lds_bytes(const Variant &v, int ns);
+size_t pairs_lds_bytes(const Variant &v, int ns);
+int occupancy_grid(DeviceCtx *ctx, const void *fn, int threads, size_t lds, int *grid);
+
+// A query scored against a whole reference set: code start/length in the query code
+// buffer, output offset of its row, and an id (query index) for the comparator route.
+struct CrossQuery {
+    int32_t start, len;
+    int64_t out;
+    int32_t id;
+};
+struct CrossUnit {
+    const Variant *v;
+    CrossQuery a, b;
+    bool has_b;
+    int kstar;
+};
+
+// Route queries to tiles, pairing packed int16 queries of the same tile (and of the same
+// last-column register when semiglobal tracks the last column).  Queries no tile covers
+// are appended to *plain.  Units come out sorted by tile.
+int plan_cross_units(const SchemeDev &s, int dtype_policy, const std::vector<CrossQuery> &queries,
+                     int max_lr, double mean_lr, size_t lds_limit, std::vector<CrossUnit> *units,
+                     std::vector<CrossQuery> *plain);
+// One item per (unit, chunk of `chunk` sorted references).
+void expand_cross_items(const CrossUnit *u0, const CrossUnit *u1, int64_t nref, int64_t chunk,
+                        std::vector<CrossItem> *items);
+
+// out[idx[i]] = val[i] on the context's stream (synchronous).
+int scatter_values(DeviceCtx *ctx, int32_t *d_out, const std::vector<int64_t> &idx,
+                   const std::vector<int32_t> &val);
+
+// Device top-k of nrows rows of d_scores (row stride ncols) into device idx/sc (nrows x k).
+void topk_launch(const int32_t *d_scores, int32_t nrows, int32_t ncols, int32_t k, int32_t *d_idx,
+                 int32_t *d_sc, hipStream_t stream);
+
+}  // namespace gsw

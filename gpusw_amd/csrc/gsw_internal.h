@@ -96,6 +96,7 @@ struct PlainJob {
     int32_t nr = 0; int64_t ldo = 0;
     // pairs mode
     const int32_t *h_qi = nullptr; const int32_t *h_rj = nullptr; const int64_t *h_outidx = nullptr;
+    const int32_t *h_qstart = nullptr; const int32_t *h_qlen = nullptr;   // instead of h_qi
     int64_t npairs = 0;
     int32_t *d_out;
 };

@@ -29,6 +29,8 @@ struct PlainArgs {
     const int32_t *qi;
     const int32_t *rj;
     const int64_t *outidx;
+    const int32_t *qstart;   // descriptor mode: query start/length per pair instead of qi
+    const int32_t *qlen;
     int64_t npairs;
     const int32_t *sub;
     int32_t ns, go, ge, free_q, free_r;
@@ -125,14 +127,15 @@ __global__ void __launch_bounds__(256) plain_kernel_body(PlainArgs a, int64_t to
             qi = a.qlist[row];
             o = (int64_t)a.outrow[row] * a.ldo + rj;
         } else {
-            qi = a.qi[k];
+            qi = a.qstart ? 0 : a.qi[k];
             rj = a.rj[k];
             o = a.outidx ? a.outidx[k] : k;
         }
-        const int qs = a.qoff[qi], rs = a.roff[rj];
+        const int qs = a.qstart ? a.qstart[k] : a.qoff[qi];
+        const int rs = a.roff[rj];
         // the workspace holds maxq+1 columns; longer queries are truncated as the
         // reference truncates at MAXQ (kernel.py:61) -- callers pass the true maximum
-        const int lq = min(a.qoff[qi + 1] - qs, maxq);
+        const int lq = min(a.qstart ? a.qlen[k] : a.qoff[qi + 1] - qs, maxq);
         a.out[o] = plain_score<MODE, ST>(a.qcodes + qs, lq, a.rcodes + rs,
                                          a.roff[rj + 1] - rs, s_sub, a, hp, fp, a.stride);
     }
@@ -167,7 +170,7 @@ int plain_run(DeviceCtx *ctx, const SchemeDev &s, const PlainJob &job, float *ms
     // stage the index lists in device scratch
     size_t need = 0;
     if (cross) need = sizeof(int32_t) * 2 * (size_t)job.nq;
-    else need = sizeof(int32_t) * 2 * (size_t)job.npairs + (job.h_outidx ? sizeof(int64_t) * job.npairs : 0);
+    else need = sizeof(int32_t) * 4 * (size_t)job.npairs + (job.h_outidx ? sizeof(int64_t) * job.npairs : 0);
     GSW_TRY(ensure_buffer(&ctx->meta, &ctx->meta_bytes, need + 256));
     PlainArgs a{};
     a.qcodes = job.qcodes;
@@ -203,9 +206,18 @@ int plain_run(DeviceCtx *ctx, const SchemeDev &s, const PlainJob &job, float *ms
     } else {
         int32_t *d_qi = reinterpret_cast<int32_t *>(m);
         int32_t *d_rj = d_qi + job.npairs;
+        int32_t *d_qs = d_rj + job.npairs;
+        int32_t *d_ql = d_qs + job.npairs;
         int64_t *d_oi = reinterpret_cast<int64_t *>(
-            (reinterpret_cast<uintptr_t>(d_rj + job.npairs) + 15) & ~uintptr_t(15));
-        GSW_HIP_TRY(hipMemcpyAsync(d_qi, job.h_qi, sizeof(int32_t) * job.npairs, hipMemcpyHostToDevice, ctx->stream));
+            (reinterpret_cast<uintptr_t>(d_ql + job.npairs) + 15) & ~uintptr_t(15));
+        if (job.h_qstart) {
+            GSW_HIP_TRY(hipMemcpyAsync(d_qs, job.h_qstart, sizeof(int32_t) * job.npairs, hipMemcpyHostToDevice, ctx->stream));
+            GSW_HIP_TRY(hipMemcpyAsync(d_ql, job.h_qlen, sizeof(int32_t) * job.npairs, hipMemcpyHostToDevice, ctx->stream));
+            a.qstart = d_qs;
+            a.qlen = d_ql;
+        } else {
+            GSW_HIP_TRY(hipMemcpyAsync(d_qi, job.h_qi, sizeof(int32_t) * job.npairs, hipMemcpyHostToDevice, ctx->stream));
+        }
         GSW_HIP_TRY(hipMemcpyAsync(d_rj, job.h_rj, sizeof(int32_t) * job.npairs, hipMemcpyHostToDevice, ctx->stream));
         if (job.h_outidx)
             GSW_HIP_TRY(hipMemcpyAsync(d_oi, job.h_outidx, sizeof(int64_t) * job.npairs, hipMemcpyHostToDevice, ctx->stream));

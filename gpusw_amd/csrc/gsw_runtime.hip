@@ -210,7 +210,9 @@ int gsw_device_name(int32_t device, char *buf, int32_t buflen) {
     GSW_TRY(init_runtime(&count));
     hipDeviceProp_t prop;
     GSW_HIP_TRY(hipGetDeviceProperties(&prop, device));
-    snprintf(buf, buflen, "%s (%s, %d CUs)", prop.name, prop.gcnArchName, prop.multiProcessorCount);
+    snprintf(buf, buflen, "%s (%s, %d CUs, %d MHz, LDS/block %zu KiB, %zu GiB)", prop.name, prop.gcnArchName,
+             prop.multiProcessorCount, prop.clockRate / 1000, prop.sharedMemPerBlock / 1024,
+             prop.totalGlobalMem >> 30);
     return GSW_OK;
 }
 
@@ -240,6 +242,11 @@ int gsw_memcpy_d2h(void *dst, const void *src, size_t bytes) {
 
 int gsw_memset(void *dptr, int32_t value, size_t bytes) {
     if (bytes) GSW_HIP_TRY(hipMemset(dptr, value, bytes));
+    return GSW_OK;
+}
+
+int gsw_memcpy_d2d(void *dst, const void *src, size_t bytes) {
+    if (bytes) GSW_HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
     return GSW_OK;
 }
 

@@ -1,0 +1,657 @@
+// Serving loop: resident reference databases, memory-budget admission and continuous
+// batching of top-k requests into the persistent cross kernel.
+//
+// A worker thread owns a private stream.  Submitted requests wait in FIFO order until
+// their device memory (query codes + score rows + top-k output) fits in the server's
+// arenas; admitted requests are planned into cross work items (query pair x chunk of
+// length-sorted references).  Every launch takes items round-robin from all admitted
+// requests of one (database, tile), up to a cell budget, so requests that arrive while
+// others run join the next launch and share the device with them.  Two launches are kept
+// in flight.  When a request's last item completes, a device top-k of its score rows
+// (score descending, ties to the lower reference index) is copied back and its memory is
+// released.  Saturated int16 results are rescored in int32 before a launch is retired.
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <thread>
+
+#include "gsw_batch.h"
+#include "gsw_internal.h"
+
+namespace gsw {
+
+namespace {
+
+constexpr size_t ALIGN = 256;
+inline size_t round_up(size_t v) { return (v + ALIGN - 1) & ~(ALIGN - 1); }
+
+// First-fit allocator over [0, cap) offsets (host bookkeeping for a device arena).
+class Arena {
+  public:
+    void init(size_t cap) {
+        cap_ = cap;
+        used_ = 0;
+        free_.clear();
+        if (cap) free_[0] = cap;
+    }
+    bool alloc(size_t bytes, size_t *off) {
+        bytes = round_up(std::max<size_t>(bytes, 1));
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < bytes) continue;
+            *off = it->first;
+            const size_t rest = it->second - bytes, start = it->first + bytes;
+            free_.erase(it);
+            if (rest) free_[start] = rest;
+            used_ += bytes;
+            return true;
+        }
+        return false;
+    }
+    void release(size_t off, size_t bytes) {
+        bytes = round_up(std::max<size_t>(bytes, 1));
+        used_ -= bytes;
+        auto it = free_.emplace(off, bytes).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+    size_t cap() const { return cap_; }
+    size_t used() const { return used_; }
+
+  private:
+    std::map<size_t, size_t> free_;
+    size_t cap_ = 0, used_ = 0;
+};
+
+struct ServerDb {
+    SchemeDev s;
+    const SeqSet *rs;
+    double mean_lr;
+    std::vector<int64_t> prefix;   // prefix sums of reference lengths in sorted order
+};
+
+struct Request {
+    int64_t id = 0;
+    int db = 0, nq = 0, k = 0, kk = 0;
+    std::vector<uint8_t> codes;
+    std::vector<int32_t> off;
+    size_t codes_at = 0, codes_bytes = 0;     // in the code arena
+    size_t scores_at = 0, scores_bytes = 0;   // in the score arena (scores + top-k output)
+    std::vector<CrossItem> items;             // grouped by tile
+    std::vector<int> item_tile;               // variant id per item
+    size_t next = 0;
+    int inflight = 0;
+    std::vector<int32_t> idx, sc;
+    int status = GSW_OK;
+    std::string error;
+    bool done = false;
+    double t_submit = 0, t_admit = 0, t_done = 0;
+    int64_t cells = 0;
+    int launches = 0;
+};
+
+struct Slot {
+    hipEvent_t ev = nullptr;
+    void *items = nullptr;
+    size_t items_cap = 0;
+    int32_t *counters = nullptr;   // [0] work counter, [1] saturation count
+    SatEntry *sat = nullptr;
+    int db = -1;
+    std::vector<std::pair<Request *, int>> reqs;
+    std::vector<CrossItem> host_items;
+};
+
+constexpr int SAT_CAP = 1 << 16;
+constexpr int NSLOTS = 2;
+
+}  // namespace
+
+struct Server {
+    DeviceCtx ctx;                       // private stream, events and scratch
+    uint8_t *code_base = nullptr;
+    uint8_t *score_base = nullptr;
+    Arena code_arena, score_arena;
+    double launch_cells = 2e11;
+    std::vector<std::unique_ptr<ServerDb>> dbs;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Request *> queue;
+    std::vector<Request *> active;
+    std::map<int64_t, std::unique_ptr<Request>> reqs;
+    Slot slots[NSLOTS];
+    std::deque<int> inflight;
+    std::thread worker;
+    bool stop = false;
+    int64_t next_id = 1;
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+    // statistics
+    int64_t submitted = 0, completed = 0, rejected = 0, launches = 0, cells_done = 0, rescored = 0;
+    double gpu_ms = 0;
+    std::string fatal;
+
+    double now_ms() const {
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    void run();
+    int admit_locked();
+    int plan_locked(Request *r);
+    Request *next_issuable_locked();
+    int launch_locked(int slot, Request *head);
+    int retire(int slot);
+    int finalize(Request *r);
+};
+
+// ------------------------------------------------------------------ planning
+int Server::plan_locked(Request *r) {
+    const ServerDb &db = *dbs[r->db];
+    const SeqSet &rs = *db.rs;
+    const int32_t nr = rs.n;
+    std::vector<CrossQuery> cq;
+    std::vector<int64_t> e_idx;
+    std::vector<int32_t> e_val;
+    for (int a = 0; a < r->nq; ++a) {
+        const int32_t len = r->off[a + 1] - r->off[a];
+        const int64_t row = (int64_t)(r->scores_at / 4) + (int64_t)a * nr;
+        if (len == 0 || rs.n_nonempty == 0) {
+            for (int b = 0; b < nr; ++b) {
+                e_idx.push_back(row + b);
+                e_val.push_back(empty_edge_score(db.s, len, rs.len[b]));
+            }
+            continue;
+        }
+        for (int p = rs.n_nonempty; p < nr; ++p) {
+            e_idx.push_back(row + rs.order_desc[p]);
+            e_val.push_back(empty_edge_score(db.s, len, 0));
+        }
+        cq.push_back(CrossQuery{(int32_t)r->codes_at + r->off[a], len, row, a});
+        r->cells += (int64_t)len * rs.total;
+    }
+    std::vector<CrossUnit> units;
+    std::vector<CrossQuery> plain;
+    GSW_TRY(plan_cross_units(db.s, GSW_DTYPE_AUTO, cq, rs.max_len, db.mean_lr, (size_t)ctx.max_smem, &units,
+                             &plain));
+    // chunk references so an item is ~1e8 cells: enough items per launch to fill the GPU
+    const int64_t nref = rs.n_nonempty;
+    for (size_t u0 = 0; u0 < units.size();) {
+        size_t u1 = u0;
+        while (u1 < units.size() && units[u1].v == units[u0].v) ++u1;
+        const Variant &v = *units[u0].v;
+        const int gpw = 64 / v.G;
+        const double per_ref = (double)(units[u0].a.len + std::max(units[u0].b.len, 0)) * db.mean_lr;
+        int64_t chunk = (int64_t)(1e8 / std::max(per_ref, 1.0));
+        chunk = std::max<int64_t>(chunk, (int64_t)gpw * CROSS_WAVES);
+        chunk = std::min<int64_t>(chunk, nref);
+        chunk = (chunk + gpw - 1) / gpw * gpw;
+        const size_t before = r->items.size();
+        expand_cross_items(units.data() + u0, units.data() + u1, nref, chunk, &r->items);
+        r->item_tile.resize(r->items.size(), v.id);
+        (void)before;
+        u0 = u1;
+    }
+    // queries no tile covers, and edge cells, are scored now with the comparator
+    if (!plain.empty() || !e_idx.empty()) {
+        if (!plain.empty()) {
+            std::vector<int32_t> qs, ql, rj;
+            std::vector<int64_t> oi;
+            int maxq = 1;
+            for (const CrossQuery &q : plain) {
+                maxq = std::max(maxq, q.len);
+                for (int b = 0; b < nr; ++b) {
+                    if (rs.len[b] == 0) continue;
+                    qs.push_back(q.start);
+                    ql.push_back(q.len);
+                    rj.push_back(b);
+                    oi.push_back(q.out + b);
+                }
+            }
+            PlainJob job{};
+            job.qcodes = code_base; job.qoff = nullptr; job.rcodes = rs.d_codes; job.roff = rs.d_off;
+            job.dtype = GSW_DTYPE_INT32; job.threads = 256; job.maxq = maxq;
+            job.h_qstart = qs.data(); job.h_qlen = ql.data(); job.h_rj = rj.data(); job.h_outidx = oi.data();
+            job.npairs = (int64_t)qs.size();
+            job.d_out = reinterpret_cast<int32_t *>(score_base);
+            float ms = 0.f;
+            GSW_TRY(plain_run(&ctx, db.s, job, &ms));
+            gpu_ms += ms;
+        }
+        if (!e_idx.empty())
+            GSW_TRY(scatter_values(&ctx, reinterpret_cast<int32_t *>(score_base), e_idx, e_val));
+    }
+    return GSW_OK;
+}
+
+int Server::admit_locked() {
+    while (!queue.empty()) {
+        Request *r = queue.front();
+        const SeqSet &rs = *dbs[r->db]->rs;
+        r->codes_bytes = std::max<size_t>(r->codes.size(), 1);
+        r->scores_bytes = 4 * ((size_t)r->nq * rs.n + 2 * (size_t)r->nq * std::max(r->kk, 1));
+        if (round_up(r->codes_bytes) > code_arena.cap() || round_up(r->scores_bytes) > score_arena.cap()) {
+            r->status = GSW_ERR_BUSY;
+            r->error = "request needs more device memory than the server budget";
+            r->done = true;
+            r->t_done = now_ms();
+            rejected++;
+            queue.pop_front();
+            cv_done.notify_all();
+            continue;
+        }
+        size_t ca = 0, sa = 0;
+        if (!code_arena.alloc(r->codes_bytes, &ca)) break;
+        if (!score_arena.alloc(r->scores_bytes, &sa)) {
+            code_arena.release(ca, r->codes_bytes);
+            break;
+        }
+        queue.pop_front();
+        r->codes_at = ca;
+        r->scores_at = sa;
+        r->t_admit = now_ms();
+        if (!r->codes.empty())
+            GSW_HIP_TRY(hipMemcpyAsync(code_base + ca, r->codes.data(), r->codes.size(), hipMemcpyHostToDevice,
+                                       ctx.stream));
+        int rc = plan_locked(r);
+        if (rc != GSW_OK) {
+            r->status = rc;
+            r->error = gsw_last_error();
+            code_arena.release(r->codes_at, r->codes_bytes);
+            score_arena.release(r->scores_at, r->scores_bytes);
+            r->done = true;
+            r->t_done = now_ms();
+            cv_done.notify_all();
+            continue;
+        }
+        active.push_back(r);
+        if (r->items.empty() && r->inflight == 0) {
+            GSW_TRY(finalize(r));
+        }
+    }
+    return GSW_OK;
+}
+
+Request *Server::next_issuable_locked() {
+    for (Request *r : active)
+        if (r->next < r->items.size()) return r;
+    return nullptr;
+}
+
+// ------------------------------------------------------------------ launching
+int Server::launch_locked(int si, Request *head) {
+    Slot &sl = slots[si];
+    const int db = head->db;
+    const int tile = head->item_tile[head->next];
+    const ServerDb &d = *dbs[db];
+    const Variant &v = variant_table()[tile];
+    sl.host_items.clear();
+    sl.reqs.clear();
+    sl.db = db;
+    // round-robin over admitted requests of this (database, tile), oldest first
+    double cells = 0;
+    const double per_item_ref = d.mean_lr;
+    bool progress = true;
+    while (progress && cells < launch_cells) {
+        progress = false;
+        for (Request *r : active) {
+            if (r->db != db || r->next >= r->items.size() || r->item_tile[r->next] != tile) continue;
+            const CrossItem &it = r->items[r->next++];
+            sl.host_items.push_back(it);
+            if (sl.reqs.empty() || sl.reqs.back().first != r) {
+                bool seen = false;
+                for (auto &pr : sl.reqs)
+                    if (pr.first == r) { pr.second++; seen = true; break; }
+                if (!seen) sl.reqs.emplace_back(r, 1);
+            } else {
+                sl.reqs.back().second++;
+            }
+            cells += (double)(it.lqa + std::max(it.lqb, 0)) * per_item_ref * (it.rend - it.rbeg);
+            progress = true;
+            if (cells >= launch_cells) break;
+        }
+    }
+    for (auto &pr : sl.reqs) {
+        pr.first->inflight++;
+        pr.first->launches++;
+    }
+    const size_t bytes = sizeof(CrossItem) * sl.host_items.size();
+    GSW_TRY(ensure_buffer(&sl.items, &sl.items_cap, bytes));
+    GSW_HIP_TRY(hipMemcpyAsync(sl.items, sl.host_items.data(), bytes, hipMemcpyHostToDevice, ctx.stream));
+    GSW_HIP_TRY(hipMemsetAsync(sl.counters, 0, 2 * sizeof(int32_t), ctx.stream));
+    const size_t lds = cross_lds_bytes(v, d.s.ns);
+    int grid = 0;
+    GSW_TRY(occupancy_grid(&ctx, v.cross_fn, CROSS_WAVES * 64, lds, &grid));
+    KArgs a{};
+    a.qcodes = code_base;
+    a.rcodes = d.rs->d_codes;
+    a.roff = d.rs->d_off;
+    a.rorder = d.rs->d_order_desc;
+    a.sub = d.s.d_sub;
+    a.ns = d.s.ns;
+    a.k = SweepConst{d.s.mode, d.s.go, d.s.ge, d.s.free_q, d.s.free_r, d.s.ns};
+    a.out = reinterpret_cast<int32_t *>(score_base);
+    a.sat = sl.sat;
+    a.sat_count = sl.counters + 1;
+    a.sat_cap = SAT_CAP;
+    a.sat_check = (v.packed && v.local) ? 1 : 0;
+    a.items = sl.items;
+    a.nitems = (int32_t)sl.host_items.size();
+    a.counter = sl.counters;
+    v.cross(a, std::min<int>(grid, a.nitems), lds, ctx.stream);
+    GSW_HIP_TRY(hipGetLastError());
+    GSW_HIP_TRY(hipEventRecord(sl.ev, ctx.stream));
+    launches++;
+    inflight.push_back(si);
+    return GSW_OK;
+}
+
+int Server::finalize(Request *r) {
+    const SeqSet &rs = *dbs[r->db]->rs;
+    const int32_t nr = rs.n;
+    r->idx.assign((size_t)r->nq * r->kk, 0);
+    r->sc.assign((size_t)r->nq * r->kk, 0);
+    if (r->nq > 0 && r->kk > 0) {
+        int32_t *scores = reinterpret_cast<int32_t *>(score_base + r->scores_at);
+        int32_t *d_idx = scores + (size_t)r->nq * nr;
+        int32_t *d_sc = d_idx + (size_t)r->nq * r->kk;
+        topk_launch(scores, r->nq, nr, r->kk, d_idx, d_sc, ctx.stream);
+        GSW_HIP_TRY(hipGetLastError());
+        GSW_HIP_TRY(hipMemcpyAsync(r->idx.data(), d_idx, 4 * r->idx.size(), hipMemcpyDeviceToHost, ctx.stream));
+        GSW_HIP_TRY(hipMemcpyAsync(r->sc.data(), d_sc, 4 * r->sc.size(), hipMemcpyDeviceToHost, ctx.stream));
+        GSW_HIP_TRY(hipStreamSynchronize(ctx.stream));
+    }
+    code_arena.release(r->codes_at, r->codes_bytes);
+    score_arena.release(r->scores_at, r->scores_bytes);
+    r->codes.clear();
+    r->codes.shrink_to_fit();
+    r->items.clear();
+    r->items.shrink_to_fit();
+    r->done = true;
+    r->t_done = now_ms();
+    completed++;
+    cells_done += r->cells;
+    active.erase(std::find(active.begin(), active.end(), r));
+    cv_done.notify_all();
+    return GSW_OK;
+}
+
+int Server::retire(int si) {
+    Slot &sl = slots[si];
+    GSW_HIP_TRY(hipEventSynchronize(sl.ev));
+    int32_t nsat = 0;
+    GSW_HIP_TRY(hipMemcpy(&nsat, sl.counters + 1, sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (nsat > 0) {
+        const ServerDb &d = *dbs[sl.db];
+        std::vector<SatEntry> h(std::min(nsat, SAT_CAP));
+        GSW_HIP_TRY(hipMemcpy(h.data(), sl.sat, sizeof(SatEntry) * h.size(), hipMemcpyDeviceToHost));
+        std::vector<int32_t> qs, ql, rj;
+        std::vector<int64_t> oi;
+        int maxq = 1;
+        if (nsat > SAT_CAP) {
+            // list overflowed: rescore every pair of this launch's items in int32
+            for (const CrossItem &it : sl.host_items) {
+                for (int p = it.rbeg; p < it.rend; ++p) {
+                    const int ref = d.rs->order_desc[p];
+                    qs.push_back(it.qsa); ql.push_back(it.lqa); rj.push_back(ref); oi.push_back(it.outa + ref);
+                    if (it.outb >= 0) {
+                        qs.push_back(it.qsb); ql.push_back(it.lqb); rj.push_back(ref); oi.push_back(it.outb + ref);
+                    }
+                    maxq = std::max({maxq, it.lqa, it.lqb});
+                }
+            }
+        } else {
+            for (const SatEntry &e : h) {
+                qs.push_back(e.qs); ql.push_back(e.ql); rj.push_back(e.r); oi.push_back(e.o);
+                maxq = std::max(maxq, e.ql);
+            }
+        }
+        PlainJob job{};
+        job.qcodes = code_base; job.qoff = nullptr; job.rcodes = d.rs->d_codes; job.roff = d.rs->d_off;
+        job.dtype = GSW_DTYPE_INT32; job.threads = 256; job.maxq = maxq;
+        job.h_qstart = qs.data(); job.h_qlen = ql.data(); job.h_rj = rj.data(); job.h_outidx = oi.data();
+        job.npairs = (int64_t)qs.size();
+        job.d_out = reinterpret_cast<int32_t *>(score_base);
+        float ms = 0.f;
+        GSW_TRY(plain_run(&ctx, d.s, job, &ms));
+        std::lock_guard<std::mutex> lk(mu);
+        rescored += (int64_t)qs.size();
+    }
+    std::unique_lock<std::mutex> lk(mu);
+    for (auto &pr : sl.reqs) {
+        Request *r = pr.first;
+        r->inflight--;
+        if (r->inflight == 0 && r->next >= r->items.size() && !r->done) GSW_TRY(finalize(r));
+    }
+    sl.reqs.clear();
+    return GSW_OK;
+}
+
+void Server::run() {
+    (void)hipSetDevice(ctx.device);
+    std::unique_lock<std::mutex> lk(mu);
+    for (;;) {
+        int rc = admit_locked();
+        Request *head = rc == GSW_OK ? next_issuable_locked() : nullptr;
+        if (rc == GSW_OK && head && (int)inflight.size() < NSLOTS) {
+            int free_slot = -1;
+            for (int i = 0; i < NSLOTS; ++i)
+                if (std::find(inflight.begin(), inflight.end(), i) == inflight.end()) free_slot = i;
+            rc = launch_locked(free_slot, head);
+            if (rc == GSW_OK) continue;
+        }
+        if (rc == GSW_OK && !inflight.empty()) {
+            const int si = inflight.front();
+            inflight.pop_front();
+            lk.unlock();
+            const double t = now_ms();
+            rc = retire(si);
+            lk.lock();
+            gpu_ms += now_ms() - t;
+            if (rc == GSW_OK) continue;
+        }
+        if (rc != GSW_OK) {
+            // a device failure ends the server: fail everything that is pending
+            fatal = gsw_last_error();
+            for (auto &kv : reqs) {
+                Request *r = kv.second.get();
+                if (!r->done) {
+                    r->status = rc;
+                    r->error = fatal;
+                    r->done = true;
+                }
+            }
+            queue.clear();
+            active.clear();
+            inflight.clear();
+            cv_done.notify_all();
+            stop = true;
+            return;
+        }
+        if (stop && queue.empty() && active.empty()) return;
+        cv_work.wait(lk);
+    }
+}
+
+}  // namespace gsw
+
+using namespace gsw;
+
+extern "C" {
+
+int gsw_server_create(size_t mem_budget_bytes, double launch_cells, gsw_server *out) {
+    if (!out) return fail(GSW_ERR_INVALID, "out is null");
+    *out = nullptr;
+    DeviceCtx *shared = nullptr;
+    GSW_TRY(ctx_get(&shared));
+    auto srv = std::make_unique<Server>();
+    srv->ctx.device = shared->device;
+    srv->ctx.num_cus = shared->num_cus;
+    srv->ctx.max_smem = shared->max_smem;
+    GSW_HIP_TRY(hipStreamCreateWithFlags(&srv->ctx.stream, hipStreamNonBlocking));
+    GSW_HIP_TRY(hipEventCreate(&srv->ctx.ev0));
+    GSW_HIP_TRY(hipEventCreate(&srv->ctx.ev1));
+    GSW_HIP_TRY(hipMalloc(&srv->ctx.counters, 64 * sizeof(int32_t)));
+    if (launch_cells > 0) srv->launch_cells = launch_cells;
+    // query codes are addressed with 32-bit offsets: at most 1 GiB of them
+    const size_t budget = std::max<size_t>(mem_budget_bytes, size_t(64) << 20);
+    const size_t code_cap = std::min<size_t>(budget / 16, size_t(1) << 30);
+    const size_t score_cap = budget - code_cap;
+    GSW_HIP_TRY(hipMalloc(&srv->code_base, code_cap));
+    GSW_HIP_TRY(hipMalloc(&srv->score_base, score_cap));
+    srv->code_arena.init(code_cap);
+    srv->score_arena.init(score_cap);
+    for (int i = 0; i < NSLOTS; ++i) {
+        GSW_HIP_TRY(hipEventCreateWithFlags(&srv->slots[i].ev, hipEventDisableTiming));
+        GSW_HIP_TRY(hipMalloc(&srv->slots[i].counters, 2 * sizeof(int32_t)));
+        GSW_HIP_TRY(hipMalloc(&srv->slots[i].sat, sizeof(SatEntry) * SAT_CAP));
+    }
+    Server *p = srv.get();
+    p->worker = std::thread([p] { p->run(); });
+    *out = reinterpret_cast<gsw_server>(srv.release());
+    return GSW_OK;
+}
+
+int gsw_server_add_db(gsw_server server, const gsw_scheme *s, gsw_seqset refs, int32_t *db_id) {
+    Server *srv = reinterpret_cast<Server *>(server);
+    const SeqSet *rs = reinterpret_cast<const SeqSet *>(refs);
+    if (!srv || !rs || !db_id) return fail(GSW_ERR_INVALID, "null argument");
+    if (rs->device != srv->ctx.device) return fail(GSW_ERR_INVALID, "database lives on another device");
+    auto db = std::make_unique<ServerDb>();
+    GSW_TRY(scheme_prepare(s, &srv->ctx, &db->s));
+    db->rs = rs;
+    db->mean_lr = rs->n_nonempty ? (double)rs->total / rs->n_nonempty : 1.0;
+    std::lock_guard<std::mutex> lk(srv->mu);
+    if (srv->stop) return fail(GSW_ERR_HIP, "server stopped: %s", srv->fatal.c_str());
+    srv->dbs.push_back(std::move(db));
+    *db_id = (int32_t)srv->dbs.size() - 1;
+    return GSW_OK;
+}
+
+int gsw_server_submit(gsw_server server, int32_t db_id, const uint8_t *h_codes, const int32_t *h_off,
+                      int32_t nq, int32_t k, int64_t *req_id) {
+    Server *srv = reinterpret_cast<Server *>(server);
+    if (!srv || !req_id || (nq > 0 && (!h_off || !h_codes))) return fail(GSW_ERR_INVALID, "null argument");
+    if (nq < 0 || k < 0) return fail(GSW_ERR_INVALID, "negative query count or k");
+    auto r = std::make_unique<Request>();
+    {
+        std::lock_guard<std::mutex> lk(srv->mu);
+        if (db_id < 0 || db_id >= (int)srv->dbs.size()) return fail(GSW_ERR_INVALID, "unknown database %d", db_id);
+        if (srv->stop) return fail(GSW_ERR_HIP, "server stopped: %s", srv->fatal.c_str());
+    }
+    r->db = db_id;
+    r->nq = nq;
+    r->k = k;
+    r->off.assign(h_off, h_off + nq + 1);
+    const int32_t base = nq > 0 ? h_off[0] : 0;
+    for (int i = 0; i <= nq; ++i) {
+        r->off[i] -= base;
+        if (i && r->off[i] < r->off[i - 1]) return fail(GSW_ERR_INVALID, "offsets must be non-decreasing");
+    }
+    if (nq > 0) r->codes.assign(h_codes + base, h_codes + base + r->off[nq]);
+    std::lock_guard<std::mutex> lk(srv->mu);
+    r->kk = std::min<int>(k, srv->dbs[db_id]->rs->n);
+    r->id = srv->next_id++;
+    r->t_submit = srv->now_ms();
+    *req_id = r->id;
+    srv->queue.push_back(r.get());
+    srv->reqs[r->id] = std::move(r);
+    srv->submitted++;
+    srv->cv_work.notify_all();
+    return GSW_OK;
+}
+
+int gsw_server_wait(gsw_server server, int64_t req_id, int32_t timeout_ms, int32_t *h_idx, int32_t *h_sc,
+                    gsw_request_info *info) {
+    Server *srv = reinterpret_cast<Server *>(server);
+    if (!srv) return fail(GSW_ERR_INVALID, "null server");
+    std::unique_lock<std::mutex> lk(srv->mu);
+    auto it = srv->reqs.find(req_id);
+    if (it == srv->reqs.end()) return fail(GSW_ERR_INVALID, "unknown request %lld", (long long)req_id);
+    Request *r = it->second.get();
+    auto ready = [&] { return r->done; };
+    if (timeout_ms < 0) {
+        srv->cv_done.wait(lk, ready);
+    } else if (!srv->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready)) {
+        return fail(GSW_ERR_TIMEOUT, "request %lld not finished", (long long)req_id);
+    }
+    if (info) {
+        *info = gsw_request_info{};
+        info->submit_ms = r->t_submit;
+        info->admit_ms = r->t_admit;
+        info->done_ms = r->t_done;
+        info->latency_ms = r->t_done - r->t_submit;
+        info->cells = r->cells;
+        info->nq = r->nq;
+        info->k = r->kk;
+        info->launches = r->launches;
+        info->status = r->status;
+    }
+    const int status = r->status;
+    std::string err = r->error;
+    if (status == GSW_OK) {
+        if (h_idx && !r->idx.empty()) std::copy(r->idx.begin(), r->idx.end(), h_idx);
+        if (h_sc && !r->sc.empty()) std::copy(r->sc.begin(), r->sc.end(), h_sc);
+    }
+    srv->reqs.erase(it);
+    if (status != GSW_OK) return fail(status, "%s", err.c_str());
+    return GSW_OK;
+}
+
+int gsw_server_stats(gsw_server server, gsw_server_status *st) {
+    Server *srv = reinterpret_cast<Server *>(server);
+    if (!srv || !st) return fail(GSW_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> lk(srv->mu);
+    *st = gsw_server_status{};
+    st->submitted = srv->submitted;
+    st->completed = srv->completed;
+    st->rejected = srv->rejected;
+    st->launches = srv->launches;
+    st->cells = srv->cells_done;
+    st->rescored = srv->rescored;
+    st->queued = (int32_t)srv->queue.size();
+    st->active = (int32_t)srv->active.size();
+    st->memory_budget = srv->code_arena.cap() + srv->score_arena.cap();
+    st->memory_used = srv->code_arena.used() + srv->score_arena.used();
+    st->uptime_ms = srv->now_ms();
+    return GSW_OK;
+}
+
+int gsw_server_destroy(gsw_server server) {
+    Server *srv = reinterpret_cast<Server *>(server);
+    if (!srv) return GSW_OK;
+    {
+        std::lock_guard<std::mutex> lk(srv->mu);
+        srv->stop = true;
+        srv->cv_work.notify_all();
+    }
+    if (srv->worker.joinable()) srv->worker.join();
+    (void)hipStreamSynchronize(srv->ctx.stream);
+    for (int i = 0; i < NSLOTS; ++i) {
+        (void)hipEventDestroy(srv->slots[i].ev);
+        (void)hipFree(srv->slots[i].counters);
+        (void)hipFree(srv->slots[i].sat);
+        if (srv->slots[i].items) (void)hipFree(srv->slots[i].items);
+    }
+    (void)hipFree(srv->code_base);
+    (void)hipFree(srv->score_base);
+    (void)hipFree(srv->ctx.counters);
+    if (srv->ctx.plain_ws) (void)hipFree(srv->ctx.plain_ws);
+    if (srv->ctx.meta) (void)hipFree(srv->ctx.meta);
+    if (srv->ctx.meta2) (void)hipFree(srv->ctx.meta2);
+    (void)hipEventDestroy(srv->ctx.ev0);
+    (void)hipEventDestroy(srv->ctx.ev1);
+    (void)hipStreamDestroy(srv->ctx.stream);
+    delete srv;
+    return GSW_OK;
+}
+
+}  // extern "C"

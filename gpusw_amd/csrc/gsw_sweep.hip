@@ -9,54 +9,15 @@
 
 #include "gsw_internal.h"
 #include "gsw_sweep.h"
+#include "gsw_batch.h"
 
 namespace gsw {
 
-// Work item of the cross kernel: one query pair (or single query) against the
-// length-sorted references [rbeg, rend).
-struct CrossItem {
-    int32_t qa, qb;          // query ids (qb < 0: single query)
-    int32_t slota, slotb;    // output rows
-    int32_t rbeg, rend;      // range in the sorted reference order
-    int32_t kstar, pad;
-};
-
-// Task of the pairs kernel: queries qa and qb (qb < 0: single) against reference ref.
-struct PairTask {
-    int32_t qa, qb, ref, kstar;
-    int64_t oa, ob;          // output indices (ob < 0: none)
-};
-
-struct SatEntry {
-    int32_t q, r;
-    int64_t o;
-};
-
-struct KArgs {
-    const uint8_t *qcodes;
-    const int32_t *qoff;
-    const uint8_t *rcodes;
-    const int32_t *roff;
-    const int32_t *rorder;
-    const int32_t *sub;
-    int32_t ns;
-    SweepConst k;
-    int32_t *out;
-    int64_t ldo;
-    SatEntry *sat;
-    int32_t *sat_count;
-    int32_t sat_cap;
-    int32_t sat_check;
-    const void *items;
-    int32_t nitems;
-    int32_t *counter;
-};
-
-__device__ __forceinline__ void emit(const KArgs &a, int64_t o, int q, int r, int score) {
+__device__ __forceinline__ void emit(const KArgs &a, int64_t o, int qs, int ql, int r, int score) {
     a.out[o] = score;
     if (a.sat_check && score >= 32767) {
         const int i = atomicAdd(a.sat_count, 1);
-        if (i < a.sat_cap) a.sat[i] = SatEntry{q, r, o};
+        if (i < a.sat_cap) a.sat[i] = SatEntry{qs, ql, r, 0, o};
     }
 }
 
@@ -84,9 +45,8 @@ __global__ void __launch_bounds__(WAVES * 64) cross_kernel(KArgs a) {
         const int it = __builtin_amdgcn_readfirstlane(s_item);
         if (it >= a.nitems) break;
         const CrossItem item = items[it];
-        const int qa = item.qa, qb = item.qb >= 0 ? item.qb : item.qa;
-        const int qsa = a.qoff[qa], lqa = a.qoff[qa + 1] - qsa;
-        const int qsb = a.qoff[qb], lqb = a.qoff[qb + 1] - qsb;
+        const int qsa = item.qsa, lqa = item.lqa;
+        const int qsb = item.lqb >= 0 ? item.qsb : qsa, lqb = item.lqb >= 0 ? item.lqb : lqa;
         const int total = a.ns * G * LW;
         for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
             const int c = idx / (G * LW);
@@ -114,8 +74,8 @@ __global__ void __launch_bounds__(WAVES * 64) cross_kernel(KArgs a) {
             const SweepOut res = sweep_group<V, G, R, LOCAL>(a.rcodes + rs, lr, nsteps, prof_lane, t,
                                                              a.k, lqa, lqb, item.kstar);
             if (t == 0 && ref >= 0) {
-                emit(a, (int64_t)item.slota * a.ldo + ref, qa, ref, res.a);
-                if (item.slotb >= 0) emit(a, (int64_t)item.slotb * a.ldo + ref, qb, ref, res.b);
+                emit(a, item.outa + ref, qsa, lqa, ref, res.a);
+                if (item.outb >= 0) emit(a, item.outb + ref, qsb, lqb, ref, res.b);
             }
         }
         __syncthreads();
@@ -176,8 +136,8 @@ __global__ void __launch_bounds__(WAVES * 64) pairs_kernel(KArgs a) {
         const SweepOut res = sweep_group<V, G, R, LOCAL>(a.rcodes + rs, lr, nsteps, glds + t * 16, t,
                                                          a.k, lqa, lqb, kstar);
         if (t == 0 && valid) {
-            emit(a, task.oa, qa, task.ref, res.a);
-            if (task.ob >= 0) emit(a, task.ob, qb, task.ref, res.b);
+            emit(a, task.oa, qsa, lqa, task.ref, res.a);
+            if (task.ob >= 0) emit(a, task.ob, qsb, lqb, task.ref, res.b);
         }
     }
 }
@@ -190,17 +150,6 @@ __global__ void scatter_kernel(int32_t *out, const int64_t *idx, const int32_t *
 }
 
 // ---------------------------------------------------------------- variants
-constexpr int CROSS_WAVES = 8;
-constexpr int PAIRS_WAVES = 4;
-
-typedef void (*LaunchFn)(const KArgs &, int grid, size_t lds, hipStream_t);
-
-struct Variant {
-    int id, G, R;
-    bool packed, local;
-    const void *cross_fn, *pairs_fn;
-    LaunchFn cross, pairs;
-};
 
 template <typename V, int G, int R, bool L>
 static void launch_cross(const KArgs &a, int grid, size_t lds, hipStream_t st) {
@@ -219,7 +168,7 @@ static void launch_pairs(const KArgs &a, int grid, size_t lds, hipStream_t st) {
     GSW_VARIANT(s2, true, G, R, true), GSW_VARIANT(s2, true, G, R, false),                        \
         GSW_VARIANT(int, false, G, R, true), GSW_VARIANT(int, false, G, R, false)
 
-static std::vector<Variant> &variant_table() {
+std::vector<Variant> &variant_table() {
     static std::vector<Variant> table = [] {
         std::vector<Variant> v = {
             GSW_VARIANT4(8, 8),   GSW_VARIANT4(8, 12),  GSW_VARIANT4(8, 16),  GSW_VARIANT4(8, 24),
@@ -247,12 +196,10 @@ static double tile_cost(int G, int R, int lq, double lr) {
     return pad * step * fill * regs;
 }
 
-static size_t cross_lds_bytes(const Variant &v, int ns);
-static size_t pairs_lds_bytes(const Variant &v, int ns);
 
 // Cheapest compiled tile covering a query of length lq whose LDS profile fits
 // (cross: one profile per workgroup; pairs: one per lane group).
-static const Variant *pick_variant(int lq, double mean_lr, bool packed, bool local, int ns,
+const Variant *pick_variant(int lq, double mean_lr, bool packed, bool local, int ns,
                                    size_t lds_limit, bool for_pairs) {
     const Variant *best = nullptr;
     double best_cost = 1e30;
@@ -270,20 +217,20 @@ static const Variant *pick_variant(int lq, double mean_lr, bool packed, bool loc
     return best;
 }
 
-static int max_width() {
+int max_width() {
     int w = 0;
     for (const Variant &v : variant_table()) w = std::max(w, v.G * v.R);
     return w;
 }
 
-static size_t cross_lds_bytes(const Variant &v, int ns) {
+size_t cross_lds_bytes(const Variant &v, int ns) {
     return (size_t)ns * ((v.R + 3) / 4) * v.G * 16;
 }
-static size_t pairs_lds_bytes(const Variant &v, int ns) {
+size_t pairs_lds_bytes(const Variant &v, int ns) {
     return (size_t)PAIRS_WAVES * ns * ((v.R + 3) / 4) * 64 * 16;
 }
 
-static int occupancy_grid(DeviceCtx *ctx, const void *fn, int threads, size_t lds, int *grid) {
+int occupancy_grid(DeviceCtx *ctx, const void *fn, int threads, size_t lds, int *grid) {
     int per_cu = 0;
     GSW_HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     GSW_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, lds));
@@ -317,7 +264,7 @@ static int prepare_sat(DeviceCtx *ctx, SatBuffers *sb, int32_t cap) {
 // Rescore the listed saturated int16 results in int32 with the plain kernel.  Sets
 // *overflowed (and rescored nothing) when more pairs saturated than the list holds.
 static int rescore_saturated(DeviceCtx *ctx, const SchemeDev &s, const SatBuffers &sb,
-                             const SeqSet &qs, const SeqSet &rs, int32_t *d_out, BatchResult *res,
+                             const uint8_t *qcodes, const SeqSet &rs, int32_t *d_out, BatchResult *res,
                              bool *overflowed) {
     int32_t count = 0;
     GSW_HIP_TRY(hipMemcpyAsync(&count, ctx->counters + 1, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
@@ -326,19 +273,21 @@ static int rescore_saturated(DeviceCtx *ctx, const SchemeDev &s, const SatBuffer
     if (count == 0 || *overflowed) return GSW_OK;
     std::vector<SatEntry> h(count);
     GSW_HIP_TRY(hipMemcpy(h.data(), sb.d_sat, sizeof(SatEntry) * count, hipMemcpyDeviceToHost));
-    std::vector<int32_t> qi(count), rj(count);
+    std::vector<int32_t> qst(count), qln(count), rj(count);
     std::vector<int64_t> oi(count);
     int maxq = 1;
     for (int i = 0; i < count; ++i) {
-        qi[i] = h[i].q;
+        qst[i] = h[i].qs;
+        qln[i] = h[i].ql;
         rj[i] = h[i].r;
         oi[i] = h[i].o;
-        maxq = std::max(maxq, qs.len[h[i].q]);
+        maxq = std::max(maxq, h[i].ql);
     }
     PlainJob job{};
-    job.qcodes = qs.d_codes; job.qoff = qs.d_off; job.rcodes = rs.d_codes; job.roff = rs.d_off;
+    job.qcodes = qcodes; job.qoff = nullptr; job.rcodes = rs.d_codes; job.roff = rs.d_off;
     job.dtype = GSW_DTYPE_INT32; job.threads = 256; job.maxq = maxq;
-    job.h_qi = qi.data(); job.h_rj = rj.data(); job.h_outidx = oi.data(); job.npairs = count;
+    job.h_qstart = qst.data(); job.h_qlen = qln.data(); job.h_rj = rj.data(); job.h_outidx = oi.data();
+    job.npairs = count;
     job.d_out = d_out;
     float pms = 0.f;
     GSW_TRY(plain_run(ctx, s, job, &pms));
@@ -348,7 +297,7 @@ static int rescore_saturated(DeviceCtx *ctx, const SchemeDev &s, const SatBuffer
     return GSW_OK;
 }
 
-static int scatter_values(DeviceCtx *ctx, int32_t *d_out, const std::vector<int64_t> &idx,
+int scatter_values(DeviceCtx *ctx, int32_t *d_out, const std::vector<int64_t> &idx,
                           const std::vector<int32_t> &val) {
     const int64_t n = (int64_t)idx.size();
     if (n == 0) return GSW_OK;
@@ -379,6 +328,61 @@ static KArgs base_args(const SchemeDev &s, const SeqSet &qs, const SeqSet &rs, i
     return a;
 }
 
+// ---------------------------------------------------------------- cross planning
+int plan_cross_units(const SchemeDev &s, int dtype_policy, const std::vector<CrossQuery> &queries,
+                     int max_lr, double mean_lr, size_t lds_limit, std::vector<CrossUnit> *units,
+                     std::vector<CrossQuery> *plain) {
+    const bool local = s.mode == GSW_MODE_LOCAL;
+    const int wmax = max_width();
+    struct QInfo { CrossQuery q; const Variant *v; int kstar; };
+    std::vector<QInfo> packed_q, single_q;
+    for (const CrossQuery &q : queries) {
+        if (!fits_int32(s, q.len, max_lr))
+            return fail(GSW_ERR_OVERFLOW, "score range of query %d exceeds the int32 kernels", q.id);
+        const bool use16 = dtype_policy != GSW_DTYPE_INT32 && fits_int16(s, q.len, max_lr);
+        const Variant *v = q.len <= wmax ? pick_variant(q.len, mean_lr, use16, local, s.ns, lds_limit, false)
+                                         : nullptr;
+        if (!v) {
+            plain->push_back(q);
+            continue;
+        }
+        (use16 ? packed_q : single_q).push_back(QInfo{q, v, (q.len - 1) % v->R});
+    }
+    // pair packed queries of the same tile (and the same last-column register when the
+    // last column is tracked, so that it stays wave-uniform)
+    const bool need_kstar = s.mode == GSW_MODE_SEMIGLOBAL && s.free_q;
+    auto key = [&](const QInfo &x) { return std::make_tuple(x.v->id, need_kstar ? x.kstar : 0, x.q.len); };
+    auto by_key = [&](const QInfo &x, const QInfo &y) { return key(x) < key(y); };
+    std::stable_sort(packed_q.begin(), packed_q.end(), by_key);
+    std::stable_sort(single_q.begin(), single_q.end(), by_key);
+    for (size_t i = 0; i < packed_q.size();) {
+        const QInfo &x = packed_q[i];
+        if (i + 1 < packed_q.size() && packed_q[i + 1].v == x.v &&
+            (!need_kstar || packed_q[i + 1].kstar == x.kstar)) {
+            units->push_back(CrossUnit{x.v, x.q, packed_q[i + 1].q, true, x.kstar});
+            i += 2;
+        } else {
+            units->push_back(CrossUnit{x.v, x.q, CrossQuery{0, 0, -1, -1}, false, x.kstar});
+            i += 1;
+        }
+    }
+    for (const QInfo &x : single_q) units->push_back(CrossUnit{x.v, x.q, CrossQuery{0, 0, -1, -1}, false, x.kstar});
+    std::stable_sort(units->begin(), units->end(),
+                     [](const CrossUnit &x, const CrossUnit &y) { return x.v->id < y.v->id; });
+    return GSW_OK;
+}
+
+void expand_cross_items(const CrossUnit *u0, const CrossUnit *u1, int64_t nref, int64_t chunk,
+                        std::vector<CrossItem> *items) {
+    for (const CrossUnit *u = u0; u < u1; ++u) {
+        for (int64_t rb = 0; rb < nref; rb += chunk) {
+            items->push_back(CrossItem{u->a.start, u->a.len, u->has_b ? u->b.start : 0, u->has_b ? u->b.len : -1,
+                                       u->a.out, u->has_b ? u->b.out : -1, (int32_t)rb,
+                                       (int32_t)std::min(nref, rb + chunk), u->kstar, 0});
+        }
+    }
+}
+
 // ---------------------------------------------------------------- cross
 int batch_cross(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqSet &qs,
                 const int32_t *h_qlist, int32_t nq, const SeqSet &rs, int32_t *d_out,
@@ -386,7 +390,6 @@ int batch_cross(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
     const int32_t nr = rs.n;
     if (nq == 0 || nr == 0) return GSW_OK;
     const bool local = s.mode == GSW_MODE_LOCAL;
-    const int wmax = max_width();
     double mean_lr = rs.n_nonempty ? (double)rs.total / rs.n_nonempty : 1.0;
 
     // -- empty rows/columns: closed forms
@@ -410,63 +413,33 @@ int batch_cross(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
         }
     }
 
-    // -- route each query: batched (int16 packed / int32) or plain
-    struct QInfo { int slot, q, lq; const Variant *v; int kstar; };
-    std::vector<QInfo> packed_q, single_q;
-    std::vector<int32_t> plain_rows, plain_q;
+    // -- route each query: batched (int16 packed / int32) or plain; pair packed queries
+    std::vector<CrossQuery> cq;
     for (int a = 0; a < nq; ++a) {
         const int q = h_qlist[a];
-        const int lq = qs.len[q];
-        if (lq == 0 || rs.n_nonempty == 0) continue;
-        if (!fits_int32(s, lq, rs.max_len))
-            return fail(GSW_ERR_OVERFLOW, "score range of query %d exceeds the int32 kernels", q);
-        bool use16 = dtype_policy != GSW_DTYPE_INT32 && fits_int16(s, lq, rs.max_len);
-        if (lq > wmax) {
-            plain_rows.push_back(a);
-            plain_q.push_back(q);
-            continue;
-        }
-        const Variant *v = pick_variant(lq, mean_lr, use16, local, s.ns, (size_t)ctx->max_smem, false);
-        if (!v) {
-            plain_rows.push_back(a);
-            plain_q.push_back(q);
-            continue;
-        }
-        const int kstar = (lq - 1) % v->R;
-        (use16 ? packed_q : single_q).push_back(QInfo{a, q, lq, v, kstar});
+        if (qs.len[q] == 0 || rs.n_nonempty == 0) continue;
+        cq.push_back(CrossQuery{qs.h_off[q], qs.len[q], (int64_t)a * nr, q});
+    }
+    std::vector<CrossUnit> units;
+    std::vector<CrossQuery> plain_cq;
+    GSW_TRY(plan_cross_units(s, dtype_policy, cq, rs.max_len, mean_lr, (size_t)ctx->max_smem, &units, &plain_cq));
+    std::vector<int32_t> plain_rows, plain_q;
+    for (const CrossQuery &x : plain_cq) {
+        plain_rows.push_back((int32_t)(x.out / nr));
+        plain_q.push_back(x.id);
     }
 
-    // -- pair packed queries of the same variant (and same kstar when the last column is tracked)
-    const bool need_kstar = s.mode == GSW_MODE_SEMIGLOBAL && s.free_q;
-    auto key = [&](const QInfo &x) { return std::make_tuple(x.v->id, need_kstar ? x.kstar : 0, x.lq); };
-    std::stable_sort(packed_q.begin(), packed_q.end(), [&](const QInfo &x, const QInfo &y) { return key(x) < key(y); });
-    std::stable_sort(single_q.begin(), single_q.end(), [&](const QInfo &x, const QInfo &y) { return key(x) < key(y); });
-    struct Unit { const Variant *v; int qa, qb, sa, sb, kstar; };
-    std::vector<Unit> units;
-    for (size_t i = 0; i < packed_q.size();) {
-        const QInfo &x = packed_q[i];
-        if (i + 1 < packed_q.size() && packed_q[i + 1].v == x.v &&
-            (!need_kstar || packed_q[i + 1].kstar == x.kstar)) {
-            const QInfo &y = packed_q[i + 1];
-            units.push_back(Unit{x.v, x.q, y.q, x.slot, y.slot, x.kstar});
-            i += 2;
-        } else {
-            units.push_back(Unit{x.v, x.q, -1, x.slot, -1, x.kstar});
-            i += 1;
-        }
-    }
-    for (const QInfo &x : single_q) units.push_back(Unit{x.v, x.q, -1, x.slot, -1, x.kstar});
-
-    // -- launch one persistent kernel per variant
+    // -- one persistent launch per tile, all queued back to back: the items of every
+    //    tile are uploaded once and each launch has its own work counter
     SatBuffers sb;
     const int32_t sat_cap = 1 << 20;
     GSW_TRY(prepare_sat(ctx, &sb, sat_cap));
-    GSW_HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+    struct Group { size_t u0, u1, i0, i1; int grid; size_t lds; };
+    std::vector<Group> groups;
     std::vector<CrossItem> items;
-    size_t u0 = 0;
     int dominant = -1;
     int64_t dominant_units = 0;
-    while (u0 < units.size()) {
+    for (size_t u0 = 0; u0 < units.size();) {
         size_t u1 = u0;
         while (u1 < units.size() && units[u1].v == units[u0].v) ++u1;
         const Variant &v = *units[u0].v;
@@ -474,43 +447,43 @@ int batch_cross(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
             dominant_units = (int64_t)(u1 - u0);
             dominant = v.id;
         }
-        const size_t lds = cross_lds_bytes(v, s.ns);
-        int grid = 0;
-        GSW_TRY(occupancy_grid(ctx, v.cross_fn, CROSS_WAVES * 64, lds, &grid));
+        Group g{u0, u1, items.size(), 0, 0, cross_lds_bytes(v, s.ns)};
+        GSW_TRY(occupancy_grid(ctx, v.cross_fn, CROSS_WAVES * 64, g.lds, &g.grid));
         const int gpw = 64 / v.G;
         const int64_t nref = rs.n_nonempty;
-        const int64_t units_here = (int64_t)(u1 - u0);
         // chunk the reference range so that there are >= 4 items per workgroup
-        int64_t chunk = (nref * units_here + 4LL * grid - 1) / (4LL * grid);
+        int64_t chunk = (nref * (int64_t)(u1 - u0) + 4LL * g.grid - 1) / (4LL * g.grid);
         chunk = std::max<int64_t>(chunk, (int64_t)gpw * CROSS_WAVES);
         chunk = std::min<int64_t>(chunk, nref);
         chunk = (chunk + gpw - 1) / gpw * gpw;
-        items.clear();
-        for (size_t u = u0; u < u1; ++u) {
-            for (int64_t rb = 0; rb < nref; rb += chunk) {
-                items.push_back(CrossItem{units[u].qa, units[u].qb, units[u].sa, units[u].sb, (int32_t)rb,
-                                          (int32_t)std::min(nref, rb + chunk), units[u].kstar, 0});
-            }
-        }
+        expand_cross_items(units.data() + u0, units.data() + u1, nref, chunk, &items);
+        g.i1 = items.size();
+        groups.push_back(g);
+        u0 = u1;
+    }
+    if (groups.size() > 60) return fail(GSW_ERR_LIMIT, "too many tile groups (%zu)", groups.size());
+    GSW_HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+    if (!groups.empty()) {
         GSW_TRY(ensure_buffer(&ctx->meta2, &ctx->meta2_bytes, sizeof(CrossItem) * items.size()));
         GSW_HIP_TRY(hipMemcpyAsync(ctx->meta2, items.data(), sizeof(CrossItem) * items.size(),
                                    hipMemcpyHostToDevice, ctx->stream));
-        GSW_HIP_TRY(hipMemsetAsync(ctx->counters, 0, sizeof(int32_t), ctx->stream));
+        GSW_HIP_TRY(hipMemsetAsync(ctx->counters + 2, 0, sizeof(int32_t) * groups.size(), ctx->stream));
+    }
+    for (size_t gi = 0; gi < groups.size(); ++gi) {
+        const Group &g = groups[gi];
+        const Variant &v = *units[g.u0].v;
         KArgs a = base_args(s, qs, rs, d_out);
         a.ldo = nr;
         a.sat = sb.d_sat;
         a.sat_count = ctx->counters + 1;
         a.sat_cap = sb.cap;
         a.sat_check = (v.packed && local) ? 1 : 0;
-        a.items = ctx->meta2;
-        a.nitems = (int32_t)items.size();
-        a.counter = ctx->counters;
-        v.cross(a, std::min<int>(grid, (int)items.size()), lds, ctx->stream);
+        a.items = static_cast<const CrossItem *>(ctx->meta2) + g.i0;
+        a.nitems = (int32_t)(g.i1 - g.i0);
+        a.counter = ctx->counters + 2 + gi;
+        v.cross(a, std::min<int>(g.grid, a.nitems), g.lds, ctx->stream);
         GSW_HIP_TRY(hipGetLastError());
         res->launches++;
-        // the item list is reused by the next variant: wait before overwriting it
-        GSW_HIP_TRY(hipStreamSynchronize(ctx->stream));
-        u0 = u1;
     }
     GSW_HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
     GSW_HIP_TRY(hipEventSynchronize(ctx->ev1));
@@ -541,15 +514,15 @@ int batch_cross(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
     // -- edges and saturation rescoring
     GSW_TRY(scatter_values(ctx, d_out, e_idx, e_val));
     bool overflowed = false;
-    GSW_TRY(rescore_saturated(ctx, s, sb, qs, rs, d_out, res, &overflowed));
+    GSW_TRY(rescore_saturated(ctx, s, sb, qs.d_codes, rs, d_out, res, &overflowed));
     if (overflowed) {
         // too many saturated pairs to list: rescore every batched int16 row in int32
         std::vector<int32_t> rows, ql;
         int maxq = 1;
-        for (const Unit &u : units) {
+        for (const CrossUnit &u : units) {
             if (!u.v->packed) continue;
-            rows.push_back(u.sa); ql.push_back(u.qa); maxq = std::max(maxq, qs.len[u.qa]);
-            if (u.qb >= 0) { rows.push_back(u.sb); ql.push_back(u.qb); maxq = std::max(maxq, qs.len[u.qb]); }
+            rows.push_back((int32_t)(u.a.out / nr)); ql.push_back(u.a.id); maxq = std::max(maxq, u.a.len);
+            if (u.has_b) { rows.push_back((int32_t)(u.b.out / nr)); ql.push_back(u.b.id); maxq = std::max(maxq, u.b.len); }
         }
         PlainJob job{};
         job.qcodes = qs.d_codes; job.qoff = qs.d_off; job.rcodes = rs.d_codes; job.roff = rs.d_off;
@@ -706,7 +679,7 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
     GSW_TRY(scatter_values(ctx, d_out, e_idx, e_val));
 
     bool overflowed = false;
-    GSW_TRY(rescore_saturated(ctx, s, sb, qs, rs, d_out, res, &overflowed));
+    GSW_TRY(rescore_saturated(ctx, s, sb, qs.d_codes, rs, d_out, res, &overflowed));
     if (overflowed)
         return fail(GSW_ERR_LIMIT, "more than %d saturated int16 pairs; use dtype='int32'", sb.cap);
     return GSW_OK;

@@ -10,6 +10,7 @@
 #include <chrono>
 #include <climits>
 
+#include "gsw_batch.h"
 #include "gsw_internal.h"
 
 namespace gsw {
@@ -81,6 +82,13 @@ topk_rows_kernel(const int32_t *__restrict__ scores, int32_t ncols, int32_t k,
     }
 }
 
+void topk_launch(const int32_t *d_scores, int32_t nrows, int32_t ncols, int32_t k, int32_t *d_idx,
+                 int32_t *d_sc, hipStream_t stream) {
+    if (nrows <= 0 || k <= 0) return;
+    hipLaunchKernelGGL(topk_rows_kernel, dim3(nrows), dim3(TOPK_THREADS), 0, stream, d_scores, ncols, k, d_idx,
+                       d_sc);
+}
+
 }  // namespace gsw
 
 using namespace gsw;
@@ -100,8 +108,7 @@ extern "C" int gsw_topk_rows(const int32_t *d_scores, int32_t nrows, int32_t nco
         int32_t *d_idx = static_cast<int32_t *>(ctx->meta);
         int32_t *d_sc = d_idx + (size_t)nrows * k;
         GSW_HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
-        hipLaunchKernelGGL(topk_rows_kernel, dim3(nrows), dim3(TOPK_THREADS), 0, ctx->stream, d_scores, ncols, k,
-                           d_idx, d_sc);
+        topk_launch(d_scores, nrows, ncols, k, d_idx, d_sc, ctx->stream);
         GSW_HIP_TRY(hipGetLastError());
         GSW_HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
         GSW_HIP_TRY(hipMemcpyAsync(h_idx, d_idx, bytes, hipMemcpyDeviceToHost, ctx->stream));

@@ -32,6 +32,7 @@ extern "C" {
 #define GSW_ERR_LIMIT     4  /* size limit exceeded: GpuSWError                 */
 #define GSW_ERR_OVERFLOW  5  /* score range exceeds int32: OverflowRiskError    */
 #define GSW_ERR_BUSY      6  /* server admission refused (memory budget)        */
+#define GSW_ERR_TIMEOUT   7  /* server request not finished within the timeout  */
 
 /* ---- scheme ------------------------------------------------------------- */
 #define GSW_MODE_LOCAL      0
@@ -80,6 +81,7 @@ int gsw_free(void *dptr);
 int gsw_memcpy_h2d(void *dst, const void *src, size_t bytes);   /* Backend.upload   */
 int gsw_memcpy_d2h(void *dst, const void *src, size_t bytes);   /* Backend.as_host  */
 int gsw_memset(void *dptr, int32_t value, size_t bytes);
+int gsw_memcpy_d2d(void *dst, const void *src, size_t bytes);  /* device copy (profiling calibration) */
 int gsw_synchronize(void);
 
 /* ---- plain comparator: one thread per pair ------------------------------
@@ -132,6 +134,43 @@ int gsw_score_pairs(const gsw_scheme *s, int32_t dtype_policy,
  * (src/gpusw/result.py:37).  Host outputs h_idx/h_sc are (nrows, k). */
 int gsw_topk_rows(const int32_t *d_scores, int32_t nrows, int32_t ncols, int32_t k,
                   int32_t *h_idx, int32_t *h_sc, gsw_stats *stats);
+
+/* ---- serving -------------------------------------------------------------
+ * No counterpart in the reference (its Aligner.top_k, src/gpusw/aligner.py:169-195,
+ * is a synchronous batch loop).  A server keeps reference databases resident, admits
+ * top-k requests by a device-memory budget (FIFO; a request larger than the whole
+ * budget is rejected with GSW_ERR_BUSY), batches the work of all admitted requests of
+ * a database into shared persistent-kernel launches, and returns per request the same
+ * top-k as Aligner.top_k (score descending, ties to the lower reference index). */
+typedef struct gsw_server_s *gsw_server;
+
+typedef struct gsw_request_info {
+    double  submit_ms, admit_ms, done_ms;   /* since server creation                 */
+    double  latency_ms;                     /* done - submit                         */
+    int64_t cells;                          /* sum(query lengths) * sum(ref lengths) */
+    int32_t nq, k, launches, status;        /* k actually returned (min(k, n_refs))  */
+} gsw_request_info;
+
+typedef struct gsw_server_status {
+    int64_t submitted, completed, rejected, launches, cells, rescored;
+    int32_t queued, active;
+    size_t  memory_budget, memory_used;
+    double  uptime_ms;
+} gsw_server_status;
+
+/* launch_cells: target cell updates per batched launch (<= 0: default 2e11). */
+int gsw_server_create(size_t mem_budget_bytes, double launch_cells, gsw_server *out);
+/* The database's scheme is fixed at registration; refs must outlive the server. */
+int gsw_server_add_db(gsw_server srv, const gsw_scheme *s, gsw_seqset refs, int32_t *db_id);
+/* Copies the request's queries (codes indexed by h_off[0..nq]); returns immediately. */
+int gsw_server_submit(gsw_server srv, int32_t db_id, const uint8_t *h_codes, const int32_t *h_off,
+                      int32_t nq, int32_t k, int64_t *req_id);
+/* Waits (timeout_ms < 0: forever) and copies (nq, info.k) results; consumes the id. */
+int gsw_server_wait(gsw_server srv, int64_t req_id, int32_t timeout_ms, int32_t *h_idx,
+                    int32_t *h_sc, gsw_request_info *info);
+int gsw_server_stats(gsw_server srv, gsw_server_status *st);
+/* Finishes all submitted requests, then frees the server. */
+int gsw_server_destroy(gsw_server srv);
 
 #ifdef __cplusplus
 }

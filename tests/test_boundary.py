@@ -62,3 +62,11 @@ def test_no_gpu_raises_no_gpu_error():
 def test_aligner_is_gpu_free_until_scoring():
     al = gpusw_amd.Aligner("dna")
     assert al.gcups == 0.0 and al.query_ids == []
+
+
+@pytest.mark.skipif(Path("/dev/kfd").exists(), reason="a GPU is present")
+def test_server_without_gpu_raises():
+    from gpusw_amd.serve import Server
+
+    with pytest.raises(NoGpuError):
+        Server()
