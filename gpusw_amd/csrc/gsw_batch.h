@@ -47,6 +47,7 @@ struct KArgs {
     const void *items;
     int32_t nitems;
     int32_t *counter;
+    const int32_t *nitems_ptr;   // pairs kernel: task count read from device memory
 };
 
 

@@ -76,6 +76,7 @@ struct SeqSet {
     int32_t max_len = 0;
     int64_t total = 0;
     std::vector<int32_t> order_desc;   // indices sorted by length, longest first
+    std::vector<int32_t> rank;         // rank[i] = position of sequence i in order_desc
     int32_t *d_order_desc = nullptr;   // device copy of order_desc (non-empty only)
     int32_t n_nonempty = 0;
     int device = -1;

@@ -281,6 +281,8 @@ int gsw_seqset_create(const uint8_t *d_codes, const int32_t *d_off, const int32_
     std::iota(set->order_desc.begin(), set->order_desc.end(), 0);
     std::stable_sort(set->order_desc.begin(), set->order_desc.end(),
                      [&](int a, int b) { return set->len[a] > set->len[b]; });
+    set->rank.resize(n);
+    for (int i = 0; i < n; ++i) set->rank[set->order_desc[i]] = i;
     set->n_nonempty = 0;
     while (set->n_nonempty < n && set->len[set->order_desc[set->n_nonempty]] > 0) set->n_nonempty++;
     if (n > 0) {

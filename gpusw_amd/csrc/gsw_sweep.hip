@@ -5,7 +5,10 @@
 #include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <map>
+#include <memory>
 #include <numeric>
+#include <tuple>
 
 #include "gsw_internal.h"
 #include "gsw_sweep.h"
@@ -97,7 +100,8 @@ __global__ void __launch_bounds__(WAVES * 64) pairs_kernel(KArgs a) {
     const int group_bytes = a.ns * KC * G * 16;
     char *glds = lds + (size_t)wave * GPW * group_bytes + (size_t)g * group_bytes;
     const PairTask *tasks = static_cast<const PairTask *>(a.items);
-    const int nblocks = (a.nitems + GPW - 1) / GPW;
+    const int nitems = a.nitems_ptr ? __builtin_amdgcn_readfirstlane(*a.nitems_ptr) : a.nitems;
+    const int nblocks = (nitems + GPW - 1) / GPW;
     for (;;) {
         int blk = 0;
         if (lane == 0) blk = atomicAdd(a.counter, 1);
@@ -105,7 +109,7 @@ __global__ void __launch_bounds__(WAVES * 64) pairs_kernel(KArgs a) {
         if (blk >= nblocks) break;
         const int ti = blk * GPW + g;
         PairTask task{-1, -1, -1, 0, -1, -1};
-        if (ti < a.nitems) task = tasks[ti];
+        if (ti < nitems) task = tasks[ti];
         const int kstar = __builtin_amdgcn_readfirstlane(tasks[blk * GPW].kstar);
         const bool valid = task.ref >= 0;
         const int qa = valid ? task.qa : 0;
@@ -541,6 +545,70 @@ int batch_cross(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
 }
 
 // ---------------------------------------------------------------- pairs
+// Explicit pairs run in batches of PAIR_BATCH pairs.  Each batch is planned on the host
+// in O(n) (counting sorts by reference length rank, then by last-column register) into a
+// pinned buffer, and replayed through a HIP graph [copy header+tasks -> persistent pairs
+// kernel]; the header carries the task count and a zeroed work counter, so the graph
+// needs no memset node.  Two graph slots alternate so that planning batch b+1 overlaps
+// the device work of batch b.
+constexpr int64_t PAIR_BATCH = 131072;
+constexpr size_t PAIR_HDR = 64;
+
+struct PairGraph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipGraphNode_t copy = nullptr, kern = nullptr;
+    char *d_buf = nullptr, *h_buf = nullptr;
+    size_t cap = 0;
+    hipEvent_t done = nullptr;
+    int grid = 0;
+    size_t lds = 0;
+};
+
+static std::mutex g_graph_mu;
+static std::map<std::tuple<int, int, int>, std::unique_ptr<PairGraph>> g_pair_graphs;
+
+static int pair_graph(DeviceCtx *ctx, const Variant &v, int slot, int ns, PairGraph **out) {
+    std::lock_guard<std::mutex> lk(g_graph_mu);
+    auto key = std::make_tuple(ctx->device, v.id, slot);
+    auto &pg = g_pair_graphs[key];
+    const size_t lds = pairs_lds_bytes(v, ns);
+    if (pg && pg->lds == lds) {
+        *out = pg.get();
+        return GSW_OK;
+    }
+    if (pg) {   // table size changed: rebuild
+        (void)hipGraphExecDestroy(pg->exec);
+        (void)hipGraphDestroy(pg->graph);
+    } else {
+        pg = std::make_unique<PairGraph>();
+        const int gpw = 64 / v.G;
+        pg->cap = PAIR_HDR + sizeof(PairTask) * (size_t)(PAIR_BATCH + 64 * gpw + gpw);
+        GSW_HIP_TRY(hipMalloc(&pg->d_buf, pg->cap));
+        GSW_HIP_TRY(hipHostMalloc(&pg->h_buf, pg->cap, hipHostMallocDefault));
+        std::memset(pg->h_buf, 0, PAIR_HDR);
+        GSW_HIP_TRY(hipEventCreateWithFlags(&pg->done, hipEventDisableTiming));
+    }
+    pg->lds = lds;
+    GSW_TRY(occupancy_grid(ctx, v.pairs_fn, PAIRS_WAVES * 64, lds, &pg->grid));
+    GSW_HIP_TRY(hipGraphCreate(&pg->graph, 0));
+    GSW_HIP_TRY(hipGraphAddMemcpyNode1D(&pg->copy, pg->graph, nullptr, 0, pg->d_buf, pg->h_buf, pg->cap,
+                                        hipMemcpyHostToDevice));
+    KArgs a{};
+    void *params[] = {&a};
+    hipKernelNodeParams kp{};
+    kp.func = const_cast<void *>(v.pairs_fn);
+    kp.gridDim = dim3(pg->grid);
+    kp.blockDim = dim3(PAIRS_WAVES * 64);
+    kp.sharedMemBytes = (unsigned)lds;
+    kp.kernelParams = params;
+    kp.extra = nullptr;
+    GSW_HIP_TRY(hipGraphAddKernelNode(&pg->kern, pg->graph, &pg->copy, 1, &kp));
+    GSW_HIP_TRY(hipGraphInstantiate(&pg->exec, pg->graph, nullptr, nullptr, 0));
+    *out = pg.get();
+    return GSW_OK;
+}
+
 int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqSet &qs,
                 const SeqSet &rs, const int32_t *h_qi, const int32_t *h_rj, int64_t n,
                 int32_t *d_out, BatchResult *res) {
@@ -550,14 +618,14 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
     const int wmax = max_width();
     const double mean_lr = rs.n_nonempty ? (double)rs.total / rs.n_nonempty : 1.0;
 
+    // -- classify: empty side (closed form), no tile (comparator), else a tile
     std::vector<int64_t> e_idx;
     std::vector<int32_t> e_val;
     std::vector<int32_t> p_qi, p_rj;
     std::vector<int64_t> p_oi;
-    // key: variant id (8 bits) | kstar (8) | 65535-lr (16) | ref (32)
-    std::vector<std::pair<uint64_t, int64_t>> keyed;
-    keyed.reserve(n);
+    std::vector<int16_t> vid(n, -1);
     std::vector<const Variant *> vcache(2 * (wmax + 1), nullptr);
+    std::vector<int64_t> per_variant(variant_table().size(), 0);
     for (int64_t k = 0; k < n; ++k) {
         const int q = h_qi[k], r = h_rj[k];
         if (q < 0 || q >= qs.n || r < 0 || r >= rs.n)
@@ -583,72 +651,99 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
             p_oi.push_back(k);
             continue;
         }
-        const int kstar = need_kstar ? (lq - 1) % v->R : 0;
-        const uint64_t key = ((uint64_t)v->id << 56) | ((uint64_t)kstar << 48) |
-                             ((uint64_t)(65535 - std::min(lr, 65535)) << 32) | (uint64_t)(uint32_t)r;
-        keyed.emplace_back(key, k);
+        vid[k] = (int16_t)v->id;
+        per_variant[v->id]++;
     }
-    std::sort(keyed.begin(), keyed.end());
 
     SatBuffers sb;
     GSW_TRY(prepare_sat(ctx, &sb, 1 << 20));
     GSW_HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
-    std::vector<PairTask> tasks;
-    size_t i0 = 0;
     int dominant = -1;
-    size_t dominant_n = 0;
-    while (i0 < keyed.size()) {
-        const int vid = (int)(keyed[i0].first >> 56);
-        size_t i1 = i0;
-        while (i1 < keyed.size() && (int)(keyed[i1].first >> 56) == vid) ++i1;
-        const Variant &v = variant_table()[vid];
-        if (i1 - i0 > dominant_n) { dominant_n = i1 - i0; dominant = vid; }
+    int64_t dominant_n = 0;
+    std::vector<int64_t> members, order, order2;
+    std::vector<int32_t> cnt;
+    int batch_no = 0;
+    for (const Variant &v : variant_table()) {
+        if (per_variant[v.id] == 0) continue;
+        if (per_variant[v.id] > dominant_n) { dominant_n = per_variant[v.id]; dominant = v.id; }
+        members.clear();
+        for (int64_t k = 0; k < n; ++k)
+            if (vid[k] == v.id) members.push_back(k);
         const int gpw = 64 / v.G;
-        tasks.clear();
-        // build tasks: pack two pairs that share a reference (packed variants)
-        size_t i = i0;
-        int cur_kstar = -1;
-        while (i < i1) {
-            const uint64_t key = keyed[i].first;
-            const int kstar = (int)((key >> 48) & 0xff);
-            if (kstar != cur_kstar) {
-                while (tasks.size() % gpw) tasks.push_back(PairTask{-1, -1, -1, cur_kstar, -1, -1});
-                cur_kstar = kstar;
-            }
-            const int64_t ka = keyed[i].second;
-            if (v.packed && i + 1 < i1 && keyed[i + 1].first == key) {
-                const int64_t kb = keyed[i + 1].second;
-                tasks.push_back(PairTask{h_qi[ka], h_qi[kb], h_rj[ka], kstar, ka, kb});
-                i += 2;
-            } else {
-                tasks.push_back(PairTask{h_qi[ka], -1, h_rj[ka], kstar, ka, -1});
-                i += 1;
-            }
+        PairGraph *pg[2];
+        GSW_TRY(pair_graph(ctx, v, 0, s.ns, &pg[0]));
+        GSW_TRY(pair_graph(ctx, v, 1, s.ns, &pg[1]));
+        for (int sl = 0; sl < 2; ++sl) {   // this call's pointers into the kernel node
+            KArgs a = base_args(s, qs, rs, d_out);
+            a.sat = sb.d_sat;
+            a.sat_count = ctx->counters + 1;
+            a.sat_cap = sb.cap;
+            a.sat_check = (v.packed && local) ? 1 : 0;
+            a.items = pg[sl]->d_buf + PAIR_HDR;
+            a.nitems_ptr = reinterpret_cast<const int32_t *>(pg[sl]->d_buf);
+            a.counter = reinterpret_cast<int32_t *>(pg[sl]->d_buf) + 1;
+            void *params[] = {&a};
+            hipKernelNodeParams kp{};
+            kp.func = const_cast<void *>(v.pairs_fn);
+            kp.gridDim = dim3(pg[sl]->grid);
+            kp.blockDim = dim3(PAIRS_WAVES * 64);
+            kp.sharedMemBytes = (unsigned)pg[sl]->lds;
+            kp.kernelParams = params;
+            kp.extra = nullptr;
+            GSW_HIP_TRY(hipGraphExecKernelNodeSetParams(pg[sl]->exec, pg[sl]->kern, &kp));
         }
-        while (tasks.size() % gpw) tasks.push_back(PairTask{-1, -1, -1, cur_kstar, -1, -1});
-        const size_t lds = pairs_lds_bytes(v, s.ns);
-        int grid = 0;
-        GSW_TRY(occupancy_grid(ctx, v.pairs_fn, PAIRS_WAVES * 64, lds, &grid));
-        GSW_TRY(ensure_buffer(&ctx->meta2, &ctx->meta2_bytes, sizeof(PairTask) * tasks.size()));
-        GSW_HIP_TRY(hipMemcpyAsync(ctx->meta2, tasks.data(), sizeof(PairTask) * tasks.size(),
-                                   hipMemcpyHostToDevice, ctx->stream));
-        GSW_HIP_TRY(hipMemsetAsync(ctx->counters, 0, sizeof(int32_t), ctx->stream));
-        KArgs a = base_args(s, qs, rs, d_out);
-        a.ldo = 0;
-        a.sat = sb.d_sat;
-        a.sat_count = ctx->counters + 1;
-        a.sat_cap = sb.cap;
-        a.sat_check = (v.packed && local) ? 1 : 0;
-        a.items = ctx->meta2;
-        a.nitems = (int32_t)tasks.size();
-        a.counter = ctx->counters;
-        const int64_t blocks = (int64_t)tasks.size() / gpw;
-        const int wg = (int)std::min<int64_t>(grid, (blocks + PAIRS_WAVES - 1) / PAIRS_WAVES);
-        v.pairs(a, std::max(wg, 1), lds, ctx->stream);
-        GSW_HIP_TRY(hipGetLastError());
-        res->launches++;
-        GSW_HIP_TRY(hipStreamSynchronize(ctx->stream));
-        i0 = i1;
+        for (size_t b0 = 0; b0 < members.size(); b0 += PAIR_BATCH) {
+            const size_t b1 = std::min(members.size(), b0 + (size_t)PAIR_BATCH);
+            PairGraph &g = *pg[batch_no & 1];
+            // counting sort of the batch by reference length rank (longest first) ...
+            cnt.assign((size_t)rs.n_nonempty + 1, 0);
+            for (size_t i = b0; i < b1; ++i) cnt[rs.rank[h_rj[members[i]]] + 1]++;
+            for (int32_t r = 0; r < rs.n_nonempty; ++r) cnt[r + 1] += cnt[r];
+            order.resize(b1 - b0);
+            for (size_t i = b0; i < b1; ++i) order[cnt[rs.rank[h_rj[members[i]]]]++] = members[i];
+            // ... then (stable) by last-column register when it is tracked
+            if (need_kstar) {
+                cnt.assign(v.R + 1, 0);
+                for (int64_t k : order) cnt[(qs.len[h_qi[k]] - 1) % v.R + 1]++;
+                for (int r = 0; r < v.R; ++r) cnt[r + 1] += cnt[r];
+                order2.resize(order.size());
+                for (int64_t k : order) order2[cnt[(qs.len[h_qi[k]] - 1) % v.R]++] = k;
+                order.swap(order2);
+            }
+            // the pinned buffer of this slot is free once its previous replay finished
+            GSW_HIP_TRY(hipEventSynchronize(g.done));
+            PairTask *tasks = reinterpret_cast<PairTask *>(g.h_buf + PAIR_HDR);
+            int32_t nt = 0;
+            int cur_kstar = -1;
+            for (size_t i = 0; i < order.size();) {
+                const int64_t ka = order[i];
+                const int kstar = need_kstar ? (qs.len[h_qi[ka]] - 1) % v.R : 0;
+                if (kstar != cur_kstar) {
+                    while (nt % gpw) tasks[nt++] = PairTask{-1, -1, -1, cur_kstar, -1, -1};
+                    cur_kstar = kstar;
+                }
+                if (v.packed && i + 1 < order.size() && h_rj[order[i + 1]] == h_rj[ka] &&
+                    (!need_kstar || (qs.len[h_qi[order[i + 1]]] - 1) % v.R == kstar)) {
+                    const int64_t kb = order[i + 1];
+                    tasks[nt++] = PairTask{h_qi[ka], h_qi[kb], h_rj[ka], kstar, ka, kb};
+                    i += 2;
+                } else {
+                    tasks[nt++] = PairTask{h_qi[ka], -1, h_rj[ka], kstar, ka, -1};
+                    i += 1;
+                }
+            }
+            while (nt % gpw) tasks[nt++] = PairTask{-1, -1, -1, cur_kstar, -1, -1};
+            int32_t *hdr = reinterpret_cast<int32_t *>(g.h_buf);
+            hdr[0] = nt;   // task count
+            hdr[1] = 0;    // work counter
+            GSW_HIP_TRY(hipGraphExecMemcpyNodeSetParams1D(g.exec, g.copy, g.d_buf, g.h_buf,
+                                                          PAIR_HDR + sizeof(PairTask) * (size_t)nt,
+                                                          hipMemcpyHostToDevice));
+            GSW_HIP_TRY(hipGraphLaunch(g.exec, ctx->stream));
+            GSW_HIP_TRY(hipEventRecord(g.done, ctx->stream));
+            res->launches++;
+            ++batch_no;
+        }
     }
     GSW_HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
     GSW_HIP_TRY(hipEventSynchronize(ctx->ev1));
