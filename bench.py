@@ -14,7 +14,10 @@ before the timed region.  Also reported:
 * ``cpu_baseline``: the oracle's C restatement on the host cores over a bounded sample
   of the same pairs (those pairs are also checked bit-exact against the GPU scores).
 
-``--workload`` selects another configuration (protein_topk, read_pairs) for extra lines.
+``--workload`` selects another configuration for extra lines: ``protein_topk`` (BLOSUM62
+top-5 of 1,000 queries against 100,000 references), ``read_pairs`` (1,000,000 explicit
+read/window pairs, semiglobal then global, graph-replayed batches), ``serving`` (Poisson
+request mix through the continuous-batching server, p50/p99 latency).
 """
 from __future__ import annotations
 
@@ -180,6 +183,170 @@ def run_dna_cross(args):
     return line
 
 
+def _plain_subsample_cross(al, scheme, qsel):
+    """GCUPS of the one-thread-per-pair kernel on a query subsample (same GPU)."""
+    import gpusw_amd
+
+    pb = gpusw_amd.backends.resolve_backend("hip-plain")
+    nr = al._refs.n
+    sc, dt, _ = pb.run_cross(scheme, al._queries.max_len, "int32", 128, al._d_qbuf, al._d_qoff,
+                             al._d_rbuf, al._d_roff, qsel, len(qsel), nr)
+    cells = int(al._queries.lengths[qsel].astype(np.int64).sum()) * int(al._refs.lengths.astype(np.int64).sum())
+    return sc.to_host(), cells / dt / 1e9
+
+
+def run_protein_topk(args):
+    """Protein top-k: 1,000 lognormal queries vs 100,000 lognormal references, BLOSUM62
+    local -11/-1, device top_k(5) per query batch of 512 (the Aligner.top_k loop)."""
+    import gpusw_amd
+    from gpusw_amd import Aligner, Scheme, synth
+
+    (qc, qo), (rc, ro) = synth.protein_topk(seed=0)
+    scheme = Scheme.preset("blosum62")
+    al = Aligner(scheme, backend="hip").index((rc, ro)).set_queries((qc, qo))
+    nq, nr = al._queries.n, al._refs.n
+    cells = int(al._queries.lengths.astype(np.int64).sum()) * int(al._refs.lengths.astype(np.int64).sum())
+    log(f"protein top-k: {nq} x {nr}, {cells:.3e} cells")
+    for _ in range(args.warmup):
+        al.top_k(k=5)
+    gpusw_amd._hip.call("gsw_synchronize")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        res = al.top_k(k=5)
+    wall = (time.perf_counter() - t0) / args.steps
+    kernel_s = al._last_seconds
+    # checks: top-k of a query subsample against the comparator's full rows
+    rng = np.random.default_rng(2)
+    qsel = np.sort(rng.choice(nq, 8, replace=False)).astype(np.int32)
+    prow, plain_gcups = _plain_subsample_cross(al, scheme, qsel)
+    order = np.argsort(-prow, axis=1, kind="stable")[:, :5]
+    ok = bool(np.array_equal(order, res.topk_idx[qsel]) and
+              np.array_equal(np.take_along_axis(prow, order, 1), res.topk_scores[qsel]))
+    value = cells / wall / 1e9
+    return {
+        "metric": METRIC, "value": round(value, 2), "unit": "GCUPS", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(wall * 1e3, 2), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "int16", "data": "synthetic",
+        "config": {"workload": "protein top-k: 1000 queries (lognormal median 350, 30-2000) x 100000 "
+                               "references (lognormal median 300, 30-3000), BLOSUM62 local -11/-1, top_k(5)",
+                   "cells": cells},
+        "roofline": {"bound": "valu", "achieved": round(cells * 10 / kernel_s / 1e12, 3), "peak": PEAK_INT16_TOPS,
+                     "unit": "TOP/s (int16)", "frac": round(cells * 10 / kernel_s / 1e12 / PEAK_INT16_TOPS, 4),
+                     "traffic": None, "kernel_ms": round(kernel_s * 1e3, 2)},
+        "plain": {"gcups": round(plain_gcups, 2), "queries": int(len(qsel)), "speedup": round(value / plain_gcups, 2)},
+        "checks": {"topk_equals_comparator_rows": ok},
+    }
+
+
+def run_read_pairs(args):
+    """1,000,000 explicit (read, window) pairs, dna scheme, semiglobal then global, int16."""
+    import gpusw_amd
+    from gpusw_amd import Aligner, Scheme, synth
+    from oracle import native
+
+    (rc, ro), (wc, wo), qi, rj = synth.read_pairs(seed=0)
+    out = []
+    for mode in ("semiglobal", "global"):
+        scheme = Scheme.preset("dna").replace(mode=mode, dtype="int16")
+        al = Aligner(scheme, backend="hip").index((wc, wo)).set_queries((rc, ro))
+        cells = int((al._queries.lengths[qi].astype(np.int64) * al._refs.lengths[rj].astype(np.int64)).sum())
+        for _ in range(args.warmup):
+            al.score_pairs(qi, rj)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            got = al.score_pairs(qi, rj)
+        wall = (time.perf_counter() - t0) / args.steps
+        st = al._b.last_stats
+        rng = np.random.default_rng(3)
+        pick = rng.choice(qi.size, 20000, replace=False)
+        want = native.score_encoded_pairs(al._queries, al._refs, qi[pick], rj[pick], scheme)
+        mism = int((got[pick].astype(np.int64) != want).sum())
+        sub = rng.choice(qi.size, 50000, replace=False)
+        pb = gpusw_amd.backends.resolve_backend("hip-plain")
+        ph, pdt, _ = pb.run_pairs(scheme, al._queries.max_len, "int16", 128, al._d_qbuf, al._d_qoff,
+                                  al._d_rbuf, al._d_roff, qi[sub], rj[sub], sub.size)
+        pcells = int((al._queries.lengths[qi[sub]].astype(np.int64) * al._refs.lengths[rj[sub]]).sum())
+        value = cells / wall / 1e9
+        out.append({
+            "metric": METRIC, "value": round(value, 2), "unit": "GCUPS", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": round(wall * 1e3, 2), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "int16", "data": "synthetic",
+            "config": {"workload": f"read pairs: 1,000,000 (150-nt read, 200-400-nt window) pairs, dna "
+                                   f"{mode}, int16, HIP-graph batches of 131,072", "cells": cells},
+            "roofline": {"bound": "valu", "achieved": round(cells * 8 / (st.kernel_ms / 1e3) / 1e12, 3),
+                         "peak": PEAK_INT16_TOPS, "unit": "TOP/s (int16)",
+                         "frac": round(cells * 8 / (st.kernel_ms / 1e3) / 1e12 / PEAK_INT16_TOPS, 4),
+                         "traffic": None, "kernel_ms": round(st.kernel_ms, 2), "host_wall_ms": round(st.wall_ms, 2),
+                         "launches": int(st.launches)},
+            "plain": {"gcups": round(pcells / pdt / 1e9, 2), "pairs": int(sub.size),
+                      "speedup": round(value / (pcells / pdt / 1e9), 2),
+                      "equal_to_batched": bool(np.array_equal(ph, got[sub]))},
+            "checks": {"oracle_mismatches": mism, "oracle_sample": int(pick.size)},
+        })
+    return out
+
+
+def run_serving(args):
+    """Serving mix: Poisson arrivals of requests holding 1-512 queries (log-uniform count),
+    70% DNA against the 16S-shape database and 30% protein against the protein database,
+    top_k(5); continuous batching under a device-memory budget.  Reports per offered rate
+    the achieved throughput and p50/p99 latency."""
+    from gpusw_amd import Scheme, synth
+    from gpusw_amd.serve import Server
+
+    (dqc, dqo), (drc, dro) = synth.dna_16s_cross(seed=0)
+    (pqc, pqo), (prc, pro) = synth.protein_topk(seed=0)
+    dq = synth.unpack(dqc, dqo)
+    pq = synth.unpack(pqc, pqo)
+    rates = [float(x) for x in args.rates.split(",")]
+    lines = []
+    with Server(memory_budget=args.budget << 30, launch_cells=args.launch_cells) as srv:
+        d_dna = srv.add_database((drc, dro), Scheme.preset("dna"))
+        d_prot = srv.add_database((prc, pro), Scheme.preset("blosum62"))
+        for rate in rates:
+            rng = np.random.default_rng(int(rate * 1000) + 7)
+            n = args.requests
+            gaps = rng.exponential(1.0 / rate, n)
+            sizes = np.floor(np.exp(rng.uniform(np.log(1), np.log(513), n))).astype(int)
+            kinds = rng.random(n) < 0.7
+            reqs = []
+            for i in range(n):
+                src = dq if kinds[i] else pq
+                pick = rng.integers(0, len(src), sizes[i])
+                seqs = [src[j] for j in pick]
+                reqs.append((d_dna if kinds[i] else d_prot, synth.pack(seqs)))
+            t0 = time.perf_counter()
+            ids = []
+            next_t = t0
+            for i, (db, q) in enumerate(reqs):
+                next_t += gaps[i]
+                delay = next_t - time.perf_counter()
+                if delay > 0:
+                    time.sleep(delay)
+                ids.append(srv.submit(db, q, k=5))
+            offered_s = time.perf_counter() - t0
+            res = [srv.result(r) for r in ids]
+            wall = time.perf_counter() - t0
+            lat = np.array([r.latency_ms for r in res])
+            cells = sum(r.cells for r in res)
+            lines.append({
+                "metric": METRIC, "value": round(cells / wall / 1e9, 2), "unit": "GCUPS", "n_gpus": 1,
+                "steps": 1, "warmup": 0, "ms_per_step": round(wall * 1e3, 2), "higher_is_better": True,
+                "scaling": "weak", "vs_baseline": None, "dtype": "int16", "data": "synthetic",
+                "config": {"workload": "serving mix: Poisson arrivals, 1-512 queries per request (log-uniform), "
+                                       "70% DNA vs 16S-shape DB (20,000 refs), 30% protein vs 100,000-ref DB, top_k(5)",
+                           "offered_rate_rps": rate, "requests": n,
+                           "achieved_rate_rps": round(n / wall, 3), "offered_window_s": round(offered_s, 3),
+                           "memory_budget_gib": args.budget, "launch_cells": args.launch_cells},
+                "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 2),
+                               "p99": round(float(np.percentile(lat, 99)), 2),
+                               "mean": round(float(lat.mean()), 2), "max": round(float(lat.max()), 2)},
+            })
+            log(f"serving rate {rate}: {lines[-1]['value']} GCUPS, p50 {lines[-1]['latency_ms']['p50']} ms, "
+                f"p99 {lines[-1]['latency_ms']['p99']} ms, achieved {n / wall:.2f} req/s")
+    return lines
+
+
 def calibrate_copy(nbytes: int) -> dict:
     """One device-to-device copy of ``nbytes`` (the reference point that scales the
     FETCH_SIZE / WRITE_SIZE counters of the profiled kernels)."""
@@ -201,7 +368,12 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="dna_cross", choices=["dna_cross"])
+    ap.add_argument("--workload", default="dna_cross",
+                    choices=["dna_cross", "protein_topk", "read_pairs", "serving"])
+    ap.add_argument("--rates", default="2,5,50", help="serving: offered request rates (req/s)")
+    ap.add_argument("--requests", type=int, default=40, help="serving: requests per rate")
+    ap.add_argument("--budget", type=int, default=64, help="serving: device memory budget (GiB)")
+    ap.add_argument("--launch-cells", type=float, default=2e11, help="serving: cells per launch")
     ap.add_argument("--plain-queries", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
@@ -214,8 +386,16 @@ def main() -> int:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(run with --gpus 1)", file=sys.stderr)
         return 2
-    line = run_dna_cross(args)
-    print(json.dumps(line), flush=True)
+    if args.workload == "dna_cross":
+        lines = [run_dna_cross(args)]
+    elif args.workload == "protein_topk":
+        lines = [run_protein_topk(args)]
+    elif args.workload == "read_pairs":
+        lines = run_read_pairs(args)
+    else:
+        lines = run_serving(args)
+    for line in lines:
+        print(json.dumps(line), flush=True)
     return 0
 
 

@@ -21,8 +21,14 @@
 // to 32767 may have saturated and is rescored in int32 by the host.
 //
 // Query profile.  prof(c, col) = sub[c][query[col]] (packed A|B for short2), stored in
-// LDS as 16-byte chunks ordered [c][chunk][lane]: lane t's four values of chunk kc of
-// row c sit at ((c*KC + kc)*G + t)*16, so the lanes of a group read consecutive chunks.
+// LDS as 16-byte chunks; lane t's four values of chunk kc of row c sit at
+//     c*ROWB + kc*CH + slot*16,   CH = max(G, 16)*16,  ROWB = KC*CH,
+// where slot = (g mod 16/G)*G + t for G < 16 (g = group index in the wave) and slot = t
+// otherwise.  An LDS pass serves 16 lanes x 16 bytes; with this layout the 16 lanes of
+// a pass always read 16 distinct 16-byte slots of one 256-byte bank row, whatever
+// reference codes they are on, so the profile reads are free of bank conflicts.  The
+// cross kernel therefore keeps 16/G copies of its shared profile when G < 16; the pairs
+// kernel interleaves the private profiles of the 16/G groups of a pass.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -33,6 +39,16 @@ namespace gsw {
 
 typedef short s2 __attribute__((ext_vector_type(2)));
 typedef unsigned short u2 __attribute__((ext_vector_type(2)));
+
+// Profile layout constants (bytes) of a (G, R) tile.
+template <int G, int R> struct ProfLayout {
+    static constexpr int KC = (R + 3) / 4;              // 16-byte chunks per lane per row
+    static constexpr int NSLOT = G < 16 ? 16 / G : 1;   // groups sharing an LDS pass
+    static constexpr int CH = (G < 16 ? 16 : G) * 16;   // bytes between chunks kc, kc+1
+    static constexpr int ROWB = KC * CH;                // bytes per profile row c
+    // byte offset of lane t of group g within its profile block
+    __device__ static __forceinline__ int lane_offset(int g, int t) { return ((g % NSLOT) * G + t) * 16; }
+};
 
 template <typename V> struct VOps;
 
@@ -122,7 +138,7 @@ struct SweepOut {
 // Sweep one group's reference through the group's query profile.
 //   rseq/lr    : this group's reference codes and length (lr = 0: idle group)
 //   nsteps     : wave-uniform step count (max lr over the wave's groups + G - 1)
-//   prof_lane  : this lane's profile base (group profile base + t*16)
+//   prof_lane  : this lane's profile base (block base + ProfLayout::lane_offset)
 //   lqa/lqb    : query lengths (for padding masks and result positions)
 //   kstar      : wave-uniform (lq-1) % R of both queries, used when tracking the last column
 template <typename V, int G, int R, bool LOCAL>
@@ -130,8 +146,8 @@ __device__ __forceinline__ SweepOut sweep_group(const uint8_t *__restrict__ rseq
                                                 const char *prof_lane, int t, const SweepConst &k,
                                                 int lqa, int lqb, int kstar) {
     using O = VOps<V>;
-    constexpr int KC = (R + 3) / 4;
-    constexpr int ROWB = KC * G * 16;   // bytes per profile row
+    using L = ProfLayout<G, R>;
+    constexpr int KC = L::KC;
     const bool semi = k.mode == 2;
     const bool track_endcol = semi && k.free_q;
 
@@ -163,11 +179,11 @@ __device__ __forceinline__ SweepOut sweep_group(const uint8_t *__restrict__ rseq
         const int row = s - t;
         if ((unsigned)row < (unsigned)lr) {
             const int c = min((int)rseq[row], k.ns - 1);   // out-of-range codes score as the sentinel
-            const char *pc = prof_lane + c * ROWB;
+            const char *pc = prof_lane + c * L::ROWB;
             V pv[KC * 4];
 #pragma unroll
             for (int kc = 0; kc < KC; ++kc) {
-                const int4 x = *reinterpret_cast<const int4 *>(pc + kc * G * 16);
+                const int4 x = *reinterpret_cast<const int4 *>(pc + kc * L::CH);
                 pv[4 * kc + 0] = O::from_bits(x.x);
                 pv[4 * kc + 1] = O::from_bits(x.y);
                 pv[4 * kc + 2] = O::from_bits(x.z);

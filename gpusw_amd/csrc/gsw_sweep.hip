@@ -31,8 +31,8 @@ __device__ __forceinline__ void emit(const KArgs &a, int64_t o, int qs, int ql, 
 template <typename V, int G, int R, bool LOCAL, int WAVES>
 __global__ void __launch_bounds__(WAVES * 64) cross_kernel(KArgs a) {
     using O = VOps<V>;
-    constexpr int KC = (R + 3) / 4;
-    constexpr int LW = KC * 4;   // padded columns per lane
+    using L = ProfLayout<G, R>;
+    constexpr int LW = L::KC * 4;   // padded columns per lane
     constexpr int GPW = 64 / G;
     extern __shared__ __align__(16) char lds[];
     __shared__ int s_item, s_blk;
@@ -58,11 +58,14 @@ __global__ void __launch_bounds__(WAVES * 64) cross_kernel(KArgs a) {
             const int col = lt * R + kk;
             const int va = (kk < R && col < lqa) ? a.sub[c * a.ns + min((int)a.qcodes[qsa + col], a.ns - 1)] : O::NEG;
             const int vb = (kk < R && col < lqb) ? a.sub[c * a.ns + min((int)a.qcodes[qsb + col], a.ns - 1)] : O::NEG;
-            *reinterpret_cast<uint32_t *>(lds + ((c * KC + (kk >> 2)) * G + lt) * 16 + (kk & 3) * 4) =
-                O::pack(va, vb);
+            const uint32_t word = O::pack(va, vb);
+            char *dst = lds + c * L::ROWB + (kk >> 2) * L::CH + lt * 16 + (kk & 3) * 4;
+#pragma unroll
+            for (int cp = 0; cp < L::NSLOT; ++cp)   // one copy per group position in an LDS pass
+                *reinterpret_cast<uint32_t *>(dst + cp * G * 16) = word;
         }
         __syncthreads();
-        const char *prof_lane = lds + t * 16;
+        const char *prof_lane = lds + L::lane_offset(g, t);
         for (;;) {
             int blk = 0;
             if (lane == 0) blk = atomicAdd(&s_blk, 1);
@@ -92,13 +95,16 @@ __global__ void __launch_bounds__(WAVES * 64) cross_kernel(KArgs a) {
 template <typename V, int G, int R, bool LOCAL, int WAVES>
 __global__ void __launch_bounds__(WAVES * 64) pairs_kernel(KArgs a) {
     using O = VOps<V>;
-    constexpr int KC = (R + 3) / 4;
+    using L = ProfLayout<G, R>;
+    constexpr int KC = L::KC;
     constexpr int GPW = 64 / G;
     extern __shared__ __align__(16) char lds[];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int t = lane % G, g = lane / G;
-    const int group_bytes = a.ns * KC * G * 16;
-    char *glds = lds + (size_t)wave * GPW * group_bytes + (size_t)g * group_bytes;
+    // blocks of NSLOT groups interleave their private profiles (see ProfLayout)
+    const int block_bytes = a.ns * L::ROWB;
+    char *lane_lds = lds + (size_t)wave * (GPW / L::NSLOT) * block_bytes + (size_t)(g / L::NSLOT) * block_bytes +
+                     L::lane_offset(g, t);
     const PairTask *tasks = static_cast<const PairTask *>(a.items);
     const int nitems = a.nitems_ptr ? __builtin_amdgcn_readfirstlane(*a.nitems_ptr) : a.nitems;
     const int nblocks = (nitems + GPW - 1) / GPW;
@@ -132,12 +138,12 @@ __global__ void __launch_bounds__(WAVES * 64) pairs_kernel(KArgs a) {
                     const int vb = (kk < R && col < lqb) ? srow[min((int)a.qcodes[qsb + col], a.ns - 1)] : O::NEG;
                     w[e] = O::pack(va, vb);
                 }
-                *reinterpret_cast<int4 *>(glds + ((c * KC + kc) * G + t) * 16) =
+                *reinterpret_cast<int4 *>(lane_lds + c * L::ROWB + kc * L::CH) =
                     int4{(int)w[0], (int)w[1], (int)w[2], (int)w[3]};
             }
         }
         const int nsteps = __builtin_amdgcn_readfirstlane(wave_max_i32(lr)) + G - 1;
-        const SweepOut res = sweep_group<V, G, R, LOCAL>(a.rcodes + rs, lr, nsteps, glds + t * 16, t,
+        const SweepOut res = sweep_group<V, G, R, LOCAL>(a.rcodes + rs, lr, nsteps, lane_lds, t,
                                                          a.k, lqa, lqb, kstar);
         if (t == 0 && valid) {
             emit(a, task.oa, qsa, lqa, task.ref, res.a);
@@ -227,8 +233,11 @@ int max_width() {
     return w;
 }
 
+// Cross: one shared profile (16/G copies when G < 16).  Pairs: 64/G private profiles per
+// wave, interleaved in blocks of 16/G.  See ProfLayout in gsw_sweep.h.
 size_t cross_lds_bytes(const Variant &v, int ns) {
-    return (size_t)ns * ((v.R + 3) / 4) * v.G * 16;
+    const size_t ch = (size_t)std::max(v.G, 16) * 16;
+    return (size_t)ns * ((v.R + 3) / 4) * ch;
 }
 size_t pairs_lds_bytes(const Variant &v, int ns) {
     return (size_t)PAIRS_WAVES * ns * ((v.R + 3) / 4) * 64 * 16;
