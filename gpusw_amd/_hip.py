@@ -93,6 +93,7 @@ _SIGNATURES = {
     "gsw_server_wait": [_vp, _i64, _i32, _vp, _vp, ctypes.POINTER(GswRequestInfo)],
     "gsw_server_stats": [_vp, ctypes.POINTER(GswServerStatus)],
     "gsw_server_destroy": [_vp],
+    "gsw_set_tile_table": [_vp, _i32],
     "gsw_force_variant": [_i32, _i32],
     "gsw_list_variants": [_vp, _i32, ctypes.POINTER(_i32)],
 }

@@ -14,12 +14,12 @@ struct CrossItem {
     int32_t qsb, lqb;        // query B (lqb < 0: single query)
     int64_t outa, outb;      // output offset of reference 0 for A / B (outb < 0: none)
     int32_t rbeg, rend;      // range in the sorted reference order
-    int32_t kstar, pad;
+    int32_t shift, pad;
 };
 
 // Task of the pairs kernel: queries qa and qb (qb < 0: single) against reference ref.
 struct PairTask {
-    int32_t qa, qb, ref, kstar;
+    int32_t qa, qb, ref, shift;
     int64_t oa, ob;          // output indices (ob < 0: none)
 };
 
@@ -82,7 +82,7 @@ struct CrossUnit {
     const Variant *v;
     CrossQuery a, b;
     bool has_b;
-    int kstar;
+    int shift;
 };
 
 // Route queries to tiles, pairing packed int16 queries of the same tile (and of the same

@@ -140,11 +140,15 @@ struct SweepOut {
 //   nsteps     : wave-uniform step count (max lr over the wave's groups + G - 1)
 //   prof_lane  : this lane's profile base (block base + ProfLayout::lane_offset)
 //   lqa/lqb    : query lengths (for padding masks and result positions)
-//   kstar      : wave-uniform (lq-1) % R of both queries, used when tracking the last column
+//   shift      : wave-uniform number of front padding columns.  Semiglobal with free query
+//                ends tracks the last query column every row; there H(i,0) = 0, and front
+//                padding columns whose profile is 0 carry exactly H = 0 and E = gap_open,
+//                the column-0 boundary, so shifting the query by (R - lq mod R) mod R puts
+//                its last column on the last register of a lane.
 template <typename V, int G, int R, bool LOCAL>
 __device__ __forceinline__ SweepOut sweep_group(const uint8_t *__restrict__ rseq, int lr, int nsteps,
                                                 const char *prof_lane, int t, const SweepConst &k,
-                                                int lqa, int lqb, int kstar) {
+                                                int lqa, int lqb, int shift) {
     using O = VOps<V>;
     using L = ProfLayout<G, R>;
     constexpr int KC = L::KC;
@@ -161,13 +165,13 @@ __device__ __forceinline__ SweepOut sweep_group(const uint8_t *__restrict__ rseq
             hup[kk] = ZERO;
             fn[kk] = ZERO;
         } else {
-            const int h0 = row0_value(k, t * R + kk + 1);
+            const int h0 = row0_value(k, t * R + kk + 1 - shift);
             hup[kk] = O::splat(h0);
             fn[kk] = O::splat(max(h0 + k.go, O::NEG + k.ge));   // F(1, col)
         }
     }
-    V hout = LOCAL ? ZERO : O::splat(row0_value(k, (t + 1) * R));
-    V hin = LOCAL ? ZERO : O::splat(t == 0 ? col0_value(k, 1) : row0_value(k, t * R));
+    V hout = LOCAL ? ZERO : O::splat(row0_value(k, (t + 1) * R - shift));
+    V hin = LOCAL ? ZERO : O::splat(t == 0 ? col0_value(k, 1) : row0_value(k, t * R - shift));
     V din = ZERO;
     V ein = LOCAL ? ZERO : O::splat(O::NEG);
     V eout = ein;
@@ -200,14 +204,14 @@ __device__ __forceinline__ SweepOut sweep_group(const uint8_t *__restrict__ rseq
                 const V xn = (kk + 1 < R) ? O::add(hup[kk], pv[kk + 1]) : xd;
                 if (LOCAL) {
                     e = O::mx(O::subu(e, GEP), hgo);
-                    h = O::mx(O::mx(xd, e), fn[kk]);
+                    h = O::mx(O::mx(xd, fn[kk]), e);   // E last: the serial chain is E -> H -> H-gap -> E
                     best = O::mx(best, h);
                     hup[kk] = h;
                     hgo = O::subu(h, GOP);
                     fn[kk] = O::mx(O::subu(fn[kk], GEP), hgo);
                 } else {
                     e = O::mx(O::add(e, GEV), hgo);
-                    h = O::mx(O::mx(xd, e), fn[kk]);
+                    h = O::mx(O::mx(xd, fn[kk]), e);
                     hup[kk] = h;
                     hgo = O::add(h, GOV);
                     fn[kk] = O::mx(O::add(fn[kk], GEV), hgo);
@@ -216,26 +220,7 @@ __device__ __forceinline__ SweepOut sweep_group(const uint8_t *__restrict__ rseq
             }
             hout = h;
             eout = e;
-            if (!LOCAL) {
-                if (track_endcol) {
-#pragma unroll
-                    for (int kk = 0; kk < R; ++kk)
-                        if (kk == kstar) endcol = O::mx(endcol, hup[kk]);
-                }
-                if (row == lr - 1) {
-                    // last row: corner and (free_r) best of the last row over real columns
-#pragma unroll
-                    for (int kk = 0; kk < R; ++kk) {
-                        const int col = t * R + kk;
-                        const int va = O::lo(hup[kk]);
-                        const int vb = O::hi(hup[kk]);
-                        if (col == lqa - 1) corner_a = va;
-                        if (col == lqb - 1) corner_b = vb;
-                        if (col < lqa) endrow_a = max(endrow_a, va);
-                        if (col < lqb) endrow_b = max(endrow_b, vb);
-                    }
-                }
-            }
+            if (!LOCAL && track_endcol) endcol = O::mx(endcol, h);   // lane's last column
         }
         din = hin;
         hin = O::from_bits(wave_shr1(O::bits(hout)));
@@ -255,12 +240,27 @@ __device__ __forceinline__ SweepOut sweep_group(const uint8_t *__restrict__ rseq
         out.b = bb;
         return out;
     }
-    // non-local: each lane offers what it holds; the group maximum is the result
+    // non-local.  A lane stops updating after its last row, so hup[] now holds
+    // H(lr, j) of the lane's columns: the corner and the last-row maximum over the
+    // real columns are read once here.
+    if (lr > 0) {
+#pragma unroll
+        for (int kk = 0; kk < R; ++kk) {
+            const int col = t * R + kk - shift;
+            const int va = O::lo(hup[kk]);
+            const int vb = O::hi(hup[kk]);
+            if (col == lqa - 1) corner_a = va;
+            if (col == lqb - 1) corner_b = vb;
+            if (col < lqa) endrow_a = max(endrow_a, va);
+            if (col < lqb) endrow_b = max(endrow_b, vb);
+        }
+    }
+    // each lane offers what it holds; the group maximum is the result
     int ca = corner_a, cb = corner_b;
     if (semi) {
         if (track_endcol) {
-            if (t == (lqa - 1) / R) ca = max(ca, O::lo(endcol));
-            if (t == (lqb - 1) / R) cb = max(cb, O::hi(endcol));
+            if (t == (lqa - 1 + shift) / R) ca = max(ca, O::lo(endcol));
+            if (t == (lqb - 1 + shift) / R) cb = max(cb, O::hi(endcol));
         }
         if (k.free_r) {
             ca = max(ca, endrow_a);

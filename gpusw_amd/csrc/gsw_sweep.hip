@@ -28,8 +28,13 @@ __device__ __forceinline__ void emit(const KArgs &a, int64_t o, int qs, int ql, 
 // A workgroup takes (query pair, reference range) items from a global counter, builds
 // the pair's packed profile in LDS once, then its waves take blocks of 64/G references
 // from an LDS counter; each group of G lanes sweeps one reference.
+// Occupancy target: 4 waves per SIMD (<= 128 VGPRs) for the narrow local tiles (G <= 8,
+// R <= 32), whose loop fits in 128 registers; the other tiles are left free.
+template <bool LOCAL, int G, int R> struct WavesPerEU { static constexpr int value = (LOCAL && G <= 8 && R <= 32) ? 4 : 1; };
+
 template <typename V, int G, int R, bool LOCAL, int WAVES>
-__global__ void __launch_bounds__(WAVES * 64) cross_kernel(KArgs a) {
+__global__ void __launch_bounds__(WAVES * 64)
+__attribute__((amdgpu_waves_per_eu(WavesPerEU<LOCAL, G, R>::value))) cross_kernel(KArgs a) {
     using O = VOps<V>;
     using L = ProfLayout<G, R>;
     constexpr int LW = L::KC * 4;   // padded columns per lane
@@ -55,9 +60,11 @@ __global__ void __launch_bounds__(WAVES * 64) cross_kernel(KArgs a) {
             const int c = idx / (G * LW);
             const int rem = idx - c * (G * LW);
             const int lt = rem / LW, kk = rem - lt * LW;
-            const int col = lt * R + kk;
-            const int va = (kk < R && col < lqa) ? a.sub[c * a.ns + min((int)a.qcodes[qsa + col], a.ns - 1)] : O::NEG;
-            const int vb = (kk < R && col < lqb) ? a.sub[c * a.ns + min((int)a.qcodes[qsb + col], a.ns - 1)] : O::NEG;
+            const int col = lt * R + kk - item.shift;   // front padding (col < 0) scores 0
+            const int va = kk >= R || col >= lqa ? O::NEG
+                           : col < 0 ? 0 : a.sub[c * a.ns + min((int)a.qcodes[qsa + col], a.ns - 1)];
+            const int vb = kk >= R || col >= lqb ? O::NEG
+                           : col < 0 ? 0 : a.sub[c * a.ns + min((int)a.qcodes[qsb + col], a.ns - 1)];
             const uint32_t word = O::pack(va, vb);
             char *dst = lds + c * L::ROWB + (kk >> 2) * L::CH + lt * 16 + (kk & 3) * 4;
 #pragma unroll
@@ -78,7 +85,7 @@ __global__ void __launch_bounds__(WAVES * 64) cross_kernel(KArgs a) {
             const int lr = ref >= 0 ? a.roff[ref + 1] - rs : 0;
             const int nsteps = __builtin_amdgcn_readfirstlane(wave_max_i32(lr)) + G - 1;
             const SweepOut res = sweep_group<V, G, R, LOCAL>(a.rcodes + rs, lr, nsteps, prof_lane, t,
-                                                             a.k, lqa, lqb, item.kstar);
+                                                             a.k, lqa, lqb, item.shift);
             if (t == 0 && ref >= 0) {
                 emit(a, item.outa + ref, qsa, lqa, ref, res.a);
                 if (item.outb >= 0) emit(a, item.outb + ref, qsb, lqb, ref, res.b);
@@ -91,9 +98,10 @@ __global__ void __launch_bounds__(WAVES * 64) cross_kernel(KArgs a) {
 // ---------------------------------------------------------------- pairs kernel
 // Waves take blocks of 64/G tasks from a global counter.  Every lane writes the
 // profile entries of its own columns into the wave's LDS region and reads only those,
-// so no barrier is needed.  Tasks of one block share kstar (the host pads blocks).
+// so no barrier is needed.  Tasks of one block share their front-padding shift (the host pads blocks).
 template <typename V, int G, int R, bool LOCAL, int WAVES>
-__global__ void __launch_bounds__(WAVES * 64) pairs_kernel(KArgs a) {
+__global__ void __launch_bounds__(WAVES * 64)
+__attribute__((amdgpu_waves_per_eu(WavesPerEU<LOCAL, G, R>::value))) pairs_kernel(KArgs a) {
     using O = VOps<V>;
     using L = ProfLayout<G, R>;
     constexpr int KC = L::KC;
@@ -116,7 +124,7 @@ __global__ void __launch_bounds__(WAVES * 64) pairs_kernel(KArgs a) {
         const int ti = blk * GPW + g;
         PairTask task{-1, -1, -1, 0, -1, -1};
         if (ti < nitems) task = tasks[ti];
-        const int kstar = __builtin_amdgcn_readfirstlane(tasks[blk * GPW].kstar);
+        const int shift = __builtin_amdgcn_readfirstlane(tasks[blk * GPW].shift);
         const bool valid = task.ref >= 0;
         const int qa = valid ? task.qa : 0;
         const int qb = (valid && task.qb >= 0) ? task.qb : qa;
@@ -124,18 +132,26 @@ __global__ void __launch_bounds__(WAVES * 64) pairs_kernel(KArgs a) {
         const int qsb = valid ? a.qoff[qb] : 0, lqb = valid ? a.qoff[qb + 1] - qsb : 0;
         const int rs = valid ? a.roff[task.ref] : 0;
         const int lr = valid ? a.roff[task.ref + 1] - rs : 0;
-        // this lane's profile columns
-        for (int c = 0; c < a.ns; ++c) {
-            const int32_t *srow = a.sub + c * a.ns;
+        // this lane's profile columns: per chunk of 4 columns, the 8 query codes are read
+        // once and the ns profile rows are written (loops kept rolled: low register use)
+#pragma unroll 1
+        for (int kc = 0; kc < KC; ++kc) {
+            int ca[4], cb[4];   // code, or -1 = back padding (NEG), -2 = front padding (0)
 #pragma unroll
-            for (int kc = 0; kc < KC; ++kc) {
+            for (int e = 0; e < 4; ++e) {
+                const int kk = kc * 4 + e;
+                const int col = t * R + kk - shift;
+                ca[e] = (kk >= R || col >= lqa) ? -1 : col < 0 ? -2 : min((int)a.qcodes[qsa + col], a.ns - 1);
+                cb[e] = (kk >= R || col >= lqb) ? -1 : col < 0 ? -2 : min((int)a.qcodes[qsb + col], a.ns - 1);
+            }
+#pragma unroll 1
+            for (int c = 0; c < a.ns; ++c) {
+                const int32_t *srow = a.sub + c * a.ns;
                 uint32_t w[4];
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
-                    const int kk = kc * 4 + e;
-                    const int col = t * R + kk;
-                    const int va = (kk < R && col < lqa) ? srow[min((int)a.qcodes[qsa + col], a.ns - 1)] : O::NEG;
-                    const int vb = (kk < R && col < lqb) ? srow[min((int)a.qcodes[qsb + col], a.ns - 1)] : O::NEG;
+                    const int va = ca[e] == -1 ? O::NEG : ca[e] == -2 ? 0 : srow[ca[e]];
+                    const int vb = cb[e] == -1 ? O::NEG : cb[e] == -2 ? 0 : srow[cb[e]];
                     w[e] = O::pack(va, vb);
                 }
                 *reinterpret_cast<int4 *>(lane_lds + c * L::ROWB + kc * L::CH) =
@@ -144,7 +160,7 @@ __global__ void __launch_bounds__(WAVES * 64) pairs_kernel(KArgs a) {
         }
         const int nsteps = __builtin_amdgcn_readfirstlane(wave_max_i32(lr)) + G - 1;
         const SweepOut res = sweep_group<V, G, R, LOCAL>(a.rcodes + rs, lr, nsteps, lane_lds, t,
-                                                         a.k, lqa, lqb, kstar);
+                                                         a.k, lqa, lqb, shift);
         if (t == 0 && valid) {
             emit(a, task.oa, qsa, lqa, task.ref, res.a);
             if (task.ob >= 0) emit(a, task.ob, qsb, lqb, task.ref, res.b);
@@ -181,7 +197,8 @@ static void launch_pairs(const KArgs &a, int grid, size_t lds, hipStream_t st) {
 std::vector<Variant> &variant_table() {
     static std::vector<Variant> table = [] {
         std::vector<Variant> v = {
-            GSW_VARIANT4(8, 8),   GSW_VARIANT4(8, 12),  GSW_VARIANT4(8, 16),  GSW_VARIANT4(8, 24),
+            GSW_VARIANT4(8, 8),   GSW_VARIANT4(8, 12),  GSW_VARIANT4(8, 16),  GSW_VARIANT4(8, 20),
+            GSW_VARIANT4(8, 24),
             GSW_VARIANT4(8, 32),  GSW_VARIANT4(8, 36),  GSW_VARIANT4(16, 24), GSW_VARIANT4(16, 32),
             GSW_VARIANT4(16, 48), GSW_VARIANT4(32, 32), GSW_VARIANT4(32, 48), GSW_VARIANT4(64, 32),
         };
@@ -193,6 +210,12 @@ std::vector<Variant> &variant_table() {
 
 // Forced variant (G, R) for tuning; 0 = automatic.
 static int g_force_g = 0, g_force_r = 0;
+
+// Measured tile table (from the autotuner): for a query length in [lo, hi] and the given
+// element type / mode, use tile (G, R) when it is compiled, covers the query and fits.
+struct TileRule { int lo, hi, packed, local, G, R; };
+static std::mutex g_rules_mu;
+static std::vector<TileRule> g_rules;
 
 // Estimated relative cost per cell of a (G, R) tile for a query of length lq against
 // references of mean length lr: padded columns, per-step overhead and pipeline fill.
@@ -213,6 +236,16 @@ const Variant *pick_variant(int lq, double mean_lr, bool packed, bool local, int
                                    size_t lds_limit, bool for_pairs) {
     const Variant *best = nullptr;
     double best_cost = 1e30;
+    if (!g_force_g) {
+        std::lock_guard<std::mutex> lk(g_rules_mu);
+        for (const TileRule &r : g_rules) {
+            if (lq < r.lo || lq > r.hi || r.packed != (int)packed || r.local != (int)local) continue;
+            for (const Variant &v : variant_table())
+                if (v.G == r.G && v.R == r.R && v.packed == packed && v.local == local && v.G * v.R >= lq &&
+                    (for_pairs ? pairs_lds_bytes(v, ns) : cross_lds_bytes(v, ns)) <= lds_limit)
+                    return &v;
+        }
+    }
     for (const Variant &v : variant_table()) {
         if (v.packed != packed || v.local != local) continue;
         if (v.G * v.R < lq) continue;
@@ -347,7 +380,10 @@ int plan_cross_units(const SchemeDev &s, int dtype_policy, const std::vector<Cro
                      std::vector<CrossQuery> *plain) {
     const bool local = s.mode == GSW_MODE_LOCAL;
     const int wmax = max_width();
-    struct QInfo { CrossQuery q; const Variant *v; int kstar; };
+    // semiglobal with free query ends tracks the last query column: front-pad the query so
+    // that column lands on a lane's last register (see sweep_group)
+    const bool track_last_col = s.mode == GSW_MODE_SEMIGLOBAL && s.free_q;
+    struct QInfo { CrossQuery q; const Variant *v; int shift; };
     std::vector<QInfo> packed_q, single_q;
     for (const CrossQuery &q : queries) {
         if (!fits_int32(s, q.len, max_lr))
@@ -359,27 +395,25 @@ int plan_cross_units(const SchemeDev &s, int dtype_policy, const std::vector<Cro
             plain->push_back(q);
             continue;
         }
-        (use16 ? packed_q : single_q).push_back(QInfo{q, v, (q.len - 1) % v->R});
+        (use16 ? packed_q : single_q).push_back(QInfo{q, v, track_last_col ? (v->R - q.len % v->R) % v->R : 0});
     }
-    // pair packed queries of the same tile (and the same last-column register when the
-    // last column is tracked, so that it stays wave-uniform)
-    const bool need_kstar = s.mode == GSW_MODE_SEMIGLOBAL && s.free_q;
-    auto key = [&](const QInfo &x) { return std::make_tuple(x.v->id, need_kstar ? x.kstar : 0, x.q.len); };
+    // pair packed queries of the same tile and the same shift (wave-uniform)
+    auto key = [&](const QInfo &x) { return std::make_tuple(x.v->id, track_last_col ? x.shift : 0, x.q.len); };
     auto by_key = [&](const QInfo &x, const QInfo &y) { return key(x) < key(y); };
     std::stable_sort(packed_q.begin(), packed_q.end(), by_key);
     std::stable_sort(single_q.begin(), single_q.end(), by_key);
     for (size_t i = 0; i < packed_q.size();) {
         const QInfo &x = packed_q[i];
         if (i + 1 < packed_q.size() && packed_q[i + 1].v == x.v &&
-            (!need_kstar || packed_q[i + 1].kstar == x.kstar)) {
-            units->push_back(CrossUnit{x.v, x.q, packed_q[i + 1].q, true, x.kstar});
+            (!track_last_col || packed_q[i + 1].shift == x.shift)) {
+            units->push_back(CrossUnit{x.v, x.q, packed_q[i + 1].q, true, x.shift});
             i += 2;
         } else {
-            units->push_back(CrossUnit{x.v, x.q, CrossQuery{0, 0, -1, -1}, false, x.kstar});
+            units->push_back(CrossUnit{x.v, x.q, CrossQuery{0, 0, -1, -1}, false, x.shift});
             i += 1;
         }
     }
-    for (const QInfo &x : single_q) units->push_back(CrossUnit{x.v, x.q, CrossQuery{0, 0, -1, -1}, false, x.kstar});
+    for (const QInfo &x : single_q) units->push_back(CrossUnit{x.v, x.q, CrossQuery{0, 0, -1, -1}, false, x.shift});
     std::stable_sort(units->begin(), units->end(),
                      [](const CrossUnit &x, const CrossUnit &y) { return x.v->id < y.v->id; });
     return GSW_OK;
@@ -391,7 +425,7 @@ void expand_cross_items(const CrossUnit *u0, const CrossUnit *u1, int64_t nref, 
         for (int64_t rb = 0; rb < nref; rb += chunk) {
             items->push_back(CrossItem{u->a.start, u->a.len, u->has_b ? u->b.start : 0, u->has_b ? u->b.len : -1,
                                        u->a.out, u->has_b ? u->b.out : -1, (int32_t)rb,
-                                       (int32_t)std::min(nref, rb + chunk), u->kstar, 0});
+                                       (int32_t)std::min(nref, rb + chunk), u->shift, 0});
         }
     }
 }
@@ -623,7 +657,7 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
                 int32_t *d_out, BatchResult *res) {
     if (n == 0) return GSW_OK;
     const bool local = s.mode == GSW_MODE_LOCAL;
-    const bool need_kstar = s.mode == GSW_MODE_SEMIGLOBAL && s.free_q;
+    const bool track_last_col = s.mode == GSW_MODE_SEMIGLOBAL && s.free_q;
     const int wmax = max_width();
     const double mean_lr = rs.n_nonempty ? (double)rs.total / rs.n_nonempty : 1.0;
 
@@ -711,37 +745,37 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
             order.resize(b1 - b0);
             for (size_t i = b0; i < b1; ++i) order[cnt[rs.rank[h_rj[members[i]]]]++] = members[i];
             // ... then (stable) by last-column register when it is tracked
-            if (need_kstar) {
+            if (track_last_col) {
                 cnt.assign(v.R + 1, 0);
-                for (int64_t k : order) cnt[(qs.len[h_qi[k]] - 1) % v.R + 1]++;
+                for (int64_t k : order) cnt[(v.R - qs.len[h_qi[k]] % v.R) % v.R + 1]++;
                 for (int r = 0; r < v.R; ++r) cnt[r + 1] += cnt[r];
                 order2.resize(order.size());
-                for (int64_t k : order) order2[cnt[(qs.len[h_qi[k]] - 1) % v.R]++] = k;
+                for (int64_t k : order) order2[cnt[(v.R - qs.len[h_qi[k]] % v.R) % v.R]++] = k;
                 order.swap(order2);
             }
             // the pinned buffer of this slot is free once its previous replay finished
             GSW_HIP_TRY(hipEventSynchronize(g.done));
             PairTask *tasks = reinterpret_cast<PairTask *>(g.h_buf + PAIR_HDR);
             int32_t nt = 0;
-            int cur_kstar = -1;
+            int cur_shift = -1;
             for (size_t i = 0; i < order.size();) {
                 const int64_t ka = order[i];
-                const int kstar = need_kstar ? (qs.len[h_qi[ka]] - 1) % v.R : 0;
-                if (kstar != cur_kstar) {
-                    while (nt % gpw) tasks[nt++] = PairTask{-1, -1, -1, cur_kstar, -1, -1};
-                    cur_kstar = kstar;
+                const int shift = track_last_col ? (v.R - qs.len[h_qi[ka]] % v.R) % v.R : 0;
+                if (shift != cur_shift) {
+                    while (nt % gpw) tasks[nt++] = PairTask{-1, -1, -1, cur_shift, -1, -1};
+                    cur_shift = shift;
                 }
                 if (v.packed && i + 1 < order.size() && h_rj[order[i + 1]] == h_rj[ka] &&
-                    (!need_kstar || (qs.len[h_qi[order[i + 1]]] - 1) % v.R == kstar)) {
+                    (!track_last_col || (v.R - qs.len[h_qi[order[i + 1]]] % v.R) % v.R == shift)) {
                     const int64_t kb = order[i + 1];
-                    tasks[nt++] = PairTask{h_qi[ka], h_qi[kb], h_rj[ka], kstar, ka, kb};
+                    tasks[nt++] = PairTask{h_qi[ka], h_qi[kb], h_rj[ka], shift, ka, kb};
                     i += 2;
                 } else {
-                    tasks[nt++] = PairTask{h_qi[ka], -1, h_rj[ka], kstar, ka, -1};
+                    tasks[nt++] = PairTask{h_qi[ka], -1, h_rj[ka], shift, ka, -1};
                     i += 1;
                 }
             }
-            while (nt % gpw) tasks[nt++] = PairTask{-1, -1, -1, cur_kstar, -1, -1};
+            while (nt % gpw) tasks[nt++] = PairTask{-1, -1, -1, cur_shift, -1, -1};
             int32_t *hdr = reinterpret_cast<int32_t *>(g.h_buf);
             hdr[0] = nt;   // task count
             hdr[1] = 0;    // work counter
@@ -872,6 +906,20 @@ extern "C" int gsw_force_variant(int32_t G, int32_t R) {
             return GSW_OK;
         }
     return fail(GSW_ERR_INVALID, "no compiled variant G=%d R=%d", G, R);
+}
+
+// Install a measured tile table: n rules of 6 int32 (lo, hi, packed, local, G, R); n = 0
+// clears it.  Rules are consulted before the cost model.
+extern "C" int gsw_set_tile_table(const int32_t *rules, int32_t n) {
+    if (n < 0 || (n > 0 && !rules)) return fail(GSW_ERR_INVALID, "bad tile table");
+    std::vector<TileRule> t;
+    for (int i = 0; i < n; ++i) {
+        const int32_t *r = rules + 6 * i;
+        t.push_back(TileRule{r[0], r[1], r[2] ? 1 : 0, r[3] ? 1 : 0, r[4], r[5]});
+    }
+    std::lock_guard<std::mutex> lk(g_rules_mu);
+    g_rules.swap(t);
+    return GSW_OK;
 }
 
 // List compiled (G, R) tiles: writes up to cap (G, R) pairs, returns the count in *n.

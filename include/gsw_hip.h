@@ -135,6 +135,17 @@ int gsw_score_pairs(const gsw_scheme *s, int32_t dtype_policy,
 int gsw_topk_rows(const int32_t *d_scores, int32_t nrows, int32_t ncols, int32_t k,
                   int32_t *h_idx, int32_t *h_sc, gsw_stats *stats);
 
+/* ---- tile selection ---------------------------------------------------------
+ * No counterpart in the reference (its kernel has no tile parameters).  The batched
+ * kernels are compiled for a set of (G lanes per reference, R query columns per lane)
+ * tiles; a cost model picks one per query length.  The autotuner measures the tiles
+ * and installs rules that take precedence: n rules of 6 int32
+ * (lq_lo, lq_hi, packed_int16, local_mode, G, R); n = 0 clears them.  Forcing a tile
+ * (G, R) applies it to every query it covers ((0, 0) restores automatic choice). */
+int gsw_set_tile_table(const int32_t *rules, int32_t n);
+int gsw_force_variant(int32_t G, int32_t R);
+int gsw_list_variants(int32_t *gr, int32_t cap, int32_t *n);   /* (G, R) pairs */
+
 /* ---- serving -------------------------------------------------------------
  * No counterpart in the reference (its Aligner.top_k, src/gpusw/aligner.py:169-195,
  * is a synchronous batch loop).  A server keeps reference databases resident, admits
