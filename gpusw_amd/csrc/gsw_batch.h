@@ -14,7 +14,8 @@ struct CrossItem {
     int32_t qsb, lqb;        // query B (lqb < 0: single query)
     int64_t outa, outb;      // output offset of reference 0 for A / B (outb < 0: none)
     int32_t rbeg, rend;      // range in the sorted reference order
-    int32_t shift, pad;
+    int32_t shift;           // front padding columns (semiglobal, free query ends)
+    int32_t nstr;            // stripes of G*R columns (stripe kernel; 1 otherwise)
 };
 
 // Task of the pairs kernel: queries qa and qb (qb < 0: single) against reference ref.
@@ -48,6 +49,9 @@ struct KArgs {
     int32_t nitems;
     int32_t *counter;
     const int32_t *nitems_ptr;   // pairs kernel: task count read from device memory
+    uint2 *bnd;                  // stripe kernel: boundary regions, one per workgroup
+    int64_t bnd_stride;          //   uint2 entries per region (>= rows of any item)
+    const int64_t *bpos;         //   row offsets of the sorted references (n + 1)
 };
 
 
@@ -59,8 +63,8 @@ typedef void (*LaunchFn)(const KArgs &, int grid, size_t lds, hipStream_t);
 struct Variant {
     int id, G, R;
     bool packed, local;
-    const void *cross_fn, *pairs_fn;
-    LaunchFn cross, pairs;
+    const void *cross_fn, *pairs_fn, *stripe_fn;   // stripe_fn null: no stripe kernel
+    LaunchFn cross, pairs, stripe;
 };
 
 std::vector<Variant> &variant_table();
@@ -83,6 +87,7 @@ struct CrossUnit {
     CrossQuery a, b;
     bool has_b;
     int shift;
+    int nstr;   // > 1: the queries run as nstr stripes of v's width (stripe kernel)
 };
 
 // Route queries to tiles, pairing packed int16 queries of the same tile (and of the same
@@ -91,9 +96,18 @@ struct CrossUnit {
 int plan_cross_units(const SchemeDev &s, int dtype_policy, const std::vector<CrossQuery> &queries,
                      int max_lr, double mean_lr, size_t lds_limit, std::vector<CrossUnit> *units,
                      std::vector<CrossQuery> *plain);
+// Tile for a cross query of length lq: the cheapest compiled tile covering it, or a tile
+// with a stripe kernel run as *nstr > 1 stripes when that is cheaper or nothing else fits.
+const Variant *pick_cross_tile(int lq, double mean_lr, bool packed, bool local, int ns, size_t lds_limit,
+                               bool track_last_col, int *nstr);
 // One item per (unit, chunk of `chunk` sorted references).
 void expand_cross_items(const CrossUnit *u0, const CrossUnit *u1, int64_t nref, int64_t chunk,
                         std::vector<CrossItem> *items);
+// Striped units: one item per (unit, run of sorted references of at most max_rows total
+// length, or one reference).  Returns the largest item's row count.
+int64_t stripe_item_rows(double mean_lr, int max_lr);
+int64_t expand_stripe_items(const CrossUnit *u0, const CrossUnit *u1, const SeqSet &rs, int64_t max_rows,
+                            std::vector<CrossItem> *items);
 
 // out[idx[i]] = val[i] on the context's stream (synchronous).
 int scatter_values(DeviceCtx *ctx, int32_t *d_out, const std::vector<int64_t> &idx,

@@ -43,6 +43,7 @@ struct DeviceCtx {
     void *plain_ws = nullptr;  size_t plain_ws_bytes = 0;
     void *meta = nullptr;      size_t meta_bytes = 0;      // task lists, lists of ids
     void *meta2 = nullptr;     size_t meta2_bytes = 0;
+    void *bnd = nullptr;       size_t bnd_bytes = 0;       // stripe boundaries
     int32_t *counters = nullptr;                            // 64 int32 counters
     std::mutex mu;                                          // serialises calls on this device
 };
@@ -78,8 +79,17 @@ struct SeqSet {
     std::vector<int32_t> order_desc;   // indices sorted by length, longest first
     std::vector<int32_t> rank;         // rank[i] = position of sequence i in order_desc
     int32_t *d_order_desc = nullptr;   // device copy of order_desc (non-empty only)
+    std::vector<int64_t> bpos;         // bpos[p] = total length of order_desc[0..p)
+    int64_t *d_bpos = nullptr;         // device copy (n + 1)
     int32_t n_nonempty = 0;
     int device = -1;
+    SeqSet() = default;
+    SeqSet(const SeqSet &) = delete;
+    SeqSet &operator=(const SeqSet &) = delete;
+    ~SeqSet() {
+        if (d_order_desc) (void)hipFree(d_order_desc);
+        if (d_bpos) (void)hipFree(d_bpos);
+    }
 };
 
 // ---------------------------------------------------------------- plain kernel

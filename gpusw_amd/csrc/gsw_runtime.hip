@@ -285,11 +285,15 @@ int gsw_seqset_create(const uint8_t *d_codes, const int32_t *d_off, const int32_
     for (int i = 0; i < n; ++i) set->rank[set->order_desc[i]] = i;
     set->n_nonempty = 0;
     while (set->n_nonempty < n && set->len[set->order_desc[set->n_nonempty]] > 0) set->n_nonempty++;
+    set->bpos.assign((size_t)n + 1, 0);
+    for (int p = 0; p < n; ++p) set->bpos[p + 1] = set->bpos[p] + set->len[set->order_desc[p]];
     if (n > 0) {
         GSW_HIP_TRY(hipMalloc(&set->d_order_desc, sizeof(int32_t) * n));
         GSW_HIP_TRY(hipMemcpy(set->d_order_desc, set->order_desc.data(), sizeof(int32_t) * n,
                               hipMemcpyHostToDevice));
     }
+    GSW_HIP_TRY(hipMalloc(&set->d_bpos, sizeof(int64_t) * (n + 1)));
+    GSW_HIP_TRY(hipMemcpy(set->d_bpos, set->bpos.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice));
     *out = reinterpret_cast<gsw_seqset>(set.release());
     return GSW_OK;
 }
@@ -297,7 +301,6 @@ int gsw_seqset_create(const uint8_t *d_codes, const int32_t *d_off, const int32_
 int gsw_seqset_destroy(gsw_seqset set) {
     SeqSet *s = reinterpret_cast<SeqSet *>(set);
     if (!s) return GSW_OK;
-    if (s->d_order_desc) (void)hipFree(s->d_order_desc);
     delete s;
     return GSW_OK;
 }

@@ -90,7 +90,7 @@ struct Request {
     size_t codes_at = 0, codes_bytes = 0;     // in the code arena
     size_t scores_at = 0, scores_bytes = 0;   // in the score arena (scores + top-k output)
     std::vector<CrossItem> items;             // grouped by tile
-    std::vector<int> item_tile;               // variant id per item
+    std::vector<int> item_tile;               // per item: 2 * variant id + 1 if striped
     size_t next = 0;
     int inflight = 0;
     std::vector<int32_t> idx, sc;
@@ -111,6 +111,8 @@ struct Slot {
     int db = -1;
     std::vector<std::pair<Request *, int>> reqs;
     std::vector<CrossItem> host_items;
+    void *bnd = nullptr;           // stripe boundaries of this slot's launch
+    size_t bnd_bytes = 0;
 };
 
 constexpr int SAT_CAP = 1 << 16;
@@ -186,18 +188,22 @@ int Server::plan_locked(Request *r) {
     const int64_t nref = rs.n_nonempty;
     for (size_t u0 = 0; u0 < units.size();) {
         size_t u1 = u0;
-        while (u1 < units.size() && units[u1].v == units[u0].v) ++u1;
+        const bool striped = units[u0].nstr > 1;
+        while (u1 < units.size() && units[u1].v == units[u0].v && (units[u1].nstr > 1) == striped) ++u1;
         const Variant &v = *units[u0].v;
-        const int gpw = 64 / v.G;
-        const double per_ref = (double)(units[u0].a.len + std::max(units[u0].b.len, 0)) * db.mean_lr;
-        int64_t chunk = (int64_t)(1e8 / std::max(per_ref, 1.0));
-        chunk = std::max<int64_t>(chunk, (int64_t)gpw * CROSS_WAVES);
-        chunk = std::min<int64_t>(chunk, nref);
-        chunk = (chunk + gpw - 1) / gpw * gpw;
-        const size_t before = r->items.size();
-        expand_cross_items(units.data() + u0, units.data() + u1, nref, chunk, &r->items);
-        r->item_tile.resize(r->items.size(), v.id);
-        (void)before;
+        if (striped) {
+            expand_stripe_items(units.data() + u0, units.data() + u1, rs, stripe_item_rows(db.mean_lr, rs.max_len),
+                                &r->items);
+        } else {
+            const int gpw = 64 / v.G;
+            const double per_ref = (double)(units[u0].a.len + std::max(units[u0].b.len, 0)) * db.mean_lr;
+            int64_t chunk = (int64_t)(1e8 / std::max(per_ref, 1.0));
+            chunk = std::max<int64_t>(chunk, (int64_t)gpw * CROSS_WAVES);
+            chunk = std::min<int64_t>(chunk, nref);
+            chunk = (chunk + gpw - 1) / gpw * gpw;
+            expand_cross_items(units.data() + u0, units.data() + u1, nref, chunk, &r->items);
+        }
+        r->item_tile.resize(r->items.size(), 2 * v.id + (striped ? 1 : 0));
         u0 = u1;
     }
     // queries no tile covers, and edge cells, are scored now with the comparator
@@ -292,7 +298,8 @@ int Server::launch_locked(int si, Request *head) {
     const int db = head->db;
     const int tile = head->item_tile[head->next];
     const ServerDb &d = *dbs[db];
-    const Variant &v = variant_table()[tile];
+    const Variant &v = variant_table()[tile / 2];
+    const bool striped = tile & 1;
     sl.host_items.clear();
     sl.reqs.clear();
     sl.db = db;
@@ -314,7 +321,8 @@ int Server::launch_locked(int si, Request *head) {
             } else {
                 sl.reqs.back().second++;
             }
-            cells += (double)(it.lqa + std::max(it.lqb, 0)) * per_item_ref * (it.rend - it.rbeg);
+            cells += striped ? (double)(it.lqa + std::max(it.lqb, 0)) * (double)(d.rs->bpos[it.rend] - d.rs->bpos[it.rbeg])
+                             : (double)(it.lqa + std::max(it.lqb, 0)) * per_item_ref * (it.rend - it.rbeg);
             progress = true;
             if (cells >= launch_cells) break;
         }
@@ -329,7 +337,14 @@ int Server::launch_locked(int si, Request *head) {
     GSW_HIP_TRY(hipMemsetAsync(sl.counters, 0, 2 * sizeof(int32_t), ctx.stream));
     const size_t lds = cross_lds_bytes(v, d.s.ns);
     int grid = 0;
-    GSW_TRY(occupancy_grid(&ctx, v.cross_fn, CROSS_WAVES * 64, lds, &grid));
+    GSW_TRY(occupancy_grid(&ctx, striped ? v.stripe_fn : v.cross_fn, CROSS_WAVES * 64, lds, &grid));
+    grid = std::min<int>(grid, (int)sl.host_items.size());
+    int64_t rows = 0;   // boundary entries per workgroup (striped)
+    if (striped) {
+        for (const CrossItem &it : sl.host_items) rows = std::max(rows, d.rs->bpos[it.rend] - d.rs->bpos[it.rbeg]);
+        // launches of both slots may run at once: each slot has its own half of the scratch
+        GSW_TRY(ensure_buffer(&sl.bnd, &sl.bnd_bytes, sizeof(uint2) * (size_t)rows * grid));
+    }
     KArgs a{};
     a.qcodes = code_base;
     a.rcodes = d.rs->d_codes;
@@ -346,7 +361,14 @@ int Server::launch_locked(int si, Request *head) {
     a.items = sl.items;
     a.nitems = (int32_t)sl.host_items.size();
     a.counter = sl.counters;
-    v.cross(a, std::min<int>(grid, a.nitems), lds, ctx.stream);
+    if (striped) {
+        a.bnd = static_cast<uint2 *>(sl.bnd);
+        a.bnd_stride = rows;
+        a.bpos = d.rs->d_bpos;
+        v.stripe(a, grid, lds, ctx.stream);
+    } else {
+        v.cross(a, grid, lds, ctx.stream);
+    }
     GSW_HIP_TRY(hipGetLastError());
     GSW_HIP_TRY(hipEventRecord(sl.ev, ctx.stream));
     launches++;
@@ -640,6 +662,7 @@ int gsw_server_destroy(gsw_server server) {
         (void)hipFree(srv->slots[i].counters);
         (void)hipFree(srv->slots[i].sat);
         if (srv->slots[i].items) (void)hipFree(srv->slots[i].items);
+        if (srv->slots[i].bnd) (void)hipFree(srv->slots[i].bnd);
     }
     (void)hipFree(srv->code_base);
     (void)hipFree(srv->score_base);

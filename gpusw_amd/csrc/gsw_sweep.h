@@ -140,15 +140,23 @@ struct SweepOut {
 //   nsteps     : wave-uniform step count (max lr over the wave's groups + G - 1)
 //   prof_lane  : this lane's profile base (block base + ProfLayout::lane_offset)
 //   lqa/lqb    : query lengths (for padding masks and result positions)
-//   shift      : wave-uniform number of front padding columns.  Semiglobal with free query
-//                ends tracks the last query column every row; there H(i,0) = 0, and front
+//   cbase      : wave-uniform query column (0-based) of lane 0's first register.  It is
+//                negative when the query is front-padded: semiglobal with free query ends
+//                tracks the last query column every row; there H(i,0) = 0, and front
 //                padding columns whose profile is 0 carry exactly H = 0 and E = gap_open,
 //                the column-0 boundary, so shifting the query by (R - lq mod R) mod R puts
 //                its last column on the last register of a lane.
-template <typename V, int G, int R, bool LOCAL>
+// STRIPE: the group covers query columns [cbase, cbase + G*R) of a longer query.  bnd[i]
+// holds (H, E) of row i+1 at the column left of the stripe when bnd_in (else the column-0
+// boundary applies), and receives (H, E) of the stripe's last column when bnd_out; the
+// row is read one step before lane 0 needs it and written G-1 steps after, so one buffer
+// serves both.  The result then covers the stripe's columns only (the caller combines
+// stripes with max).
+template <typename V, int G, int R, bool LOCAL, bool STRIPE = false>
 __device__ __forceinline__ SweepOut sweep_group(const uint8_t *__restrict__ rseq, int lr, int nsteps,
                                                 const char *prof_lane, int t, const SweepConst &k,
-                                                int lqa, int lqb, int shift) {
+                                                int lqa, int lqb, int cbase, uint2 *bnd = nullptr,
+                                                bool bnd_in = false, bool bnd_out = false) {
     using O = VOps<V>;
     using L = ProfLayout<G, R>;
     constexpr int KC = L::KC;
@@ -165,22 +173,29 @@ __device__ __forceinline__ SweepOut sweep_group(const uint8_t *__restrict__ rseq
             hup[kk] = ZERO;
             fn[kk] = ZERO;
         } else {
-            const int h0 = row0_value(k, t * R + kk + 1 - shift);
+            const int h0 = row0_value(k, cbase + t * R + kk + 1);
             hup[kk] = O::splat(h0);
             fn[kk] = O::splat(max(h0 + k.go, O::NEG + k.ge));   // F(1, col)
         }
     }
-    V hout = LOCAL ? ZERO : O::splat(row0_value(k, (t + 1) * R - shift));
-    V hin = LOCAL ? ZERO : O::splat(t == 0 ? col0_value(k, 1) : row0_value(k, t * R - shift));
-    V din = ZERO;
+    V hout = LOCAL ? ZERO : O::splat(row0_value(k, cbase + (t + 1) * R));
+    V hin = LOCAL ? ZERO : O::splat(t == 0 ? col0_value(k, 1) : row0_value(k, cbase + t * R));
+    V din = LOCAL ? ZERO : O::splat(row0_value(k, cbase));   // H(0, left column), lane 0
     V ein = LOCAL ? ZERO : O::splat(O::NEG);
     V eout = ein;
+    if (STRIPE && bnd_in && t == 0 && lr > 0) {
+        const uint2 b0 = bnd[0];
+        hin = O::from_bits((int)b0.x);
+        ein = O::from_bits((int)b0.y);
+    }
     V best = ZERO;
     V endcol = O::splat(O::NEG);
     int corner_a = RES_NONE, corner_b = RES_NONE, endrow_a = RES_NONE, endrow_b = RES_NONE;
 
     for (int s = 0; s < nsteps; ++s) {
         const int row = s - t;
+        uint2 nb{0u, 0u};
+        if (STRIPE && bnd_in && t == 0 && row + 1 < lr) nb = bnd[row + 1];   // lane 0's next row
         if ((unsigned)row < (unsigned)lr) {
             const int c = min((int)rseq[row], k.ns - 1);   // out-of-range codes score as the sentinel
             const char *pc = prof_lane + c * L::ROWB;
@@ -221,13 +236,19 @@ __device__ __forceinline__ SweepOut sweep_group(const uint8_t *__restrict__ rseq
             hout = h;
             eout = e;
             if (!LOCAL && track_endcol) endcol = O::mx(endcol, h);   // lane's last column
+            if (STRIPE && bnd_out && t == G - 1) bnd[row] = uint2{(unsigned)O::bits(h), (unsigned)O::bits(e)};
         }
         din = hin;
         hin = O::from_bits(wave_shr1(O::bits(hout)));
         ein = O::from_bits(wave_shr1(O::bits(eout)));
         if (t == 0) {
-            hin = LOCAL ? ZERO : O::splat(col0_value(k, s + 2));
-            ein = LOCAL ? ZERO : O::splat(O::NEG);
+            if (STRIPE && bnd_in) {
+                hin = O::from_bits((int)nb.x);
+                ein = O::from_bits((int)nb.y);
+            } else {
+                hin = LOCAL ? ZERO : O::splat(col0_value(k, s + 2));
+                ein = LOCAL ? ZERO : O::splat(O::NEG);
+            }
         }
     }
 
@@ -246,7 +267,7 @@ __device__ __forceinline__ SweepOut sweep_group(const uint8_t *__restrict__ rseq
     if (lr > 0) {
 #pragma unroll
         for (int kk = 0; kk < R; ++kk) {
-            const int col = t * R + kk - shift;
+            const int col = cbase + t * R + kk;
             const int va = O::lo(hup[kk]);
             const int vb = O::hi(hup[kk]);
             if (col == lqa - 1) corner_a = va;
@@ -259,8 +280,9 @@ __device__ __forceinline__ SweepOut sweep_group(const uint8_t *__restrict__ rseq
     int ca = corner_a, cb = corner_b;
     if (semi) {
         if (track_endcol) {
-            if (t == (lqa - 1 + shift) / R) ca = max(ca, O::lo(endcol));
-            if (t == (lqb - 1 + shift) / R) cb = max(cb, O::hi(endcol));
+            // the lane whose last register is column lq-1 (none in other stripes)
+            if (cbase + t * R + R - 1 == lqa - 1) ca = max(ca, O::lo(endcol));
+            if (cbase + t * R + R - 1 == lqb - 1) cb = max(cb, O::hi(endcol));
         }
         if (k.free_r) {
             ca = max(ca, endrow_a);

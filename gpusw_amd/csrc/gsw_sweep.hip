@@ -85,13 +85,100 @@ __attribute__((amdgpu_waves_per_eu(WavesPerEU<LOCAL, G, R>::value))) cross_kerne
             const int lr = ref >= 0 ? a.roff[ref + 1] - rs : 0;
             const int nsteps = __builtin_amdgcn_readfirstlane(wave_max_i32(lr)) + G - 1;
             const SweepOut res = sweep_group<V, G, R, LOCAL>(a.rcodes + rs, lr, nsteps, prof_lane, t,
-                                                             a.k, lqa, lqb, item.shift);
+                                                             a.k, lqa, lqb, -item.shift);
             if (t == 0 && ref >= 0) {
                 emit(a, item.outa + ref, qsa, lqa, ref, res.a);
                 if (item.outb >= 0) emit(a, item.outb + ref, qsb, lqb, ref, res.b);
             }
         }
         __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------- stripe kernel
+// Queries wider than a tile: the query is cut into stripes of W = G*R columns, and a
+// workgroup takes (query pair, reference range) items as the cross kernel does but runs
+// the item stripe by stripe: build the stripe's profile, let the waves sweep all the
+// item's references through it, then move right.  Each reference row's (H, E) at the
+// stripe edge is kept in the workgroup's boundary region of global memory (8 bytes per
+// reference row of the item, in sorted order); results of a reference are combined over
+// the stripes with max in the output, and the saturation check sees the final value.
+template <typename V, int G, int R, bool LOCAL, int WAVES>
+__global__ void __launch_bounds__(WAVES * 64)
+__attribute__((amdgpu_waves_per_eu(WavesPerEU<LOCAL, G, R>::value))) stripe_kernel(KArgs a) {
+    using O = VOps<V>;
+    using L = ProfLayout<G, R>;
+    constexpr int LW = L::KC * 4;   // padded columns per lane
+    constexpr int GPW = 64 / G;
+    constexpr int W = G * R;
+    extern __shared__ __align__(16) char lds[];
+    __shared__ int s_item, s_blk;
+    const int lane = threadIdx.x & 63;
+    const int t = lane % G, g = lane / G;
+    const CrossItem *items = static_cast<const CrossItem *>(a.items);
+    uint2 *bnd_wg = a.bnd + (size_t)blockIdx.x * (size_t)a.bnd_stride;
+    for (;;) {
+        if (threadIdx.x == 0) s_item = atomicAdd(a.counter, 1);
+        __syncthreads();
+        const int it = __builtin_amdgcn_readfirstlane(s_item);
+        if (it >= a.nitems) break;
+        const CrossItem item = items[it];
+        const int qsa = item.qsa, lqa = item.lqa;
+        const int qsb = item.lqb >= 0 ? item.qsb : qsa, lqb = item.lqb >= 0 ? item.lqb : lqa;
+        const int64_t row_base = a.bpos[item.rbeg];
+        for (int st = 0; st < item.nstr; ++st) {
+            const int cbase = st * W - item.shift;
+            if (threadIdx.x == 0) s_blk = 0;
+            const int total = a.ns * G * LW;
+            for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+                const int c = idx / (G * LW);
+                const int rem = idx - c * (G * LW);
+                const int lt = rem / LW, kk = rem - lt * LW;
+                const int col = cbase + lt * R + kk;   // front padding (col < 0) scores 0
+                const int va = kk >= R || col >= lqa ? O::NEG
+                               : col < 0 ? 0 : a.sub[c * a.ns + min((int)a.qcodes[qsa + col], a.ns - 1)];
+                const int vb = kk >= R || col >= lqb ? O::NEG
+                               : col < 0 ? 0 : a.sub[c * a.ns + min((int)a.qcodes[qsb + col], a.ns - 1)];
+                const uint32_t word = O::pack(va, vb);
+                char *dst = lds + c * L::ROWB + (kk >> 2) * L::CH + lt * 16 + (kk & 3) * 4;
+#pragma unroll
+                for (int cp = 0; cp < L::NSLOT; ++cp)
+                    *reinterpret_cast<uint32_t *>(dst + cp * G * 16) = word;
+            }
+            __syncthreads();
+            const bool first = st == 0, last = st + 1 == item.nstr;
+            const char *prof_lane = lds + L::lane_offset(g, t);
+            for (;;) {
+                int blk = 0;
+                if (lane == 0) blk = atomicAdd(&s_blk, 1);
+                blk = __builtin_amdgcn_readfirstlane(blk);
+                const int r0 = item.rbeg + blk * GPW;
+                if (r0 >= item.rend) break;
+                const int rpos = r0 + g;
+                const int ref = rpos < item.rend ? a.rorder[rpos] : -1;
+                const int rs = ref >= 0 ? a.roff[ref] : 0;
+                const int lr = ref >= 0 ? a.roff[ref + 1] - rs : 0;
+                uint2 *bnd = bnd_wg + (ref >= 0 ? a.bpos[rpos] - row_base : 0);
+                const int nsteps = __builtin_amdgcn_readfirstlane(wave_max_i32(lr)) + G - 1;
+                const SweepOut res = sweep_group<V, G, R, LOCAL, true>(a.rcodes + rs, lr, nsteps, prof_lane, t,
+                                                                       a.k, lqa, lqb, cbase, bnd, !first, !last);
+                if (t == 0 && ref >= 0) {
+                    int va = res.a, vb = res.b;
+                    if (!first) {   // written by this workgroup in an earlier stripe
+                        va = max(va, a.out[item.outa + ref]);
+                        if (item.outb >= 0) vb = max(vb, a.out[item.outb + ref]);
+                    }
+                    if (last) {
+                        emit(a, item.outa + ref, qsa, lqa, ref, va);
+                        if (item.outb >= 0) emit(a, item.outb + ref, qsb, lqb, ref, vb);
+                    } else {
+                        a.out[item.outa + ref] = va;
+                        if (item.outb >= 0) a.out[item.outb + ref] = vb;
+                    }
+                }
+            }
+            __syncthreads();   // profile, partial results and boundaries complete
+        }
     }
 }
 
@@ -160,7 +247,7 @@ __attribute__((amdgpu_waves_per_eu(WavesPerEU<LOCAL, G, R>::value))) pairs_kerne
         }
         const int nsteps = __builtin_amdgcn_readfirstlane(wave_max_i32(lr)) + G - 1;
         const SweepOut res = sweep_group<V, G, R, LOCAL>(a.rcodes + rs, lr, nsteps, lane_lds, t,
-                                                         a.k, lqa, lqb, shift);
+                                                         a.k, lqa, lqb, -shift);
         if (t == 0 && valid) {
             emit(a, task.oa, qsa, lqa, task.ref, res.a);
             if (task.ob >= 0) emit(a, task.ob, qsb, lqb, task.ref, res.b);
@@ -186,20 +273,34 @@ static void launch_pairs(const KArgs &a, int grid, size_t lds, hipStream_t st) {
     hipLaunchKernelGGL((pairs_kernel<V, G, R, L, PAIRS_WAVES>), dim3(grid), dim3(PAIRS_WAVES * 64), lds, st, a);
 }
 
+template <typename V, int G, int R, bool L>
+static void launch_stripe(const KArgs &a, int grid, size_t lds, hipStream_t st) {
+    hipLaunchKernelGGL((stripe_kernel<V, G, R, L, CROSS_WAVES>), dim3(grid), dim3(CROSS_WAVES * 64), lds, st, a);
+}
+
 #define GSW_VARIANT(V, PK, G, R, L)                                                               \
     Variant{0, G, R, PK, L, reinterpret_cast<const void *>(&cross_kernel<V, G, R, L, CROSS_WAVES>), \
-            reinterpret_cast<const void *>(&pairs_kernel<V, G, R, L, PAIRS_WAVES>),               \
-            &launch_cross<V, G, R, L>, &launch_pairs<V, G, R, L>}
+            reinterpret_cast<const void *>(&pairs_kernel<V, G, R, L, PAIRS_WAVES>), nullptr,      \
+            &launch_cross<V, G, R, L>, &launch_pairs<V, G, R, L>, nullptr}
 #define GSW_VARIANT4(G, R)                                                                        \
     GSW_VARIANT(s2, true, G, R, true), GSW_VARIANT(s2, true, G, R, false),                        \
         GSW_VARIANT(int, false, G, R, true), GSW_VARIANT(int, false, G, R, false)
+// tiles that also run stripes of longer queries
+#define GSW_SVARIANT(V, PK, G, R, L)                                                              \
+    Variant{0, G, R, PK, L, reinterpret_cast<const void *>(&cross_kernel<V, G, R, L, CROSS_WAVES>), \
+            reinterpret_cast<const void *>(&pairs_kernel<V, G, R, L, PAIRS_WAVES>),               \
+            reinterpret_cast<const void *>(&stripe_kernel<V, G, R, L, CROSS_WAVES>),              \
+            &launch_cross<V, G, R, L>, &launch_pairs<V, G, R, L>, &launch_stripe<V, G, R, L>}
+#define GSW_SVARIANT4(G, R)                                                                       \
+    GSW_SVARIANT(s2, true, G, R, true), GSW_SVARIANT(s2, true, G, R, false),                      \
+        GSW_SVARIANT(int, false, G, R, true), GSW_SVARIANT(int, false, G, R, false)
 
 std::vector<Variant> &variant_table() {
     static std::vector<Variant> table = [] {
         std::vector<Variant> v = {
             GSW_VARIANT4(8, 8),   GSW_VARIANT4(8, 12),  GSW_VARIANT4(8, 16),  GSW_VARIANT4(8, 20),
             GSW_VARIANT4(8, 24),
-            GSW_VARIANT4(8, 32),  GSW_VARIANT4(8, 36),  GSW_VARIANT4(16, 24), GSW_VARIANT4(16, 32),
+            GSW_SVARIANT4(8, 32), GSW_VARIANT4(8, 36),  GSW_VARIANT4(16, 24), GSW_VARIANT4(16, 32),
             GSW_VARIANT4(16, 48), GSW_VARIANT4(32, 32), GSW_VARIANT4(32, 48), GSW_VARIANT4(64, 32),
         };
         for (size_t i = 0; i < v.size(); ++i) v[i].id = (int)i;
@@ -257,6 +358,64 @@ const Variant *pick_variant(int lq, double mean_lr, bool packed, bool local, int
             best = &v;
         }
     }
+    return best;
+}
+
+// Relative cost per cell of running a query of length lq as nstr stripes of a (G, R)
+// tile: padding of the last stripe, per-step overhead including the boundary load and
+// store, pipeline fill per stripe, and the per-stripe profile build and barrier.
+static double stripe_cost(int G, int R, int lq, double lr, int nstr) {
+    const double pad = (double)nstr * G * R / std::max(lq, 1);
+    const double step = (9.0 * R + 17.0) / (9.0 * R);
+    const double fill = (lr + G - 1) / std::max(lr, 1.0);
+    return pad * step * fill * 1.03;
+}
+
+static int stripes_for(const Variant &v, int lq, bool track_last_col) {
+    const int shift = track_last_col ? (v.R - lq % v.R) % v.R : 0;
+    const int w = v.G * v.R;
+    return (lq + shift + w - 1) / w;
+}
+
+const Variant *pick_cross_tile(int lq, double mean_lr, bool packed, bool local, int ns, size_t lds_limit,
+                               bool track_last_col, int *nstr) {
+    *nstr = 1;
+    auto usable = [&](const Variant &v, int G, int R) {
+        return v.G == G && v.R == R && v.packed == packed && v.local == local && cross_lds_bytes(v, ns) <= lds_limit;
+    };
+    auto take = [&](const Variant &v) -> const Variant * {
+        if (v.G * v.R >= lq) return &v;
+        if (!v.stripe_fn) return nullptr;
+        *nstr = stripes_for(v, lq, track_last_col);
+        return &v;
+    };
+    if (g_force_g) {
+        for (const Variant &v : variant_table())
+            if (usable(v, g_force_g, g_force_r)) return take(v);
+        return nullptr;
+    }
+    {
+        std::lock_guard<std::mutex> lk(g_rules_mu);
+        for (const TileRule &r : g_rules) {
+            if (lq < r.lo || lq > r.hi || r.packed != (int)packed || r.local != (int)local) continue;
+            for (const Variant &v : variant_table())
+                if (usable(v, r.G, r.R) && (v.G * v.R >= lq || v.stripe_fn)) return take(v);
+        }
+    }
+    const Variant *best = pick_variant(lq, mean_lr, packed, local, ns, lds_limit, false);
+    double best_cost = best ? tile_cost(best->G, best->R, lq, mean_lr) : 1e30;
+    for (const Variant &v : variant_table()) {
+        if (!v.stripe_fn || v.packed != packed || v.local != local || v.G * v.R >= lq) continue;
+        if (cross_lds_bytes(v, ns) > lds_limit) continue;
+        const int n = stripes_for(v, lq, track_last_col);
+        const double c = stripe_cost(v.G, v.R, lq, mean_lr, n);
+        if (c < best_cost) {
+            best_cost = c;
+            best = &v;
+            *nstr = n;
+        }
+    }
+    if (best && best->G * best->R >= lq) *nstr = 1;
     return best;
 }
 
@@ -383,39 +542,44 @@ int plan_cross_units(const SchemeDev &s, int dtype_policy, const std::vector<Cro
     // semiglobal with free query ends tracks the last query column: front-pad the query so
     // that column lands on a lane's last register (see sweep_group)
     const bool track_last_col = s.mode == GSW_MODE_SEMIGLOBAL && s.free_q;
-    struct QInfo { CrossQuery q; const Variant *v; int shift; };
+    (void)wmax;
+    struct QInfo { CrossQuery q; const Variant *v; int shift, nstr; };
     std::vector<QInfo> packed_q, single_q;
     for (const CrossQuery &q : queries) {
         if (!fits_int32(s, q.len, max_lr))
             return fail(GSW_ERR_OVERFLOW, "score range of query %d exceeds the int32 kernels", q.id);
         const bool use16 = dtype_policy != GSW_DTYPE_INT32 && fits_int16(s, q.len, max_lr);
-        const Variant *v = q.len <= wmax ? pick_variant(q.len, mean_lr, use16, local, s.ns, lds_limit, false)
-                                         : nullptr;
+        int nstr = 1;
+        const Variant *v = pick_cross_tile(q.len, mean_lr, use16, local, s.ns, lds_limit, track_last_col, &nstr);
         if (!v) {
             plain->push_back(q);
             continue;
         }
-        (use16 ? packed_q : single_q).push_back(QInfo{q, v, track_last_col ? (v->R - q.len % v->R) % v->R : 0});
+        (use16 ? packed_q : single_q)
+            .push_back(QInfo{q, v, track_last_col ? (v->R - q.len % v->R) % v->R : 0, nstr});
     }
-    // pair packed queries of the same tile and the same shift (wave-uniform)
-    auto key = [&](const QInfo &x) { return std::make_tuple(x.v->id, track_last_col ? x.shift : 0, x.q.len); };
+    // pair packed queries of the same tile, stripe count and shift (wave-uniform)
+    auto key = [&](const QInfo &x) { return std::make_tuple(x.v->id, x.nstr, track_last_col ? x.shift : 0, x.q.len); };
     auto by_key = [&](const QInfo &x, const QInfo &y) { return key(x) < key(y); };
     std::stable_sort(packed_q.begin(), packed_q.end(), by_key);
     std::stable_sort(single_q.begin(), single_q.end(), by_key);
+    const CrossQuery none{0, 0, -1, -1};
     for (size_t i = 0; i < packed_q.size();) {
         const QInfo &x = packed_q[i];
-        if (i + 1 < packed_q.size() && packed_q[i + 1].v == x.v &&
+        if (i + 1 < packed_q.size() && packed_q[i + 1].v == x.v && packed_q[i + 1].nstr == x.nstr &&
             (!track_last_col || packed_q[i + 1].shift == x.shift)) {
-            units->push_back(CrossUnit{x.v, x.q, packed_q[i + 1].q, true, x.shift});
+            units->push_back(CrossUnit{x.v, x.q, packed_q[i + 1].q, true, x.shift, x.nstr});
             i += 2;
         } else {
-            units->push_back(CrossUnit{x.v, x.q, CrossQuery{0, 0, -1, -1}, false, x.shift});
+            units->push_back(CrossUnit{x.v, x.q, none, false, x.shift, x.nstr});
             i += 1;
         }
     }
-    for (const QInfo &x : single_q) units->push_back(CrossUnit{x.v, x.q, CrossQuery{0, 0, -1, -1}, false, x.shift});
-    std::stable_sort(units->begin(), units->end(),
-                     [](const CrossUnit &x, const CrossUnit &y) { return x.v->id < y.v->id; });
+    for (const QInfo &x : single_q) units->push_back(CrossUnit{x.v, x.q, none, false, x.shift, x.nstr});
+    // grouped by kernel: striped units of a tile form their own group, launched first
+    std::stable_sort(units->begin(), units->end(), [](const CrossUnit &x, const CrossUnit &y) {
+        return std::make_pair(x.nstr == 1, x.v->id) < std::make_pair(y.nstr == 1, y.v->id);
+    });
     return GSW_OK;
 }
 
@@ -425,9 +589,32 @@ void expand_cross_items(const CrossUnit *u0, const CrossUnit *u1, int64_t nref, 
         for (int64_t rb = 0; rb < nref; rb += chunk) {
             items->push_back(CrossItem{u->a.start, u->a.len, u->has_b ? u->b.start : 0, u->has_b ? u->b.len : -1,
                                        u->a.out, u->has_b ? u->b.out : -1, (int32_t)rb,
-                                       (int32_t)std::min(nref, rb + chunk), u->shift, 0});
+                                       (int32_t)std::min(nref, rb + chunk), u->shift, u->nstr});
         }
     }
+}
+
+int64_t stripe_item_rows(double mean_lr, int max_lr) {
+    // >= 4 blocks of 64/G references per wave and stripe, for balance at the stripe barrier
+    return std::max<int64_t>({(int64_t)1 << 17, (int64_t)(256.0 * mean_lr), (int64_t)max_lr});
+}
+
+int64_t expand_stripe_items(const CrossUnit *u0, const CrossUnit *u1, const SeqSet &rs, int64_t max_rows,
+                            std::vector<CrossItem> *items) {
+    const int64_t nref = rs.n_nonempty;
+    int64_t widest = 0;
+    for (const CrossUnit *u = u0; u < u1; ++u) {
+        for (int64_t rb = 0; rb < nref;) {
+            int64_t re = rb + 1;
+            while (re < nref && rs.bpos[re + 1] - rs.bpos[rb] <= max_rows) ++re;
+            widest = std::max(widest, rs.bpos[re] - rs.bpos[rb]);
+            items->push_back(CrossItem{u->a.start, u->a.len, u->has_b ? u->b.start : 0, u->has_b ? u->b.len : -1,
+                                       u->a.out, u->has_b ? u->b.out : -1, (int32_t)rb, (int32_t)re, u->shift,
+                                       u->nstr});
+            rb = re;
+        }
+    }
+    return widest;
 }
 
 // ---------------------------------------------------------------- cross
@@ -481,33 +668,42 @@ int batch_cross(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
     SatBuffers sb;
     const int32_t sat_cap = 1 << 20;
     GSW_TRY(prepare_sat(ctx, &sb, sat_cap));
-    struct Group { size_t u0, u1, i0, i1; int grid; size_t lds; };
+    struct Group { size_t u0, u1, i0, i1; int grid; size_t lds; bool striped; };
     std::vector<Group> groups;
     std::vector<CrossItem> items;
     int dominant = -1;
     int64_t dominant_units = 0;
+    int64_t bnd_entries = 0;   // boundary entries of the widest striped launch
     for (size_t u0 = 0; u0 < units.size();) {
         size_t u1 = u0;
-        while (u1 < units.size() && units[u1].v == units[u0].v) ++u1;
+        const bool striped = units[u0].nstr > 1;
+        while (u1 < units.size() && units[u1].v == units[u0].v && (units[u1].nstr > 1) == striped) ++u1;
         const Variant &v = *units[u0].v;
         if ((int64_t)(u1 - u0) > dominant_units) {
             dominant_units = (int64_t)(u1 - u0);
             dominant = v.id;
         }
-        Group g{u0, u1, items.size(), 0, 0, cross_lds_bytes(v, s.ns)};
-        GSW_TRY(occupancy_grid(ctx, v.cross_fn, CROSS_WAVES * 64, g.lds, &g.grid));
+        Group g{u0, u1, items.size(), 0, 0, cross_lds_bytes(v, s.ns), striped};
+        GSW_TRY(occupancy_grid(ctx, striped ? v.stripe_fn : v.cross_fn, CROSS_WAVES * 64, g.lds, &g.grid));
         const int gpw = 64 / v.G;
         const int64_t nref = rs.n_nonempty;
-        // chunk the reference range so that there are >= 4 items per workgroup
-        int64_t chunk = (nref * (int64_t)(u1 - u0) + 4LL * g.grid - 1) / (4LL * g.grid);
-        chunk = std::max<int64_t>(chunk, (int64_t)gpw * CROSS_WAVES);
-        chunk = std::min<int64_t>(chunk, nref);
-        chunk = (chunk + gpw - 1) / gpw * gpw;
-        expand_cross_items(units.data() + u0, units.data() + u1, nref, chunk, &items);
+        if (striped) {
+            const int64_t rows = expand_stripe_items(units.data() + u0, units.data() + u1, rs,
+                                                     stripe_item_rows(mean_lr, rs.max_len), &items);
+            bnd_entries = std::max(bnd_entries, rows * std::min<int64_t>(g.grid, (int64_t)(items.size() - g.i0)));
+        } else {
+            // chunk the reference range so that there are >= 4 items per workgroup
+            int64_t chunk = (nref * (int64_t)(u1 - u0) + 4LL * g.grid - 1) / (4LL * g.grid);
+            chunk = std::max<int64_t>(chunk, (int64_t)gpw * CROSS_WAVES);
+            chunk = std::min<int64_t>(chunk, nref);
+            chunk = (chunk + gpw - 1) / gpw * gpw;
+            expand_cross_items(units.data() + u0, units.data() + u1, nref, chunk, &items);
+        }
         g.i1 = items.size();
         groups.push_back(g);
         u0 = u1;
     }
+    if (bnd_entries > 0) GSW_TRY(ensure_buffer(&ctx->bnd, &ctx->bnd_bytes, sizeof(uint2) * (size_t)bnd_entries));
     if (groups.size() > 60) return fail(GSW_ERR_LIMIT, "too many tile groups (%zu)", groups.size());
     GSW_HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
     if (!groups.empty()) {
@@ -528,7 +724,15 @@ int batch_cross(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
         a.items = static_cast<const CrossItem *>(ctx->meta2) + g.i0;
         a.nitems = (int32_t)(g.i1 - g.i0);
         a.counter = ctx->counters + 2 + gi;
-        v.cross(a, std::min<int>(g.grid, a.nitems), g.lds, ctx->stream);
+        const int grid = std::min<int>(g.grid, a.nitems);
+        if (g.striped) {
+            a.bnd = static_cast<uint2 *>(ctx->bnd);
+            a.bnd_stride = bnd_entries / grid;
+            a.bpos = rs.d_bpos;
+            v.stripe(a, grid, g.lds, ctx->stream);
+        } else {
+            v.cross(a, grid, g.lds, ctx->stream);
+        }
         GSW_HIP_TRY(hipGetLastError());
         res->launches++;
     }

@@ -139,7 +139,7 @@ def test_pairs_ragged(mode):
 
 
 def test_protein_long_queries():
-    """Queries longer than the widest tile go to the comparator and stay exact."""
+    """Queries longer than the widest tile run as stripes and stay exact."""
     scheme = Scheme.preset("blosum62")
     rng = np.random.default_rng(11)
     queries = rand_seqs(rng, 3, 2000, 2600, "ARNDCQEGHILKMFPSTWYV")

@@ -1,0 +1,94 @@
+"""Queries wider than a tile run as stripes of G*R columns (stripe kernel): scores stay
+bit-exact with the oracle in every mode, in int16 and int32, when striped and unstriped
+queries are mixed and paired, when int16 saturates in a late stripe, and when served."""
+import numpy as np
+import pytest
+
+from gpusw_amd import Aligner, Scheme, _hip
+from gpusw_amd.encode import funnel
+from gpusw_amd.serve import Server
+from oracle import native
+from tests.helpers import PROT, rand_seqs, schemes, stable_seed
+
+pytestmark = pytest.mark.gpu
+
+SCHEMES = schemes()
+STRIPE_TILE = (8, 32)   # W = 256 columns per stripe
+
+
+def _oracle(queries, refs, scheme):
+    return native.score_encoded_cross(funnel(queries, scheme), funnel(refs, scheme), scheme)
+
+
+def _score(queries, refs, scheme, tile=None):
+    if tile:
+        _hip.force_variant(*tile)
+    try:
+        al = Aligner(scheme, backend="hip").index(refs)
+        got = al.score_cross(queries)
+        return got, al._b.last_stats
+    finally:
+        if tile:
+            _hip.force_variant(0, 0)
+
+
+@pytest.mark.parametrize("dtype", ["auto", "int32"])
+@pytest.mark.parametrize("name", list(SCHEMES))
+def test_forced_stripes_every_scheme(name, dtype):
+    scheme, alpha = SCHEMES[name]
+    scheme = scheme.replace(dtype=dtype)
+    rng = np.random.default_rng(stable_seed("stripes" + name + dtype))
+    # stripe edges (256, 512), lengths 1 past them, a ragged mix, and short queries that
+    # take the unstriped kernel in the same call; 11 queries leave one unpaired
+    qlens = [255, 256, 257, 300, 511, 512, 513, 777, 1030, 40, 129]
+    queries = ["".join(rng.choice(list(alpha), L)) for L in qlens]
+    refs = rand_seqs(rng, 45, 1, 420, alpha) + ["", queries[3][:150]]
+    got, st = _score(queries, refs, scheme, STRIPE_TILE)
+    want = _oracle(queries, refs, scheme)
+    bad = np.argwhere(got.astype(np.int64) != want)
+    assert bad.size == 0, f"{len(bad)} mismatches, first {bad[:5].tolist()}"
+    assert st.plain_pairs == 0
+
+
+@pytest.mark.parametrize("mode", ["local", "global", "semiglobal"])
+def test_protein_beyond_widest_tile(mode):
+    """No comparator fallback: 1500-3300 aa protein queries are striped by the cost model."""
+    scheme = Scheme.preset("blosum62").replace(mode=mode)
+    rng = np.random.default_rng(stable_seed("longprot" + mode))
+    queries = rand_seqs(rng, 5, 1500, 3300, PROT) + rand_seqs(rng, 4, 100, 400, PROT)
+    refs = rand_seqs(rng, 70, 20, 900, PROT)
+    # a reference that shares a long stretch with query 0: a high score across stripes
+    refs.append(queries[0][200:1400])
+    got, st = _score(queries, refs, scheme)
+    want = _oracle(queries, refs, scheme)
+    assert np.array_equal(got.astype(np.int64), want)
+    assert st.plain_pairs == 0
+
+
+def test_saturation_in_late_stripe_rescored():
+    """Local int16 reaches 32767 only after several stripes; the pair is rescored in int32."""
+    scheme = Scheme.preset("blosum62")
+    rng = np.random.default_rng(21)
+    w = "W" * 3100                      # self score 11 per residue: 34100 > 32767
+    queries = [w, "".join(rng.choice(list(PROT), 2000)), w[:600]]
+    refs = [w, "W" * 200, "".join(rng.choice(list(PROT), 700)), w[:2990] + "A" * 20]
+    got, st = _score(queries, refs, scheme)
+    want = _oracle(queries, refs, scheme)
+    assert int(want.max()) > 32767
+    assert np.array_equal(got.astype(np.int64), want)
+    assert st.rescored >= 2
+
+
+def test_served_long_queries_equal_aligner():
+    scheme = Scheme.preset("blosum62")
+    rng = np.random.default_rng(22)
+    refs = rand_seqs(rng, 300, 30, 700, PROT)
+    queries = rand_seqs(rng, 6, 1600, 2600, PROT) + rand_seqs(rng, 5, 50, 300, PROT)
+    want = Aligner(scheme, backend="hip").index(refs).top_k(queries, k=7)
+    with Server(memory_budget=1 << 30, launch_cells=2e8) as srv:
+        db = srv.add_database(refs, scheme)
+        ids = [srv.submit(db, queries, k=7), srv.submit(db, queries[:3], k=7)]
+        res = [srv.result(i) for i in ids]
+    assert np.array_equal(res[0].result.topk_idx, want.topk_idx)
+    assert np.array_equal(res[0].result.topk_scores, want.topk_scores)
+    assert np.array_equal(res[1].result.topk_scores, want.topk_scores[:3])
