@@ -56,7 +56,7 @@ struct KArgs {
 
 
 constexpr int CROSS_WAVES = 8;
-constexpr int PAIRS_WAVES = 4;
+constexpr int PAIRS_WAVES = 8;   // most waves of a pairs workgroup (LDS decides the actual count)
 
 typedef void (*LaunchFn)(const KArgs &, int grid, size_t lds, hipStream_t);
 
@@ -72,7 +72,8 @@ const Variant *pick_variant(int lq, double mean_lr, bool packed, bool local, int
                             bool for_pairs);
 int max_width();
 size_t cross_lds_bytes(const Variant &v, int ns);
-size_t pairs_lds_bytes(const Variant &v, int ns);
+size_t pairs_wave_bytes(const Variant &v, int ns);               // one wave's private profiles
+int pairs_waves(const Variant &v, int ns, size_t lds_limit);      // waves per workgroup (0: none fit)
 int occupancy_grid(DeviceCtx *ctx, const void *fn, int threads, size_t lds, int *grid);
 
 // A query scored against a whole reference set: code start/length in the query code
@@ -100,9 +101,10 @@ int plan_cross_units(const SchemeDev &s, int dtype_policy, const std::vector<Cro
 // with a stripe kernel run as *nstr > 1 stripes when that is cheaper or nothing else fits.
 const Variant *pick_cross_tile(int lq, double mean_lr, bool packed, bool local, int ns, size_t lds_limit,
                                bool track_last_col, int *nstr);
-// One item per (unit, chunk of `chunk` sorted references).
-void expand_cross_items(const CrossUnit *u0, const CrossUnit *u1, int64_t nref, int64_t chunk,
-                        std::vector<CrossItem> *items);
+// One item per (unit, run of whole blocks of gpw sorted references holding about
+// target_rows rows); the appended items are ordered longest first.
+void expand_cross_items(const CrossUnit *u0, const CrossUnit *u1, const SeqSet &rs, int64_t target_rows,
+                        int gpw, std::vector<CrossItem> *items);
 // Striped units: one item per (unit, run of sorted references of at most max_rows total
 // length, or one reference).  Returns the largest item's row count.
 int64_t stripe_item_rows(double mean_lr, int max_lr);

@@ -195,13 +195,13 @@ int Server::plan_locked(Request *r) {
             expand_stripe_items(units.data() + u0, units.data() + u1, rs, stripe_item_rows(db.mean_lr, rs.max_len),
                                 &r->items);
         } else {
+            // items of ~1e8 cells, at least one block of references per wave
             const int gpw = 64 / v.G;
-            const double per_ref = (double)(units[u0].a.len + std::max(units[u0].b.len, 0)) * db.mean_lr;
-            int64_t chunk = (int64_t)(1e8 / std::max(per_ref, 1.0));
-            chunk = std::max<int64_t>(chunk, (int64_t)gpw * CROSS_WAVES);
-            chunk = std::min<int64_t>(chunk, nref);
-            chunk = (chunk + gpw - 1) / gpw * gpw;
-            expand_cross_items(units.data() + u0, units.data() + u1, nref, chunk, &r->items);
+            const double qcols = (double)(units[u0].a.len + std::max(units[u0].b.len, 0));
+            int64_t rows = (int64_t)(1e8 / std::max(qcols, 1.0));
+            rows = std::max<int64_t>(rows, (int64_t)(gpw * CROSS_WAVES * db.mean_lr));
+            expand_cross_items(units.data() + u0, units.data() + u1, rs, rows, gpw, &r->items);
+            (void)nref;
         }
         r->item_tile.resize(r->items.size(), 2 * v.id + (striped ? 1 : 0));
         u0 = u1;
@@ -305,7 +305,6 @@ int Server::launch_locked(int si, Request *head) {
     sl.db = db;
     // round-robin over admitted requests of this (database, tile), oldest first
     double cells = 0;
-    const double per_item_ref = d.mean_lr;
     bool progress = true;
     while (progress && cells < launch_cells) {
         progress = false;
@@ -321,8 +320,7 @@ int Server::launch_locked(int si, Request *head) {
             } else {
                 sl.reqs.back().second++;
             }
-            cells += striped ? (double)(it.lqa + std::max(it.lqb, 0)) * (double)(d.rs->bpos[it.rend] - d.rs->bpos[it.rbeg])
-                             : (double)(it.lqa + std::max(it.lqb, 0)) * per_item_ref * (it.rend - it.rbeg);
+            cells += (double)(it.lqa + std::max(it.lqb, 0)) * (double)(d.rs->bpos[it.rend] - d.rs->bpos[it.rbeg]);
             progress = true;
             if (cells >= launch_cells) break;
         }
@@ -331,6 +329,12 @@ int Server::launch_locked(int si, Request *head) {
         pr.first->inflight++;
         pr.first->launches++;
     }
+    // longest first, so the launch's work queue ends on the smallest items
+    const int64_t *bp = d.rs->bpos.data();
+    std::stable_sort(sl.host_items.begin(), sl.host_items.end(), [&](const CrossItem &x, const CrossItem &y) {
+        return (bp[x.rend] - bp[x.rbeg]) * (int64_t)(x.lqa + std::max(x.lqb, 0)) * x.nstr >
+               (bp[y.rend] - bp[y.rbeg]) * (int64_t)(y.lqa + std::max(y.lqb, 0)) * y.nstr;
+    });
     const size_t bytes = sizeof(CrossItem) * sl.host_items.size();
     GSW_TRY(ensure_buffer(&sl.items, &sl.items_cap, bytes));
     GSW_HIP_TRY(hipMemcpyAsync(sl.items, sl.host_items.data(), bytes, hipMemcpyHostToDevice, ctx.stream));

@@ -152,7 +152,10 @@ struct SweepOut {
 // row is read one step before lane 0 needs it and written G-1 steps after, so one buffer
 // serves both.  The result then covers the stripe's columns only (the caller combines
 // stripes with max).
-template <typename V, int G, int R, bool LOCAL, bool STRIPE = false>
+// PF: for low occupancy (one or two waves per SIMD) the reference code of the next row is
+// loaded one step ahead, from a clamped index so the load needs no branch and nothing
+// waits for it until the next step's profile address.
+template <typename V, int G, int R, bool LOCAL, bool STRIPE = false, bool PF = false>
 __device__ __forceinline__ SweepOut sweep_group(const uint8_t *__restrict__ rseq, int lr, int nsteps,
                                                 const char *prof_lane, int t, const SweepConst &k,
                                                 int lqa, int lqb, int cbase, uint2 *bnd = nullptr,
@@ -192,22 +195,35 @@ __device__ __forceinline__ SweepOut sweep_group(const uint8_t *__restrict__ rseq
     V endcol = O::splat(O::NEG);
     int corner_a = RES_NONE, corner_b = RES_NONE, endrow_a = RES_NONE, endrow_b = RES_NONE;
 
+    // out-of-range codes score as the sentinel
+    auto code_at = [&](int r) -> int { return min((int)rseq[r], k.ns - 1); };
+    auto load_row = [&](int c, V *dst) {
+        const char *pc = prof_lane + c * L::ROWB;
+#pragma unroll
+        for (int kc = 0; kc < KC; ++kc) {
+            const int4 x = *reinterpret_cast<const int4 *>(pc + kc * L::CH);
+            dst[4 * kc + 0] = O::from_bits(x.x);
+            dst[4 * kc + 1] = O::from_bits(x.y);
+            dst[4 * kc + 2] = O::from_bits(x.z);
+            dst[4 * kc + 3] = O::from_bits(x.w);
+        }
+    };
+    // column-0 boundary H(i, 0) of lane 0's next row, kept as a running (saturating) value
+    const bool ladder = !LOCAL && (k.mode == 1 || (k.mode == 2 && !k.free_q));
+    V c0v = ladder ? O::splat(col0_value(k, 2)) : ZERO;
+    const int rlast = max(lr - 1, 0);   // idle groups (lr = 0) read their row 0 harmlessly
+    int c_raw = PF ? (int)rseq[min(max(-t, 0), rlast)] : 0;
+
     for (int s = 0; s < nsteps; ++s) {
         const int row = s - t;
         uint2 nb{0u, 0u};
         if (STRIPE && bnd_in && t == 0 && row + 1 < lr) nb = bnd[row + 1];   // lane 0's next row
+        int c_nxt = 0;
+        if constexpr (PF) c_nxt = (int)rseq[min(max(row + 1, 0), rlast)];
         if ((unsigned)row < (unsigned)lr) {
-            const int c = min((int)rseq[row], k.ns - 1);   // out-of-range codes score as the sentinel
-            const char *pc = prof_lane + c * L::ROWB;
             V pv[KC * 4];
-#pragma unroll
-            for (int kc = 0; kc < KC; ++kc) {
-                const int4 x = *reinterpret_cast<const int4 *>(pc + kc * L::CH);
-                pv[4 * kc + 0] = O::from_bits(x.x);
-                pv[4 * kc + 1] = O::from_bits(x.y);
-                pv[4 * kc + 2] = O::from_bits(x.z);
-                pv[4 * kc + 3] = O::from_bits(x.w);
-            }
+            if constexpr (PF) load_row(min(c_raw, k.ns - 1), pv);
+            else load_row(code_at(row), pv);
             V e = ein;
             V hgo = LOCAL ? O::subu(hin, GOP) : O::add(hin, GOV);
             V h = hin;
@@ -238,6 +254,7 @@ __device__ __forceinline__ SweepOut sweep_group(const uint8_t *__restrict__ rseq
             if (!LOCAL && track_endcol) endcol = O::mx(endcol, h);   // lane's last column
             if (STRIPE && bnd_out && t == G - 1) bnd[row] = uint2{(unsigned)O::bits(h), (unsigned)O::bits(e)};
         }
+        if constexpr (PF) c_raw = c_nxt;
         din = hin;
         hin = O::from_bits(wave_shr1(O::bits(hout)));
         ein = O::from_bits(wave_shr1(O::bits(eout)));
@@ -246,10 +263,11 @@ __device__ __forceinline__ SweepOut sweep_group(const uint8_t *__restrict__ rseq
                 hin = O::from_bits((int)nb.x);
                 ein = O::from_bits((int)nb.y);
             } else {
-                hin = LOCAL ? ZERO : O::splat(col0_value(k, s + 2));
+                hin = c0v;
                 ein = LOCAL ? ZERO : O::splat(O::NEG);
             }
         }
+        if (ladder) c0v = O::add(c0v, GEV);
     }
 
     SweepOut out;

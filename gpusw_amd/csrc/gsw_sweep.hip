@@ -4,6 +4,8 @@
 // (src/gpusw/backends/cuda.py:47-85, src/gpusw/aligner.py:95-167).
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -186,9 +188,10 @@ __attribute__((amdgpu_waves_per_eu(WavesPerEU<LOCAL, G, R>::value))) stripe_kern
 // Waves take blocks of 64/G tasks from a global counter.  Every lane writes the
 // profile entries of its own columns into the wave's LDS region and reads only those,
 // so no barrier is needed.  Tasks of one block share their front-padding shift (the host pads blocks).
+// Private profiles limit it to one workgroup per CU for DNA, so registers are not capped
+// for occupancy; the sweep is software-pipelined instead.
 template <typename V, int G, int R, bool LOCAL, int WAVES>
-__global__ void __launch_bounds__(WAVES * 64)
-__attribute__((amdgpu_waves_per_eu(WavesPerEU<LOCAL, G, R>::value))) pairs_kernel(KArgs a) {
+__global__ void __launch_bounds__(WAVES * 64) pairs_kernel(KArgs a) {
     using O = VOps<V>;
     using L = ProfLayout<G, R>;
     constexpr int KC = L::KC;
@@ -246,8 +249,8 @@ __attribute__((amdgpu_waves_per_eu(WavesPerEU<LOCAL, G, R>::value))) pairs_kerne
             }
         }
         const int nsteps = __builtin_amdgcn_readfirstlane(wave_max_i32(lr)) + G - 1;
-        const SweepOut res = sweep_group<V, G, R, LOCAL>(a.rcodes + rs, lr, nsteps, lane_lds, t,
-                                                         a.k, lqa, lqb, -shift);
+        const SweepOut res = sweep_group<V, G, R, LOCAL, false, true>(a.rcodes + rs, lr, nsteps, lane_lds, t,
+                                                                      a.k, lqa, lqb, -shift);
         if (t == 0 && valid) {
             emit(a, task.oa, qsa, lqa, task.ref, res.a);
             if (task.ob >= 0) emit(a, task.ob, qsb, lqb, task.ref, res.b);
@@ -343,14 +346,14 @@ const Variant *pick_variant(int lq, double mean_lr, bool packed, bool local, int
             if (lq < r.lo || lq > r.hi || r.packed != (int)packed || r.local != (int)local) continue;
             for (const Variant &v : variant_table())
                 if (v.G == r.G && v.R == r.R && v.packed == packed && v.local == local && v.G * v.R >= lq &&
-                    (for_pairs ? pairs_lds_bytes(v, ns) : cross_lds_bytes(v, ns)) <= lds_limit)
+                    (for_pairs ? pairs_waves(v, ns, lds_limit) > 0 : cross_lds_bytes(v, ns) <= lds_limit))
                     return &v;
         }
     }
     for (const Variant &v : variant_table()) {
         if (v.packed != packed || v.local != local) continue;
         if (v.G * v.R < lq) continue;
-        if ((for_pairs ? pairs_lds_bytes(v, ns) : cross_lds_bytes(v, ns)) > lds_limit) continue;
+        if (for_pairs ? pairs_waves(v, ns, lds_limit) == 0 : cross_lds_bytes(v, ns) > lds_limit) continue;
         if (g_force_g && (v.G != g_force_g || v.R != g_force_r)) continue;
         const double c = tile_cost(v.G, v.R, lq, mean_lr);
         if (c < best_cost) {
@@ -431,8 +434,9 @@ size_t cross_lds_bytes(const Variant &v, int ns) {
     const size_t ch = (size_t)std::max(v.G, 16) * 16;
     return (size_t)ns * ((v.R + 3) / 4) * ch;
 }
-size_t pairs_lds_bytes(const Variant &v, int ns) {
-    return (size_t)PAIRS_WAVES * ns * ((v.R + 3) / 4) * 64 * 16;
+size_t pairs_wave_bytes(const Variant &v, int ns) { return (size_t)ns * ((v.R + 3) / 4) * 64 * 16; }
+int pairs_waves(const Variant &v, int ns, size_t lds_limit) {
+    return (int)std::min<size_t>(PAIRS_WAVES, lds_limit / pairs_wave_bytes(v, ns));
 }
 
 int occupancy_grid(DeviceCtx *ctx, const void *fn, int threads, size_t lds, int *grid) {
@@ -583,15 +587,25 @@ int plan_cross_units(const SchemeDev &s, int dtype_policy, const std::vector<Cro
     return GSW_OK;
 }
 
-void expand_cross_items(const CrossUnit *u0, const CrossUnit *u1, int64_t nref, int64_t chunk,
-                        std::vector<CrossItem> *items) {
+void expand_cross_items(const CrossUnit *u0, const CrossUnit *u1, const SeqSet &rs, int64_t target_rows,
+                        int gpw, std::vector<CrossItem> *items) {
+    const int64_t nref = rs.n_nonempty;
+    const size_t first = items->size();
     for (const CrossUnit *u = u0; u < u1; ++u) {
-        for (int64_t rb = 0; rb < nref; rb += chunk) {
+        for (int64_t rb = 0; rb < nref;) {
+            // whole blocks of gpw references until the item holds target_rows rows
+            int64_t re = std::min(nref, rb + gpw);
+            while (re < nref && rs.bpos[re] - rs.bpos[rb] < target_rows) re = std::min(nref, re + gpw);
             items->push_back(CrossItem{u->a.start, u->a.len, u->has_b ? u->b.start : 0, u->has_b ? u->b.len : -1,
-                                       u->a.out, u->has_b ? u->b.out : -1, (int32_t)rb,
-                                       (int32_t)std::min(nref, rb + chunk), u->shift, u->nstr});
+                                       u->a.out, u->has_b ? u->b.out : -1, (int32_t)rb, (int32_t)re, u->shift,
+                                       u->nstr});
+            rb = re;
         }
     }
+    // longest first: the work queue then ends on the smallest items
+    std::stable_sort(items->begin() + first, items->end(), [&](const CrossItem &x, const CrossItem &y) {
+        return rs.bpos[x.rend] - rs.bpos[x.rbeg] > rs.bpos[y.rend] - rs.bpos[y.rbeg];
+    });
 }
 
 int64_t stripe_item_rows(double mean_lr, int max_lr) {
@@ -602,6 +616,7 @@ int64_t stripe_item_rows(double mean_lr, int max_lr) {
 int64_t expand_stripe_items(const CrossUnit *u0, const CrossUnit *u1, const SeqSet &rs, int64_t max_rows,
                             std::vector<CrossItem> *items) {
     const int64_t nref = rs.n_nonempty;
+    const size_t first = items->size();
     int64_t widest = 0;
     for (const CrossUnit *u = u0; u < u1; ++u) {
         for (int64_t rb = 0; rb < nref;) {
@@ -614,6 +629,10 @@ int64_t expand_stripe_items(const CrossUnit *u0, const CrossUnit *u1, const SeqS
             rb = re;
         }
     }
+    // most work (rows x stripes) first
+    std::stable_sort(items->begin() + first, items->end(), [&](const CrossItem &x, const CrossItem &y) {
+        return (rs.bpos[x.rend] - rs.bpos[x.rbeg]) * x.nstr > (rs.bpos[y.rend] - rs.bpos[y.rbeg]) * y.nstr;
+    });
     return widest;
 }
 
@@ -692,12 +711,11 @@ int batch_cross(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
                                                      stripe_item_rows(mean_lr, rs.max_len), &items);
             bnd_entries = std::max(bnd_entries, rows * std::min<int64_t>(g.grid, (int64_t)(items.size() - g.i0)));
         } else {
-            // chunk the reference range so that there are >= 4 items per workgroup
-            int64_t chunk = (nref * (int64_t)(u1 - u0) + 4LL * g.grid - 1) / (4LL * g.grid);
-            chunk = std::max<int64_t>(chunk, (int64_t)gpw * CROSS_WAVES);
-            chunk = std::min<int64_t>(chunk, nref);
-            chunk = (chunk + gpw - 1) / gpw * gpw;
-            expand_cross_items(units.data() + u0, units.data() + u1, nref, chunk, &items);
+            // >= 4 items per workgroup, each at least one block per wave
+            int64_t rows = (rs.total * (int64_t)(u1 - u0) + 4LL * g.grid - 1) / (4LL * g.grid);
+            rows = std::max<int64_t>(rows, (int64_t)(gpw * CROSS_WAVES * mean_lr));
+            expand_cross_items(units.data() + u0, units.data() + u1, rs, rows, gpw, &items);
+            (void)nref;
         }
         g.i1 = items.size();
         groups.push_back(g);
@@ -705,6 +723,9 @@ int batch_cross(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
     }
     if (bnd_entries > 0) GSW_TRY(ensure_buffer(&ctx->bnd, &ctx->bnd_bytes, sizeof(uint2) * (size_t)bnd_entries));
     if (groups.size() > 60) return fail(GSW_ERR_LIMIT, "too many tile groups (%zu)", groups.size());
+    // GSW_TRACE_LAUNCHES=1: time every launch and print tile, cells and GCUPS to stderr
+    static const bool trace = [] { const char *e = std::getenv("GSW_TRACE_LAUNCHES"); return e && *e == '1'; }();
+    std::vector<hipEvent_t> marks;
     GSW_HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
     if (!groups.empty()) {
         GSW_TRY(ensure_buffer(&ctx->meta2, &ctx->meta2_bytes, sizeof(CrossItem) * items.size()));
@@ -735,11 +756,34 @@ int batch_cross(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
         }
         GSW_HIP_TRY(hipGetLastError());
         res->launches++;
+        if (trace) {
+            hipEvent_t e;
+            GSW_HIP_TRY(hipEventCreate(&e));
+            GSW_HIP_TRY(hipEventRecord(e, ctx->stream));
+            marks.push_back(e);
+        }
     }
     GSW_HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
     GSW_HIP_TRY(hipEventSynchronize(ctx->ev1));
     float ms = 0.f;
     GSW_HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    for (size_t gi = 0; gi < marks.size(); ++gi) {
+        const Group &g = groups[gi];
+        const Variant &v = *units[g.u0].v;
+        float gms = 0.f;
+        (void)hipEventElapsedTime(&gms, gi ? marks[gi - 1] : ctx->ev0, marks[gi]);
+        double cells = 0, width = 0;
+        for (size_t u = g.u0; u < g.u1; ++u) {
+            const double ql = (double)units[u].a.len + (units[u].has_b ? units[u].b.len : 0);
+            cells += ql * (double)rs.total;
+            width += (double)v.G * v.R * units[u].nstr * (v.packed ? 2 : 1) * (double)rs.total;
+        }
+        std::fprintf(stderr, "[gsw] launch %zu: G=%d R=%d %s %s%s units=%zu items=%zu cells=%.3e ms=%.2f "
+                     "GCUPS=%.0f padded_GCUPS=%.0f\n", gi, v.G, v.R, v.packed ? "int16x2" : "int32",
+                     v.local ? "local" : "nonlocal", g.striped ? " striped" : "", g.u1 - g.u0, g.i1 - g.i0, cells,
+                     gms, cells / (gms * 1e6), width / (gms * 1e6));
+    }
+    for (hipEvent_t e : marks) (void)hipEventDestroy(e);
     res->kernel_ms += ms;
     res->variant = dominant;
 
@@ -808,7 +852,7 @@ struct PairGraph {
     char *d_buf = nullptr, *h_buf = nullptr;
     size_t cap = 0;
     hipEvent_t done = nullptr;
-    int grid = 0;
+    int grid = 0, waves = 0;
     size_t lds = 0;
 };
 
@@ -819,7 +863,9 @@ static int pair_graph(DeviceCtx *ctx, const Variant &v, int slot, int ns, PairGr
     std::lock_guard<std::mutex> lk(g_graph_mu);
     auto key = std::make_tuple(ctx->device, v.id, slot);
     auto &pg = g_pair_graphs[key];
-    const size_t lds = pairs_lds_bytes(v, ns);
+    const int waves = pairs_waves(v, ns, (size_t)ctx->max_smem);
+    if (waves == 0) return fail(GSW_ERR_LIMIT, "pairs tile (%d, %d) does not fit in LDS", v.G, v.R);
+    const size_t lds = (size_t)waves * pairs_wave_bytes(v, ns);
     if (pg && pg->lds == lds) {
         *out = pg.get();
         return GSW_OK;
@@ -837,7 +883,8 @@ static int pair_graph(DeviceCtx *ctx, const Variant &v, int slot, int ns, PairGr
         GSW_HIP_TRY(hipEventCreateWithFlags(&pg->done, hipEventDisableTiming));
     }
     pg->lds = lds;
-    GSW_TRY(occupancy_grid(ctx, v.pairs_fn, PAIRS_WAVES * 64, lds, &pg->grid));
+    pg->waves = waves;
+    GSW_TRY(occupancy_grid(ctx, v.pairs_fn, waves * 64, lds, &pg->grid));
     GSW_HIP_TRY(hipGraphCreate(&pg->graph, 0));
     GSW_HIP_TRY(hipGraphAddMemcpyNode1D(&pg->copy, pg->graph, nullptr, 0, pg->d_buf, pg->h_buf, pg->cap,
                                         hipMemcpyHostToDevice));
@@ -846,7 +893,7 @@ static int pair_graph(DeviceCtx *ctx, const Variant &v, int slot, int ns, PairGr
     hipKernelNodeParams kp{};
     kp.func = const_cast<void *>(v.pairs_fn);
     kp.gridDim = dim3(pg->grid);
-    kp.blockDim = dim3(PAIRS_WAVES * 64);
+    kp.blockDim = dim3(waves * 64);
     kp.sharedMemBytes = (unsigned)lds;
     kp.kernelParams = params;
     kp.extra = nullptr;
@@ -933,7 +980,7 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
             hipKernelNodeParams kp{};
             kp.func = const_cast<void *>(v.pairs_fn);
             kp.gridDim = dim3(pg[sl]->grid);
-            kp.blockDim = dim3(PAIRS_WAVES * 64);
+            kp.blockDim = dim3(pg[sl]->waves * 64);
             kp.sharedMemBytes = (unsigned)pg[sl]->lds;
             kp.kernelParams = params;
             kp.extra = nullptr;
