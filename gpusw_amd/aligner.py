@@ -145,8 +145,14 @@ class Aligner:
                 self.scheme, self._queries.max_len, self._resolved_dtype(), self.threads,
                 self._d_qbuf, self._d_qoff, self._d_rbuf, self._d_roff, qi, rj, n,
             )
-        cells = int((self._queries.lengths[qi].astype(np.int64)
-                     * self._refs.lengths[rj].astype(np.int64)).sum())
+        # sum of lq*lr (reference aligner.py:160-161); a backend that counted the cells
+        # while planning the call hands them over once (last_cells)
+        cells = getattr(self._b, "last_cells", None)
+        if cells is not None:
+            self._b.last_cells = None
+        else:
+            cells = int((self._queries.lengths[qi].astype(np.int64)
+                         * self._refs.lengths[rj].astype(np.int64)).sum())
         self._record(cells, dt, src)
         host = np.asarray(host, dtype=np.int32)
         if return_ids:

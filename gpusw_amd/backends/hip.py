@@ -94,6 +94,7 @@ class HipBackend(Backend):
         _hip.call("gsw_score_pairs", ctypes.byref(s), _policy(scheme), qset.handle, rset.handle,
                   _hip.ptr(qi), _hip.ptr(rj), int(n), out.ptr, ctypes.byref(stats))
         self.last_stats = stats
+        self.last_cells = int(stats.cells)   # counted by the library while planning
         return out.to_host(), stats.kernel_ms / 1e3, _describe("pairs", scheme, stats, dtype)
 
     def as_host(self, scores_native):

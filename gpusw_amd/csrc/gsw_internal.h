@@ -115,7 +115,7 @@ int plain_run(DeviceCtx *ctx, const SchemeDev &s, const PlainJob &job, float *ms
 
 // ---------------------------------------------------------------- batched kernels
 struct BatchResult { double kernel_ms = 0; int64_t rescored = 0; int64_t plain_pairs = 0;
-                     int launches = 0; int variant = -1; };
+                     int launches = 0; int variant = -1; int64_t cells = -1; /* -1: not counted */ };
 
 int batch_cross(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqSet &qs,
                 const int32_t *h_qlist, int32_t nq, const SeqSet &rs, int32_t *d_out,

@@ -11,6 +11,10 @@
 #include <memory>
 #include <numeric>
 #include <tuple>
+#include <atomic>
+#include <condition_variable>
+#include <thread>
+#include <string>
 
 #include "gsw_internal.h"
 #include "gsw_sweep.h"
@@ -301,9 +305,9 @@ static void launch_stripe(const KArgs &a, int grid, size_t lds, hipStream_t st) 
 std::vector<Variant> &variant_table() {
     static std::vector<Variant> table = [] {
         std::vector<Variant> v = {
-            GSW_VARIANT4(8, 8),   GSW_VARIANT4(8, 12),  GSW_VARIANT4(8, 16),  GSW_VARIANT4(8, 20),
-            GSW_VARIANT4(8, 24),
-            GSW_SVARIANT4(8, 32), GSW_VARIANT4(8, 36),  GSW_VARIANT4(16, 24), GSW_VARIANT4(16, 32),
+            GSW_VARIANT4(8, 8),   GSW_VARIANT4(8, 12),  GSW_VARIANT4(8, 16),  GSW_SVARIANT4(8, 20),
+            GSW_SVARIANT4(8, 24), GSW_SVARIANT4(8, 32), GSW_VARIANT4(8, 36),  GSW_VARIANT4(8, 40),
+            GSW_VARIANT4(8, 44),  GSW_VARIANT4(8, 48),  GSW_VARIANT4(16, 24), GSW_VARIANT4(16, 32),
             GSW_VARIANT4(16, 48), GSW_VARIANT4(32, 32), GSW_VARIANT4(32, 48), GSW_VARIANT4(64, 32),
         };
         for (size_t i = 0; i < v.size(); ++i) v[i].id = (int)i;
@@ -328,8 +332,9 @@ static double tile_cost(int G, int R, int lq, double lr) {
     const double pad = width / std::max(lq, 1);
     const double step = (9.0 * R + 14.0) / (9.0 * R);
     const double fill = (lr + G - 1) / std::max(lr, 1.0);
-    double regs = 1.0;
-    if (R > 40) regs = 1.08;   // fewer waves per SIMD
+    double regs = 1.0;                 // fewer waves per SIMD above R = 32 (more registers)
+    if (R > 44) regs = 1.08;           // 2 waves
+    else if (R > 32) regs = 1.03;      // 3 waves
     return pad * step * fill * regs;
 }
 
@@ -549,10 +554,15 @@ int plan_cross_units(const SchemeDev &s, int dtype_policy, const std::vector<Cro
     (void)wmax;
     struct QInfo { CrossQuery q; const Variant *v; int shift, nstr; };
     std::vector<QInfo> packed_q, single_q;
+    std::vector<CrossQuery> packed_free;   // int16 queries paired before their tile is chosen
     for (const CrossQuery &q : queries) {
         if (!fits_int32(s, q.len, max_lr))
             return fail(GSW_ERR_OVERFLOW, "score range of query %d exceeds the int32 kernels", q.id);
         const bool use16 = dtype_policy != GSW_DTYPE_INT32 && fits_int16(s, q.len, max_lr);
+        if (use16 && !track_last_col) {
+            packed_free.push_back(q);
+            continue;
+        }
         int nstr = 1;
         const Variant *v = pick_cross_tile(q.len, mean_lr, use16, local, s.ns, lds_limit, track_last_col, &nstr);
         if (!v) {
@@ -562,12 +572,31 @@ int plan_cross_units(const SchemeDev &s, int dtype_policy, const std::vector<Cro
         (use16 ? packed_q : single_q)
             .push_back(QInfo{q, v, track_last_col ? (v->R - q.len % v->R) % v->R : 0, nstr});
     }
+    // Without last-column tracking any two int16 queries can share a unit: pair neighbours
+    // in length order and give the pair the tile of the longer one (the shorter one's extra
+    // columns are padding), so every tile and stripe count is left with no lone query.
+    std::stable_sort(packed_free.begin(), packed_free.end(),
+                     [](const CrossQuery &x, const CrossQuery &y) { return x.len < y.len; });
+    const CrossQuery none{0, 0, -1, -1};
+    for (size_t i = 0; i < packed_free.size(); i += 2) {
+        const CrossQuery &a = packed_free[i];
+        const bool has_b = i + 1 < packed_free.size();
+        const CrossQuery &b = has_b ? packed_free[i + 1] : none;
+        const int lq = std::max(a.len, has_b ? b.len : 0);
+        int nstr = 1;
+        const Variant *v = pick_cross_tile(lq, mean_lr, true, local, s.ns, lds_limit, false, &nstr);
+        if (!v) {
+            plain->push_back(a);
+            if (has_b) plain->push_back(b);
+            continue;
+        }
+        units->push_back(CrossUnit{v, has_b ? b : a, has_b ? a : none, has_b, 0, nstr});
+    }
     // pair packed queries of the same tile, stripe count and shift (wave-uniform)
     auto key = [&](const QInfo &x) { return std::make_tuple(x.v->id, x.nstr, track_last_col ? x.shift : 0, x.q.len); };
     auto by_key = [&](const QInfo &x, const QInfo &y) { return key(x) < key(y); };
     std::stable_sort(packed_q.begin(), packed_q.end(), by_key);
     std::stable_sort(single_q.begin(), single_q.end(), by_key);
-    const CrossQuery none{0, 0, -1, -1};
     for (size_t i = 0; i < packed_q.size();) {
         const QInfo &x = packed_q[i];
         if (i + 1 < packed_q.size() && packed_q[i + 1].v == x.v && packed_q[i + 1].nstr == x.nstr &&
@@ -903,6 +932,27 @@ static int pair_graph(DeviceCtx *ctx, const Variant &v, int slot, int ns, PairGr
     return GSW_OK;
 }
 
+// Planning threads for explicit pairs (GSW_HOST_THREADS overrides; default up to 8).
+static int host_threads() {
+    static const int t = [] {
+        const char *e = std::getenv("GSW_HOST_THREADS");
+        int v = e ? std::atoi(e) : 0;
+        if (v <= 0) v = (int)std::min(8u, std::max(1u, std::thread::hardware_concurrency()));
+        return std::min(v, 64);
+    }();
+    return t;
+}
+
+// fn(tid) on T threads (tid 0 on the calling thread).
+template <typename F> static void run_threads(int T, F &&fn) {
+    std::vector<std::thread> th;
+    for (int i = 1; i < T; ++i) th.emplace_back(fn, i);
+    fn(0);
+    for (auto &x : th) x.join();
+}
+
+constexpr int MAX_PAIR_SLOTS = 8;   // pinned buffers + graphs per tile
+
 int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqSet &qs,
                 const SeqSet &rs, const int32_t *h_qi, const int32_t *h_rj, int64_t n,
                 int32_t *d_out, BatchResult *res) {
@@ -911,134 +961,290 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
     const bool track_last_col = s.mode == GSW_MODE_SEMIGLOBAL && s.free_q;
     const int wmax = max_width();
     const double mean_lr = rs.n_nonempty ? (double)rs.total / rs.n_nonempty : 1.0;
+    const int T = n >= 65536 ? host_threads() : 1;
+    const size_t nv = variant_table().size();
 
-    // -- classify: empty side (closed form), no tile (comparator), else a tile
-    std::vector<int64_t> e_idx;
-    std::vector<int32_t> e_val;
-    std::vector<int32_t> p_qi, p_rj;
-    std::vector<int64_t> p_oi;
+    // -- classify in parallel over contiguous index ranges: empty side (closed form), no
+    //    tile (comparator), else a tile
+    struct Part {
+        std::vector<int64_t> e_idx, p_oi;
+        std::vector<int32_t> e_val, p_qi, p_rj;
+        std::vector<int64_t> count;
+        int64_t cells = 0;
+        int64_t bad = -1;
+        int bad_code = GSW_OK;
+    };
+    std::vector<Part> parts(T);
     std::vector<int16_t> vid(n, -1);
-    std::vector<const Variant *> vcache(2 * (wmax + 1), nullptr);
-    std::vector<int64_t> per_variant(variant_table().size(), 0);
-    for (int64_t k = 0; k < n; ++k) {
-        const int q = h_qi[k], r = h_rj[k];
-        if (q < 0 || q >= qs.n || r < 0 || r >= rs.n)
-            return fail(GSW_ERR_INVALID, "pair %lld index out of range", (long long)k);
-        const int lq = qs.len[q], lr = rs.len[r];
-        if (lq == 0 || lr == 0) {
-            e_idx.push_back(k);
-            e_val.push_back(empty_edge_score(s, lq, lr));
-            continue;
+    run_threads(T, [&](int tid) {
+        Part &P = parts[tid];
+        P.count.assign(nv, 0);
+        std::vector<const Variant *> vcache(2 * (wmax + 1), nullptr);
+        std::vector<char> vdone(2 * (wmax + 1), 0);
+        const int64_t k0 = n * tid / T, k1 = n * (tid + 1) / T;
+        for (int64_t k = k0; k < k1; ++k) {
+            const int q = h_qi[k], r = h_rj[k];
+            if (q < 0 || q >= qs.n || r < 0 || r >= rs.n) {
+                P.bad = k;
+                P.bad_code = GSW_ERR_INVALID;
+                return;
+            }
+            const int lq = qs.len[q], lr = rs.len[r];
+            P.cells += (int64_t)lq * lr;
+            if (lq == 0 || lr == 0) {
+                P.e_idx.push_back(k);
+                P.e_val.push_back(empty_edge_score(s, lq, lr));
+                continue;
+            }
+            if (!fits_int32(s, lq, lr)) {
+                P.bad = k;
+                P.bad_code = GSW_ERR_OVERFLOW;
+                return;
+            }
+            const bool use16 = dtype_policy != GSW_DTYPE_INT32 && fits_int16(s, lq, lr);
+            const Variant *v = nullptr;
+            if (lq <= wmax) {
+                const size_t ci = (size_t)(use16 ? 1 : 0) * (wmax + 1) + lq;
+                if (!vdone[ci]) {
+                    vcache[ci] = pick_variant(lq, mean_lr, use16, local, s.ns, (size_t)ctx->max_smem, true);
+                    vdone[ci] = 1;
+                }
+                v = vcache[ci];
+            }
+            if (!v) {
+                P.p_qi.push_back(q);
+                P.p_rj.push_back(r);
+                P.p_oi.push_back(k);
+                continue;
+            }
+            vid[k] = (int16_t)v->id;
+            P.count[v->id]++;
         }
-        if (!fits_int32(s, lq, lr))
-            return fail(GSW_ERR_OVERFLOW, "score range of pair %lld exceeds the int32 kernels", (long long)k);
-        const bool use16 = dtype_policy != GSW_DTYPE_INT32 && fits_int16(s, lq, lr);
-        const Variant *v = nullptr;
-        if (lq <= wmax) {
-            const Variant *&slot = vcache[(use16 ? 1 : 0) * (wmax + 1) + lq];
-            if (!slot) slot = pick_variant(lq, mean_lr, use16, local, s.ns, (size_t)ctx->max_smem, true);
-            v = slot;
+    });
+    for (const Part &P : parts)   // ranges are in index order: the first error is the lowest index
+        if (P.bad >= 0)
+            return fail(P.bad_code, P.bad_code == GSW_ERR_INVALID ? "pair %lld index out of range"
+                                                                  : "score range of pair %lld exceeds the int32 kernels",
+                        (long long)P.bad);
+    std::vector<int64_t> e_idx, p_oi;
+    std::vector<int32_t> e_val, p_qi, p_rj;
+    res->cells = 0;
+    for (Part &P : parts) {
+        res->cells += P.cells;
+        e_idx.insert(e_idx.end(), P.e_idx.begin(), P.e_idx.end());
+        e_val.insert(e_val.end(), P.e_val.begin(), P.e_val.end());
+        p_qi.insert(p_qi.end(), P.p_qi.begin(), P.p_qi.end());
+        p_rj.insert(p_rj.end(), P.p_rj.begin(), P.p_rj.end());
+        p_oi.insert(p_oi.end(), P.p_oi.begin(), P.p_oi.end());
+    }
+
+    // -- members of each tile, in index order (parallel scatter)
+    std::vector<int64_t> vbeg(nv + 1, 0);
+    std::vector<std::vector<int64_t>> pos(T, std::vector<int64_t>(nv));
+    for (size_t v = 0; v < nv; ++v) {
+        int64_t p = vbeg[v];
+        for (int t = 0; t < T; ++t) {
+            pos[t][v] = p;
+            p += parts[t].count[v];
         }
-        if (!v) {
-            p_qi.push_back(q);
-            p_rj.push_back(r);
-            p_oi.push_back(k);
-            continue;
-        }
-        vid[k] = (int16_t)v->id;
-        per_variant[v->id]++;
+        vbeg[v + 1] = p;
+    }
+    std::vector<int64_t> members((size_t)vbeg[nv]);
+    run_threads(T, [&](int tid) {
+        const int64_t k0 = n * tid / T, k1 = n * (tid + 1) / T;
+        std::vector<int64_t> &ps = pos[tid];
+        for (int64_t k = k0; k < k1; ++k)
+            if (vid[k] >= 0) members[ps[vid[k]]++] = k;
+    });
+
+    // -- batches of PAIR_BATCH members of one tile; a tile's batches cycle through its slots
+    struct Batch { const Variant *v; int64_t b0, b1; int bi, nslot; };
+    std::vector<Batch> batches;
+    int dominant = -1;
+    int64_t dominant_n = 0;
+    for (const Variant &v : variant_table()) {
+        const int64_t cnt_v = vbeg[v.id + 1] - vbeg[v.id];
+        if (cnt_v == 0) continue;
+        if (cnt_v > dominant_n) { dominant_n = cnt_v; dominant = v.id; }
+        const int nb = (int)((cnt_v + PAIR_BATCH - 1) / PAIR_BATCH);
+        for (int bi = 0; bi < nb; ++bi)
+            batches.push_back(Batch{&v, vbeg[v.id] + bi * PAIR_BATCH, std::min(vbeg[v.id + 1], vbeg[v.id] + (bi + 1) * PAIR_BATCH),
+                                    bi, std::min(nb, MAX_PAIR_SLOTS)});
     }
 
     SatBuffers sb;
     GSW_TRY(prepare_sat(ctx, &sb, 1 << 20));
-    GSW_HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
-    int dominant = -1;
-    int64_t dominant_n = 0;
-    std::vector<int64_t> members, order, order2;
-    std::vector<int32_t> cnt;
-    int batch_no = 0;
-    for (const Variant &v : variant_table()) {
-        if (per_variant[v.id] == 0) continue;
-        if (per_variant[v.id] > dominant_n) { dominant_n = per_variant[v.id]; dominant = v.id; }
-        members.clear();
-        for (int64_t k = 0; k < n; ++k)
-            if (vid[k] == v.id) members.push_back(k);
-        const int gpw = 64 / v.G;
-        PairGraph *pg[2];
-        GSW_TRY(pair_graph(ctx, v, 0, s.ns, &pg[0]));
-        GSW_TRY(pair_graph(ctx, v, 1, s.ns, &pg[1]));
-        for (int sl = 0; sl < 2; ++sl) {   // this call's pointers into the kernel node
-            KArgs a = base_args(s, qs, rs, d_out);
-            a.sat = sb.d_sat;
-            a.sat_count = ctx->counters + 1;
-            a.sat_cap = sb.cap;
-            a.sat_check = (v.packed && local) ? 1 : 0;
-            a.items = pg[sl]->d_buf + PAIR_HDR;
-            a.nitems_ptr = reinterpret_cast<const int32_t *>(pg[sl]->d_buf);
-            a.counter = reinterpret_cast<int32_t *>(pg[sl]->d_buf) + 1;
-            void *params[] = {&a};
-            hipKernelNodeParams kp{};
-            kp.func = const_cast<void *>(v.pairs_fn);
-            kp.gridDim = dim3(pg[sl]->grid);
-            kp.blockDim = dim3(pg[sl]->waves * 64);
-            kp.sharedMemBytes = (unsigned)pg[sl]->lds;
-            kp.kernelParams = params;
-            kp.extra = nullptr;
-            GSW_HIP_TRY(hipGraphExecKernelNodeSetParams(pg[sl]->exec, pg[sl]->kern, &kp));
-        }
-        for (size_t b0 = 0; b0 < members.size(); b0 += PAIR_BATCH) {
-            const size_t b1 = std::min(members.size(), b0 + (size_t)PAIR_BATCH);
-            PairGraph &g = *pg[batch_no & 1];
-            // counting sort of the batch by reference length rank (longest first) ...
-            cnt.assign((size_t)rs.n_nonempty + 1, 0);
-            for (size_t i = b0; i < b1; ++i) cnt[rs.rank[h_rj[members[i]]] + 1]++;
-            for (int32_t r = 0; r < rs.n_nonempty; ++r) cnt[r + 1] += cnt[r];
-            order.resize(b1 - b0);
-            for (size_t i = b0; i < b1; ++i) order[cnt[rs.rank[h_rj[members[i]]]]++] = members[i];
-            // ... then (stable) by last-column register when it is tracked
-            if (track_last_col) {
-                cnt.assign(v.R + 1, 0);
-                for (int64_t k : order) cnt[(v.R - qs.len[h_qi[k]] % v.R) % v.R + 1]++;
-                for (int r = 0; r < v.R; ++r) cnt[r + 1] += cnt[r];
-                order2.resize(order.size());
-                for (int64_t k : order) order2[cnt[(v.R - qs.len[h_qi[k]] % v.R) % v.R]++] = k;
-                order.swap(order2);
-            }
-            // the pinned buffer of this slot is free once its previous replay finished
-            GSW_HIP_TRY(hipEventSynchronize(g.done));
-            PairTask *tasks = reinterpret_cast<PairTask *>(g.h_buf + PAIR_HDR);
-            int32_t nt = 0;
-            int cur_shift = -1;
-            for (size_t i = 0; i < order.size();) {
-                const int64_t ka = order[i];
-                const int shift = track_last_col ? (v.R - qs.len[h_qi[ka]] % v.R) % v.R : 0;
-                if (shift != cur_shift) {
-                    while (nt % gpw) tasks[nt++] = PairTask{-1, -1, -1, cur_shift, -1, -1};
-                    cur_shift = shift;
-                }
-                if (v.packed && i + 1 < order.size() && h_rj[order[i + 1]] == h_rj[ka] &&
-                    (!track_last_col || (v.R - qs.len[h_qi[order[i + 1]]] % v.R) % v.R == shift)) {
-                    const int64_t kb = order[i + 1];
-                    tasks[nt++] = PairTask{h_qi[ka], h_qi[kb], h_rj[ka], shift, ka, kb};
-                    i += 2;
-                } else {
-                    tasks[nt++] = PairTask{h_qi[ka], -1, h_rj[ka], shift, ka, -1};
-                    i += 1;
-                }
-            }
-            while (nt % gpw) tasks[nt++] = PairTask{-1, -1, -1, cur_shift, -1, -1};
-            int32_t *hdr = reinterpret_cast<int32_t *>(g.h_buf);
-            hdr[0] = nt;   // task count
-            hdr[1] = 0;    // work counter
-            GSW_HIP_TRY(hipGraphExecMemcpyNodeSetParams1D(g.exec, g.copy, g.d_buf, g.h_buf,
-                                                          PAIR_HDR + sizeof(PairTask) * (size_t)nt,
-                                                          hipMemcpyHostToDevice));
-            GSW_HIP_TRY(hipGraphLaunch(g.exec, ctx->stream));
-            GSW_HIP_TRY(hipEventRecord(g.done, ctx->stream));
-            res->launches++;
-            ++batch_no;
-        }
+    // graphs of every slot, with this call's pointers in their kernel nodes
+    std::map<std::pair<int, int>, PairGraph *> slot_of;
+    for (const Batch &B : batches) {
+        if (B.bi >= B.nslot) continue;
+        PairGraph *pg = nullptr;
+        GSW_TRY(pair_graph(ctx, *B.v, B.bi, s.ns, &pg));
+        slot_of[{B.v->id, B.bi}] = pg;
+        KArgs a = base_args(s, qs, rs, d_out);
+        a.sat = sb.d_sat;
+        a.sat_count = ctx->counters + 1;
+        a.sat_cap = sb.cap;
+        a.sat_check = (B.v->packed && local) ? 1 : 0;
+        a.items = pg->d_buf + PAIR_HDR;
+        a.nitems_ptr = reinterpret_cast<const int32_t *>(pg->d_buf);
+        a.counter = reinterpret_cast<int32_t *>(pg->d_buf) + 1;
+        void *params[] = {&a};
+        hipKernelNodeParams kp{};
+        kp.func = const_cast<void *>(B.v->pairs_fn);
+        kp.gridDim = dim3(pg->grid);
+        kp.blockDim = dim3(pg->waves * 64);
+        kp.sharedMemBytes = (unsigned)pg->lds;
+        kp.kernelParams = params;
+        kp.extra = nullptr;
+        GSW_HIP_TRY(hipGraphExecKernelNodeSetParams(pg->exec, pg->kern, &kp));
     }
+
+    // Plan one batch into its slot's pinned buffer: counting sort by reference length rank
+    // (longest first), then (stable) by last-column register when it is tracked; pair
+    // adjacent members of one reference; pad blocks of 64/G tasks at shift changes.
+    struct Scratch { std::vector<int64_t> order, order2; std::vector<int32_t> cnt; };
+    auto plan = [&](const Batch &B, PairGraph &g, Scratch &S) -> int32_t {
+        const Variant &v = *B.v;
+        const int gpw = 64 / v.G;
+        S.cnt.assign((size_t)rs.n_nonempty + 1, 0);
+        for (int64_t i = B.b0; i < B.b1; ++i) S.cnt[rs.rank[h_rj[members[i]]] + 1]++;
+        for (int32_t r = 0; r < rs.n_nonempty; ++r) S.cnt[r + 1] += S.cnt[r];
+        S.order.resize(B.b1 - B.b0);
+        for (int64_t i = B.b0; i < B.b1; ++i) S.order[S.cnt[rs.rank[h_rj[members[i]]]]++] = members[i];
+        auto shift_of = [&](int64_t k) { return track_last_col ? (v.R - qs.len[h_qi[k]] % v.R) % v.R : 0; };
+        if (track_last_col) {
+            S.cnt.assign(v.R + 1, 0);
+            for (int64_t k : S.order) S.cnt[shift_of(k) + 1]++;
+            for (int r = 0; r < v.R; ++r) S.cnt[r + 1] += S.cnt[r];
+            S.order2.resize(S.order.size());
+            for (int64_t k : S.order) S.order2[S.cnt[shift_of(k)]++] = k;
+            S.order.swap(S.order2);
+        }
+        PairTask *tasks = reinterpret_cast<PairTask *>(g.h_buf + PAIR_HDR);
+        int32_t nt = 0;
+        int cur_shift = -1;
+        for (size_t i = 0; i < S.order.size();) {
+            const int64_t ka = S.order[i];
+            const int shift = shift_of(ka);
+            if (shift != cur_shift) {
+                while (nt % gpw) tasks[nt++] = PairTask{-1, -1, -1, cur_shift, -1, -1};
+                cur_shift = shift;
+            }
+            if (v.packed && i + 1 < S.order.size() && h_rj[S.order[i + 1]] == h_rj[ka] &&
+                shift_of(S.order[i + 1]) == shift) {
+                const int64_t kb = S.order[i + 1];
+                tasks[nt++] = PairTask{h_qi[ka], h_qi[kb], h_rj[ka], shift, ka, kb};
+                i += 2;
+            } else {
+                tasks[nt++] = PairTask{h_qi[ka], -1, h_rj[ka], shift, ka, -1};
+                i += 1;
+            }
+        }
+        while (nt % gpw) tasks[nt++] = PairTask{-1, -1, -1, cur_shift, -1, -1};
+        int32_t *hdr = reinterpret_cast<int32_t *>(g.h_buf);
+        hdr[0] = nt;   // task count
+        hdr[1] = 0;    // work counter (the graph's copy zeroes it on the device)
+        return nt;
+    };
+
+    // -- planners (worker threads) and the launching thread.  A planner reuses a slot's
+    //    pinned buffer only after the batch that last used it was launched and finished.
+    const size_t nb = batches.size();
+    std::vector<int> state(nb, 0);   // 0 pending, 1 planned, 2 launched
+    std::vector<int32_t> ntasks(nb, 0);
+    std::mutex m;
+    std::condition_variable cv;
+    bool abort = false;
+    int werr = GSW_OK;
+    std::string werr_msg;
+    std::atomic<size_t> next{0};
+    auto worker = [&]() {
+        Scratch S;
+        for (;;) {
+            const size_t b = next.fetch_add(1);
+            if (b >= nb) return;
+            const Batch &B = batches[b];
+            PairGraph &g = *slot_of.at({B.v->id, B.bi % B.nslot});
+            if (B.bi >= B.nslot) {
+                const size_t prev = b - B.nslot;   // same tile: its batches are contiguous
+                std::unique_lock<std::mutex> lk(m);
+                cv.wait(lk, [&] { return state[prev] == 2 || abort; });
+                if (abort) return;
+                lk.unlock();
+                const hipError_t e = hipEventSynchronize(g.done);
+                if (e != hipSuccess) {
+                    std::lock_guard<std::mutex> lk2(m);
+                    abort = true;
+                    werr = GSW_ERR_HIP;
+                    werr_msg = hipGetErrorString(e);
+                    cv.notify_all();
+                    return;
+                }
+            } else {
+                const hipError_t e = hipEventSynchronize(g.done);   // an earlier call's replay
+                if (e != hipSuccess) {
+                    std::lock_guard<std::mutex> lk2(m);
+                    abort = true;
+                    werr = GSW_ERR_HIP;
+                    werr_msg = hipGetErrorString(e);
+                    cv.notify_all();
+                    return;
+                }
+            }
+            const int32_t nt = plan(B, g, S);
+            std::lock_guard<std::mutex> lk(m);
+            ntasks[b] = nt;
+            state[b] = 1;
+            cv.notify_all();
+        }
+    };
+    const int W = (int)std::min<size_t>((size_t)T, nb);
+    std::vector<std::thread> workers;
+    for (int i = 0; i < W; ++i) workers.emplace_back(worker);
+    auto stop = [&]() {
+        {
+            std::lock_guard<std::mutex> lk(m);
+            abort = true;
+        }
+        cv.notify_all();
+        for (auto &x : workers) x.join();
+        workers.clear();
+    };
+    int rc = GSW_OK;
+    if (hipEventRecord(ctx->ev0, ctx->stream) != hipSuccess) rc = fail(GSW_ERR_HIP, "event record failed");
+    for (size_t b = 0; b < nb && rc == GSW_OK; ++b) {
+        {
+            std::unique_lock<std::mutex> lk(m);
+            cv.wait(lk, [&] { return state[b] >= 1 || abort; });
+            if (abort) {
+                rc = fail(werr != GSW_OK ? werr : GSW_ERR_HIP, "pairs planning failed: %s", werr_msg.c_str());
+                break;
+            }
+        }
+        const Batch &B = batches[b];
+        PairGraph &g = *slot_of.at({B.v->id, B.bi % B.nslot});
+        hipError_t e = hipGraphExecMemcpyNodeSetParams1D(g.exec, g.copy, g.d_buf, g.h_buf,
+                                                         PAIR_HDR + sizeof(PairTask) * (size_t)ntasks[b],
+                                                         hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipGraphLaunch(g.exec, ctx->stream);
+        if (e == hipSuccess) e = hipEventRecord(g.done, ctx->stream);
+        if (e != hipSuccess) {
+            rc = fail(GSW_ERR_HIP, "pairs batch launch: %s", hipGetErrorString(e));
+            break;
+        }
+        res->launches++;
+        {
+            std::lock_guard<std::mutex> lk(m);
+            state[b] = 2;
+        }
+        cv.notify_all();
+    }
+    if (rc != GSW_OK) {
+        stop();
+        return rc;
+    }
+    for (auto &x : workers) x.join();
     GSW_HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
     GSW_HIP_TRY(hipEventSynchronize(ctx->ev1));
     float ms = 0.f;
@@ -1138,8 +1344,11 @@ extern "C" int gsw_score_pairs(const gsw_scheme *s, int32_t dtype_policy, gsw_se
     std::lock_guard<std::mutex> lk(ctx->mu);
     BatchResult r;
     GSW_TRY(batch_pairs(ctx, sd, dtype_policy, *qs, *rs, h_qi, h_rj, n, d_out, &r));
-    int64_t cells = 0;
-    for (int64_t k = 0; k < n; ++k) cells += (int64_t)qs->len[h_qi[k]] * rs->len[h_rj[k]];
+    int64_t cells = r.cells;
+    if (cells < 0) {
+        cells = 0;
+        for (int64_t k = 0; k < n; ++k) cells += (int64_t)qs->len[h_qi[k]] * rs->len[h_rj[k]];
+    }
     fill_stats(stats, r, cells, n, t0);
     return GSW_OK;
 }

@@ -34,3 +34,32 @@ def test_pair_batches_match_eager_and_oracle(mode):
     perm = rng.permutation(qi.size)
     shuffled = al.score_pairs(qi[perm], rj[perm])
     assert np.array_equal(shuffled, first[perm])
+
+
+def test_many_batches_two_tiles_slot_reuse():
+    """1.3M pairs of two read lengths: one tile gets 9 batches (more than its 8 slots), so planning
+    threads reuse pinned slots after their earlier replays finish."""
+    rng = np.random.default_rng(17)
+    nw = 300
+    wlen = rng.integers(200, 401, nw)
+    windows = [rng.integers(0, 4, int(L)).astype(np.uint8) for L in wlen]
+    n = 1_300_000
+    rj = rng.integers(0, nw, n).astype(np.int32)
+    rlen = np.where(rng.random(n) < 0.15, 40, 150)   # ~1.1M of 150 nt: 9 batches of one tile
+    reads = []
+    for j, L in zip(rj.tolist(), rlen.tolist()):
+        w = windows[j]
+        s0 = int(rng.integers(0, w.size - L + 1))
+        reads.append(w[s0:s0 + L])
+    qi = np.arange(n, dtype=np.int32)
+    scheme = Scheme.preset("dna").replace(mode="semiglobal")
+    al = Aligner(scheme, backend="hip").index(synth.pack(windows)).set_queries(synth.pack(reads))
+    got = al.score_pairs(qi, rj)
+    assert al._b.last_stats.launches >= 10 and al._b.last_stats.plain_pairs == 0
+    # cells counted by the planning threads == sum of lq * lr
+    assert al._b.last_stats.cells == int((rlen.astype(np.int64) * wlen[rj].astype(np.int64)).sum())
+    eager = Aligner(scheme, backend="hip-plain").index(synth.pack(windows)).set_queries(synth.pack(reads)).score_pairs(qi, rj)
+    assert np.array_equal(got, eager)
+    pick = rng.choice(n, 2000, replace=False)
+    want = native.score_encoded_pairs(al._queries, al._refs, qi[pick], rj[pick], scheme)
+    assert np.array_equal(got[pick].astype(np.int64), want)
