@@ -49,8 +49,8 @@ struct KArgs {
     int32_t nitems;
     int32_t *counter;
     const int32_t *nitems_ptr;   // pairs kernel: task count read from device memory
-    uint2 *bnd;                  // stripe kernel: boundary regions, one per workgroup
-    int64_t bnd_stride;          //   uint2 entries per region (>= rows of any item)
+    uint32_t *bnd;               // stripe kernel: boundary regions, one per workgroup,
+    int64_t bnd_stride;          //   each an H plane then an E plane of bnd_stride words (>= item rows)
     const int64_t *bpos;         //   row offsets of the sorted references (n + 1)
 };
 

@@ -347,7 +347,7 @@ int Server::launch_locked(int si, Request *head) {
     if (striped) {
         for (const CrossItem &it : sl.host_items) rows = std::max(rows, d.rs->bpos[it.rend] - d.rs->bpos[it.rbeg]);
         // launches of both slots may run at once: each slot has its own half of the scratch
-        GSW_TRY(ensure_buffer(&sl.bnd, &sl.bnd_bytes, sizeof(uint2) * (size_t)rows * grid));
+        GSW_TRY(ensure_buffer(&sl.bnd, &sl.bnd_bytes, 2 * sizeof(uint32_t) * (size_t)rows * grid));
     }
     KArgs a{};
     a.qcodes = code_base;
@@ -366,7 +366,7 @@ int Server::launch_locked(int si, Request *head) {
     a.nitems = (int32_t)sl.host_items.size();
     a.counter = sl.counters;
     if (striped) {
-        a.bnd = static_cast<uint2 *>(sl.bnd);
+        a.bnd = static_cast<uint32_t *>(sl.bnd);
         a.bnd_stride = rows;
         a.bpos = d.rs->d_bpos;
         v.stripe(a, grid, lds, ctx.stream);

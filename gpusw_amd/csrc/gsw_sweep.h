@@ -146,20 +146,23 @@ struct SweepOut {
 //                padding columns whose profile is 0 carry exactly H = 0 and E = gap_open,
 //                the column-0 boundary, so shifting the query by (R - lq mod R) mod R puts
 //                its last column on the last register of a lane.
-// STRIPE: the group covers query columns [cbase, cbase + G*R) of a longer query.  bnd[i]
-// holds (H, E) of row i+1 at the column left of the stripe when bnd_in (else the column-0
-// boundary applies), and receives (H, E) of the stripe's last column when bnd_out; the
+// STRIPE: the group covers query columns [cbase, cbase + G*R) of a longer query.  bh[i]
+// and be[i] hold H and E of row i+1 at the column left of the stripe when bnd_in (else
+// the column-0 boundary applies), and receive those of the stripe's last column when
+// bnd_out; the
 // row is read one step before lane 0 needs it and written G-1 steps after, so one buffer
 // serves both.  The result then covers the stripe's columns only (the caller combines
 // stripes with max).
 // PF: for low occupancy (one or two waves per SIMD) the reference code of the next row is
 // loaded one step ahead, from a clamped index so the load needs no branch and nothing
 // waits for it until the next step's profile address.
-template <typename V, int G, int R, bool LOCAL, bool STRIPE = false, bool PF = false>
+template <typename V, int G, int R, bool LOCAL, bool STRIPE = false, bool PF_ = false>
 __device__ __forceinline__ SweepOut sweep_group(const uint8_t *__restrict__ rseq, int lr, int nsteps,
                                                 const char *prof_lane, int t, const SweepConst &k,
-                                                int lqa, int lqb, int cbase, uint2 *bnd = nullptr,
-                                                bool bnd_in = false, bool bnd_out = false) {
+                                                int lqa, int lqb, int cbase, uint32_t *bh = nullptr,
+                                                uint32_t *be = nullptr, bool bnd_in = false,
+                                                bool bnd_out = false) {
+    constexpr bool PF = PF_ || STRIPE;   // stripes: no code-load wait right after the boundary store
     using O = VOps<V>;
     using L = ProfLayout<G, R>;
     constexpr int KC = L::KC;
@@ -187,9 +190,8 @@ __device__ __forceinline__ SweepOut sweep_group(const uint8_t *__restrict__ rseq
     V ein = LOCAL ? ZERO : O::splat(O::NEG);
     V eout = ein;
     if (STRIPE && bnd_in && t == 0 && lr > 0) {
-        const uint2 b0 = bnd[0];
-        hin = O::from_bits((int)b0.x);
-        ein = O::from_bits((int)b0.y);
+        hin = O::from_bits((int)bh[0]);
+        ein = O::from_bits((int)be[0]);
     }
     V best = ZERO;
     V endcol = O::splat(O::NEG);
@@ -217,7 +219,7 @@ __device__ __forceinline__ SweepOut sweep_group(const uint8_t *__restrict__ rseq
     for (int s = 0; s < nsteps; ++s) {
         const int row = s - t;
         uint2 nb{0u, 0u};
-        if (STRIPE && bnd_in && t == 0 && row + 1 < lr) nb = bnd[row + 1];   // lane 0's next row
+        if (STRIPE && bnd_in && t == 0 && row + 1 < lr) nb = uint2{bh[row + 1], be[row + 1]};   // lane 0's next row
         int c_nxt = 0;
         if constexpr (PF) c_nxt = (int)rseq[min(max(row + 1, 0), rlast)];
         if ((unsigned)row < (unsigned)lr) {
@@ -252,9 +254,23 @@ __device__ __forceinline__ SweepOut sweep_group(const uint8_t *__restrict__ rseq
             hout = h;
             eout = e;
             if (!LOCAL && track_endcol) endcol = O::mx(endcol, h);   // lane's last column
-            if (STRIPE && bnd_out && t == G - 1) bnd[row] = uint2{(unsigned)O::bits(h), (unsigned)O::bits(e)};
         }
         if constexpr (PF) c_raw = c_nxt;
+        if constexpr (STRIPE) {
+            // Loads and stores share one wait counter, and the compiler waits for all of
+            // them once both kinds are pending: take this step's loads (empty asm reading
+            // their registers; memory clobber keeps the store below it), then store lane
+            // G-1's boundary row, which drains during the next step's columns.
+            asm volatile("" ::"v"(nb.x), "v"(nb.y), "v"(c_nxt) : "memory");
+            if (bnd_out && t == G - 1 && (unsigned)row < (unsigned)lr) {
+                // two planes: the stores read H and E where they live (loop-carried, not
+                // rewritten until late in the next step); one 64-bit store would need an
+                // adjacent copy that the next step's first profile load overwrites,
+                // forcing an early wait
+                bh[row] = (uint32_t)O::bits(hout);
+                be[row] = (uint32_t)O::bits(eout);
+            }
+        }
         din = hin;
         hin = O::from_bits(wave_shr1(O::bits(hout)));
         ein = O::from_bits(wave_shr1(O::bits(eout)));

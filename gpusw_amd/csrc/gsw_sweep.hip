@@ -105,10 +105,11 @@ __attribute__((amdgpu_waves_per_eu(WavesPerEU<LOCAL, G, R>::value))) cross_kerne
 // Queries wider than a tile: the query is cut into stripes of W = G*R columns, and a
 // workgroup takes (query pair, reference range) items as the cross kernel does but runs
 // the item stripe by stripe: build the stripe's profile, let the waves sweep all the
-// item's references through it, then move right.  Each reference row's (H, E) at the
-// stripe edge is kept in the workgroup's boundary region of global memory (8 bytes per
-// reference row of the item, in sorted order); results of a reference are combined over
-// the stripes with max in the output, and the saturation check sees the final value.
+// item's references through it, then move right.  Each reference row's H and E at the
+// stripe edge are kept in the workgroup's boundary region of global memory (an H plane
+// and an E plane of bnd_stride words, one word per reference row of the item in sorted
+// order); results of a reference are combined over the stripes with max in the output,
+// and the saturation check sees the final value.
 template <typename V, int G, int R, bool LOCAL, int WAVES>
 __global__ void __launch_bounds__(WAVES * 64)
 __attribute__((amdgpu_waves_per_eu(WavesPerEU<LOCAL, G, R>::value))) stripe_kernel(KArgs a) {
@@ -122,7 +123,7 @@ __attribute__((amdgpu_waves_per_eu(WavesPerEU<LOCAL, G, R>::value))) stripe_kern
     const int lane = threadIdx.x & 63;
     const int t = lane % G, g = lane / G;
     const CrossItem *items = static_cast<const CrossItem *>(a.items);
-    uint2 *bnd_wg = a.bnd + (size_t)blockIdx.x * (size_t)a.bnd_stride;
+    uint32_t *bnd_wg = a.bnd + (size_t)blockIdx.x * 2 * (size_t)a.bnd_stride;
     for (;;) {
         if (threadIdx.x == 0) s_item = atomicAdd(a.counter, 1);
         __syncthreads();
@@ -164,10 +165,11 @@ __attribute__((amdgpu_waves_per_eu(WavesPerEU<LOCAL, G, R>::value))) stripe_kern
                 const int ref = rpos < item.rend ? a.rorder[rpos] : -1;
                 const int rs = ref >= 0 ? a.roff[ref] : 0;
                 const int lr = ref >= 0 ? a.roff[ref + 1] - rs : 0;
-                uint2 *bnd = bnd_wg + (ref >= 0 ? a.bpos[rpos] - row_base : 0);
+                uint32_t *bh = bnd_wg + (ref >= 0 ? a.bpos[rpos] - row_base : 0);
                 const int nsteps = __builtin_amdgcn_readfirstlane(wave_max_i32(lr)) + G - 1;
-                const SweepOut res = sweep_group<V, G, R, LOCAL, true>(a.rcodes + rs, lr, nsteps, prof_lane, t,
-                                                                       a.k, lqa, lqb, cbase, bnd, !first, !last);
+                const SweepOut res = sweep_group<V, G, R, LOCAL, true>(a.rcodes + rs, lr, nsteps, prof_lane, t, a.k,
+                                                                       lqa, lqb, cbase, bh, bh + a.bnd_stride,
+                                                                       !first, !last);
                 if (t == 0 && ref >= 0) {
                     int va = res.a, vb = res.b;
                     if (!first) {   // written by this workgroup in an earlier stripe
@@ -638,8 +640,11 @@ void expand_cross_items(const CrossUnit *u0, const CrossUnit *u1, const SeqSet &
 }
 
 int64_t stripe_item_rows(double mean_lr, int max_lr) {
-    // >= 4 blocks of 64/G references per wave and stripe, for balance at the stripe barrier
-    return std::max<int64_t>({(int64_t)1 << 17, (int64_t)(256.0 * mean_lr), (int64_t)max_lr});
+    // about 16 blocks of 64/G references per wave and stripe, so the waves reach a stripe's
+    // barrier within about one block of each other; at most 2^19 rows (4 MiB of boundaries
+    // per workgroup) unless a single reference is longer
+    const int64_t want = std::min<int64_t>((int64_t)(1024.0 * mean_lr), (int64_t)1 << 19);
+    return std::max<int64_t>({want, (int64_t)1 << 17, (int64_t)max_lr});
 }
 
 int64_t expand_stripe_items(const CrossUnit *u0, const CrossUnit *u1, const SeqSet &rs, int64_t max_rows,
@@ -750,7 +755,7 @@ int batch_cross(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
         groups.push_back(g);
         u0 = u1;
     }
-    if (bnd_entries > 0) GSW_TRY(ensure_buffer(&ctx->bnd, &ctx->bnd_bytes, sizeof(uint2) * (size_t)bnd_entries));
+    if (bnd_entries > 0) GSW_TRY(ensure_buffer(&ctx->bnd, &ctx->bnd_bytes, 2 * sizeof(uint32_t) * (size_t)bnd_entries));
     if (groups.size() > 60) return fail(GSW_ERR_LIMIT, "too many tile groups (%zu)", groups.size());
     // GSW_TRACE_LAUNCHES=1: time every launch and print tile, cells and GCUPS to stderr
     static const bool trace = [] { const char *e = std::getenv("GSW_TRACE_LAUNCHES"); return e && *e == '1'; }();
@@ -776,7 +781,7 @@ int batch_cross(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
         a.counter = ctx->counters + 2 + gi;
         const int grid = std::min<int>(g.grid, a.nitems);
         if (g.striped) {
-            a.bnd = static_cast<uint2 *>(ctx->bnd);
+            a.bnd = static_cast<uint32_t *>(ctx->bnd);
             a.bnd_stride = bnd_entries / grid;
             a.bpos = rs.d_bpos;
             v.stripe(a, grid, g.lds, ctx->stream);
