@@ -37,7 +37,8 @@ class GswScheme(ctypes.Structure):
 class GswStats(ctypes.Structure):
     _fields_ = [("kernel_ms", ctypes.c_double), ("wall_ms", ctypes.c_double), ("cells", _i64),
                 ("pairs", _i64), ("rescored", _i64), ("plain_pairs", _i64), ("launches", _i32),
-                ("variant", _i32)]
+                ("variant", _i32), ("top_ms", ctypes.c_double), ("top_cells", _i64),
+                ("top_variant", _i32), ("top_striped", _i32)]
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}

@@ -51,9 +51,15 @@ class HipBackend(Backend):
     """Batched wavefront kernels: packed-int16 lanes, int32 fallback, device top-k."""
 
     name = "hip"
+    history = None   # set to a list to collect every call's GswStats (benchmarks)
 
     def available(self) -> bool:
         return _hip.gpu_available()
+
+    def _note(self, stats) -> None:
+        self.last_stats = stats
+        if self.history is not None:
+            self.history.append(stats)
 
     @contextmanager
     def device_scope(self, device: int):
@@ -79,7 +85,7 @@ class HipBackend(Backend):
         stats = _hip.GswStats()
         _hip.call("gsw_score_cross", ctypes.byref(s), _policy(scheme), qset.handle, _hip.ptr(qlist),
                   int(nq), rset.handle, out.ptr, ctypes.byref(stats))
-        self.last_stats = stats
+        self._note(stats)
         return DeviceScores(out, (nq, nr)), stats.kernel_ms / 1e3, _describe("cross", scheme, stats, dtype)
 
     def run_pairs(self, scheme, maxq, dtype, threads, qbuf, qoff, rbuf, roff, qi_host, rj_host, n):
@@ -93,7 +99,7 @@ class HipBackend(Backend):
         stats = _hip.GswStats()
         _hip.call("gsw_score_pairs", ctypes.byref(s), _policy(scheme), qset.handle, rset.handle,
                   _hip.ptr(qi), _hip.ptr(rj), int(n), out.ptr, ctypes.byref(stats))
-        self.last_stats = stats
+        self._note(stats)
         self.last_cells = int(stats.cells)   # counted by the library while planning
         return out.to_host(), stats.kernel_ms / 1e3, _describe("pairs", scheme, stats, dtype)
 

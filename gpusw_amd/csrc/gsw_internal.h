@@ -45,6 +45,7 @@ struct DeviceCtx {
     void *meta2 = nullptr;     size_t meta2_bytes = 0;
     void *bnd = nullptr;       size_t bnd_bytes = 0;       // stripe boundaries
     int32_t *counters = nullptr;                            // 64 int32 counters
+    std::vector<hipEvent_t> marks;                          // per-launch timing events
     std::mutex mu;                                          // serialises calls on this device
 };
 
@@ -115,7 +116,8 @@ int plain_run(DeviceCtx *ctx, const SchemeDev &s, const PlainJob &job, float *ms
 
 // ---------------------------------------------------------------- batched kernels
 struct BatchResult { double kernel_ms = 0; int64_t rescored = 0; int64_t plain_pairs = 0;
-                     int launches = 0; int variant = -1; int64_t cells = -1; /* -1: not counted */ };
+                     int launches = 0; int variant = -1; int64_t cells = -1; /* -1: not counted */
+                     double top_ms = 0; int64_t top_cells = 0; int top_variant = -1, top_striped = 0; };
 
 int batch_cross(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqSet &qs,
                 const int32_t *h_qlist, int32_t nq, const SeqSet &rs, int32_t *d_out,

@@ -757,9 +757,14 @@ int batch_cross(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
     }
     if (bnd_entries > 0) GSW_TRY(ensure_buffer(&ctx->bnd, &ctx->bnd_bytes, 2 * sizeof(uint32_t) * (size_t)bnd_entries));
     if (groups.size() > 60) return fail(GSW_ERR_LIMIT, "too many tile groups (%zu)", groups.size());
-    // GSW_TRACE_LAUNCHES=1: time every launch and print tile, cells and GCUPS to stderr
+    // every launch is timed with an event after it (the longest one is reported);
+    // GSW_TRACE_LAUNCHES=1 also prints tile, cells and GCUPS of each launch to stderr
     static const bool trace = [] { const char *e = std::getenv("GSW_TRACE_LAUNCHES"); return e && *e == '1'; }();
-    std::vector<hipEvent_t> marks;
+    while (ctx->marks.size() < groups.size()) {
+        hipEvent_t e;
+        GSW_HIP_TRY(hipEventCreate(&e));
+        ctx->marks.push_back(e);
+    }
     GSW_HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
     if (!groups.empty()) {
         GSW_TRY(ensure_buffer(&ctx->meta2, &ctx->meta2_bytes, sizeof(CrossItem) * items.size()));
@@ -790,34 +795,36 @@ int batch_cross(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
         }
         GSW_HIP_TRY(hipGetLastError());
         res->launches++;
-        if (trace) {
-            hipEvent_t e;
-            GSW_HIP_TRY(hipEventCreate(&e));
-            GSW_HIP_TRY(hipEventRecord(e, ctx->stream));
-            marks.push_back(e);
-        }
+        GSW_HIP_TRY(hipEventRecord(ctx->marks[gi], ctx->stream));
     }
     GSW_HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
     GSW_HIP_TRY(hipEventSynchronize(ctx->ev1));
     float ms = 0.f;
     GSW_HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-    for (size_t gi = 0; gi < marks.size(); ++gi) {
+    for (size_t gi = 0; gi < groups.size(); ++gi) {
         const Group &g = groups[gi];
         const Variant &v = *units[g.u0].v;
         float gms = 0.f;
-        (void)hipEventElapsedTime(&gms, gi ? marks[gi - 1] : ctx->ev0, marks[gi]);
-        double cells = 0, width = 0;
+        GSW_HIP_TRY(hipEventElapsedTime(&gms, gi ? ctx->marks[gi - 1] : ctx->ev0, ctx->marks[gi]));
+        int64_t qcols = 0;
+        double width = 0;
         for (size_t u = g.u0; u < g.u1; ++u) {
-            const double ql = (double)units[u].a.len + (units[u].has_b ? units[u].b.len : 0);
-            cells += ql * (double)rs.total;
+            qcols += (int64_t)units[u].a.len + (units[u].has_b ? units[u].b.len : 0);
             width += (double)v.G * v.R * units[u].nstr * (v.packed ? 2 : 1) * (double)rs.total;
         }
-        std::fprintf(stderr, "[gsw] launch %zu: G=%d R=%d %s %s%s units=%zu items=%zu cells=%.3e ms=%.2f "
-                     "GCUPS=%.0f padded_GCUPS=%.0f\n", gi, v.G, v.R, v.packed ? "int16x2" : "int32",
-                     v.local ? "local" : "nonlocal", g.striped ? " striped" : "", g.u1 - g.u0, g.i1 - g.i0, cells,
-                     gms, cells / (gms * 1e6), width / (gms * 1e6));
+        const int64_t cells = qcols * rs.total;
+        if (gms > res->top_ms) {
+            res->top_ms = gms;
+            res->top_cells = cells;
+            res->top_variant = v.id;
+            res->top_striped = g.striped ? 1 : 0;
+        }
+        if (trace)
+            std::fprintf(stderr, "[gsw] launch %zu: G=%d R=%d %s %s%s units=%zu items=%zu cells=%.3e ms=%.2f "
+                         "GCUPS=%.0f padded_GCUPS=%.0f\n", gi, v.G, v.R, v.packed ? "int16x2" : "int32",
+                         v.local ? "local" : "nonlocal", g.striped ? " striped" : "", g.u1 - g.u0, g.i1 - g.i0,
+                         (double)cells, gms, (double)cells / (gms * 1e6), width / (gms * 1e6));
     }
-    for (hipEvent_t e : marks) (void)hipEventDestroy(e);
     res->kernel_ms += ms;
     res->variant = dominant;
 
@@ -1216,6 +1223,14 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
         for (auto &x : workers) x.join();
         workers.clear();
     };
+    while (ctx->marks.size() < nb) {   // one timing event after each batch
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) {
+            stop();
+            return fail(GSW_ERR_HIP, "event create failed");
+        }
+        ctx->marks.push_back(e);
+    }
     int rc = GSW_OK;
     if (hipEventRecord(ctx->ev0, ctx->stream) != hipSuccess) rc = fail(GSW_ERR_HIP, "event record failed");
     for (size_t b = 0; b < nb && rc == GSW_OK; ++b) {
@@ -1234,6 +1249,7 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
                                                          hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipGraphLaunch(g.exec, ctx->stream);
         if (e == hipSuccess) e = hipEventRecord(g.done, ctx->stream);
+        if (e == hipSuccess) e = hipEventRecord(ctx->marks[b], ctx->stream);
         if (e != hipSuccess) {
             rc = fail(GSW_ERR_HIP, "pairs batch launch: %s", hipGetErrorString(e));
             break;
@@ -1256,6 +1272,29 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
     GSW_HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     res->kernel_ms += ms;
     res->variant = dominant;
+    // the dominant tile's batches: mean time between consecutive batch events (the graph's
+    // task copy and kernel) and mean cells per batch
+    {
+        double sum_ms = 0, sum_cells = 0;
+        int cnt = 0;
+        for (size_t b = 0; b < nb; ++b) {
+            if (batches[b].v->id != dominant) continue;
+            float bms = 0.f;
+            GSW_HIP_TRY(hipEventElapsedTime(&bms, b ? ctx->marks[b - 1] : ctx->ev0, ctx->marks[b]));
+            double c = 0;
+            for (int64_t i = batches[b].b0; i < batches[b].b1; ++i)
+                c += (double)qs.len[h_qi[members[i]]] * rs.len[h_rj[members[i]]];
+            sum_ms += bms;
+            sum_cells += c;
+            ++cnt;
+        }
+        if (cnt) {
+            res->top_ms = sum_ms / cnt;
+            res->top_cells = (int64_t)(sum_cells / cnt);
+            res->top_variant = dominant;
+            res->top_striped = 2;   // pairs kernel
+        }
+    }
 
     if (!p_qi.empty()) {
         int maxq = 1;
@@ -1308,6 +1347,10 @@ static void fill_stats(gsw_stats *st, const BatchResult &r, int64_t cells, int64
     st->plain_pairs = r.plain_pairs;
     st->launches = r.launches;
     st->variant = r.variant;
+    st->top_ms = r.top_ms;
+    st->top_cells = r.top_cells;
+    st->top_variant = r.top_variant;
+    st->top_striped = r.top_striped;
     st->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
