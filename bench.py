@@ -14,7 +14,8 @@ before the timed region.  Also reported:
 * ``cpu_baseline``: the oracle's C restatement on the host cores over a bounded sample
   of the same pairs (those pairs are also checked bit-exact against the GPU scores).
 
-``--workload`` selects another configuration for extra lines: ``protein_topk`` (BLOSUM62
+``--workload`` selects another configuration for extra lines: ``plain_dna`` (the 64 x 256
+int32 case, oracle-checked in full), ``protein_topk`` (BLOSUM62
 top-5 of 1,000 queries against 100,000 references), ``read_pairs`` (1,000,000 explicit
 read/window pairs, semiglobal then global, graph-replayed batches), ``serving`` (Poisson
 request mix through the continuous-batching server, p50/p99 latency).
@@ -232,6 +233,51 @@ def run_dna_cross(args):
     return line
 
 
+def run_plain_dna(args):
+    """64 x 256 DNA local int32 (lengths 100-300, seed 0): the reference's CPU-runnable case.
+    Batched and plain kernels are both timed; the oracle scores all 16,384 pairs (the CPU
+    baseline here is the whole workload) and both GPU matrices must equal it."""
+    import gpusw_amd
+    from gpusw_amd import Aligner, Scheme, synth
+    from oracle import native
+
+    (qc, qo), (rc, ro) = synth.plain_dna(seed=0)
+    scheme = Scheme.preset("dna").replace(dtype="int32")
+    res = {}
+    for backend in ("hip", "hip-plain"):
+        al = Aligner(scheme, backend=backend).index((rc, ro)).set_queries((qc, qo))
+        for _ in range(args.warmup):
+            al.score_cross()
+        gpusw_amd._hip.call("gsw_synchronize")
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            got = al.score_cross()
+        wall = (time.perf_counter() - t0) / args.steps
+        res[backend] = (got, wall, al)
+    al = res["hip"][2]
+    cells = int(al._queries.lengths.astype(np.int64).sum()) * int(al._refs.lengths.astype(np.int64).sum())
+    threads = len(os.sched_getaffinity(0))
+    t0 = time.perf_counter()
+    want = native.score_cross(qc, qo, rc, ro, np.arange(64, dtype=np.int32), scheme.substitution_table(),
+                              "local", scheme.gap_open, scheme.gap_extend, True, True, nthreads=threads)
+    cpu_s = time.perf_counter() - t0
+    hip_got, hip_wall, _ = res["hip"]
+    plain_got, plain_wall, _ = res["hip-plain"]
+    value = cells / hip_wall / 1e9
+    return {
+        "metric": METRIC, "value": round(value, 2), "unit": "GCUPS", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(hip_wall * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "int32", "data": "synthetic",
+        "config": {"workload": "plain DNA: 64 x 256 (lengths 100-300, seed 0), dna local, int32; host-side "
+                               "call including the 64 KB result copy", "cells": cells},
+        "plain": {"gcups": round(cells / plain_wall / 1e9, 2), "ms_per_step": round(plain_wall * 1e3, 3)},
+        "checks": {"batched_equals_oracle": bool(np.array_equal(hip_got.astype(np.int64), want)),
+                   "plain_equals_oracle": bool(np.array_equal(plain_got.astype(np.int64), want))},
+        "cpu_baseline": {"value": round(cells / cpu_s / 1e9, 4), "unit": "GCUPS", "cores": threads, "kind": "port",
+                         "sample": f"all 16,384 pairs ({cells:.3e} cells), {cpu_s:.2f} s"},
+    }
+
+
 def _plain_subsample_cross(al, scheme, qsel):
     """GCUPS of the one-thread-per-pair kernel on a query subsample (same GPU)."""
     import gpusw_amd
@@ -420,7 +466,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="dna_cross",
-                    choices=["dna_cross", "protein_topk", "read_pairs", "serving"])
+                    choices=["dna_cross", "plain_dna", "protein_topk", "read_pairs", "serving"])
     ap.add_argument("--rates", default="2,5,50", help="serving: offered request rates (req/s)")
     ap.add_argument("--requests", type=int, default=40, help="serving: requests per rate")
     ap.add_argument("--budget", type=int, default=64, help="serving: device memory budget (GiB)")
@@ -439,6 +485,8 @@ def main() -> int:
         return 2
     if args.workload == "dna_cross":
         lines = [run_dna_cross(args)]
+    elif args.workload == "plain_dna":
+        lines = [run_plain_dna(args)]
     elif args.workload == "protein_topk":
         lines = [run_protein_topk(args)]
     elif args.workload == "read_pairs":

@@ -97,6 +97,8 @@ _SIGNATURES = {
     "gsw_set_tile_table": [_vp, _i32],
     "gsw_force_variant": [_i32, _i32],
     "gsw_list_variants": [_vp, _i32, ctypes.POINTER(_i32)],
+    "gsw_fasta_scan": [_vp, _i64, ctypes.POINTER(_i64), ctypes.POINTER(_i64)],
+    "gsw_fasta_encode": [_vp, _i64, _vp, _vp, _vp, _vp],
 }
 _RESTYPE = {"gsw_last_error": ctypes.c_char_p, "gsw_version": ctypes.c_char_p}
 

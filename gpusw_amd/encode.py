@@ -16,7 +16,7 @@ import numpy as np
 
 from .errors import EncodeError
 
-__all__ = ["Encoded", "funnel", "build_lut", "read_fasta"]
+__all__ = ["Encoded", "funnel", "build_lut", "read_fasta", "encode_fasta", "save_db", "load_db"]
 
 _PATH_SUFFIXES = (".fasta", ".fa", ".fna", ".faa", ".ffn", ".fastq", ".fq", ".txt", ".gz")
 
@@ -89,6 +89,99 @@ def read_fasta(path_or_text) -> list[tuple[str, str]]:
     if name is not None:
         records.append((name, "".join(chunks)))
     return records
+
+
+def _read_bytes(path: str) -> bytes:
+    opener = gzip.open if str(path).endswith(".gz") else open
+    with opener(path, "rb") as fh:
+        return fh.read()
+
+
+def encode_fasta(path, scheme) -> "Encoded | None":
+    """FASTA file -> :class:`Encoded` in two native passes (``gsw_fasta_scan`` and
+    ``gsw_fasta_encode``), with the records :func:`read_fasta` gives.  Returns None when
+    the file is not ASCII or the native library is not built; the caller then takes the
+    record-by-record path."""
+    import ctypes
+
+    from . import _hip
+    from .errors import NoGpuError
+
+    try:
+        lib = _hip.load()
+    except NoGpuError:
+        return None
+    raw = _read_bytes(str(path))
+    buf = np.frombuffer(raw, dtype=np.uint8)
+    nrec, nres = ctypes.c_int64(0), ctypes.c_int64(0)
+    if lib.gsw_fasta_scan(_hip.ptr(buf), buf.size, ctypes.byref(nrec), ctypes.byref(nres)) != 0:
+        return None
+    lut, sentinel = build_lut(scheme)
+    codes = np.empty(max(nres.value, 1), dtype=np.uint8)
+    offsets = np.empty(nrec.value + 1, dtype=np.int64)
+    spans = np.empty(2 * max(nrec.value, 1), dtype=np.int64)
+    _hip.check(lib.gsw_fasta_encode(_hip.ptr(buf), buf.size, _hip.ptr(lut), _hip.ptr(codes),
+                                    _hip.ptr(offsets), _hip.ptr(spans)))
+    codes = codes[:nres.value]
+    ids = [raw[a:a + m].decode("ascii") if m else f"seq{k}"
+           for k, (a, m) in enumerate(spans[:2 * nrec.value].reshape(-1, 2).tolist())]
+    if offsets[-1] > np.iinfo(np.int32).max:
+        raise EncodeError("sequence set exceeds 2^31 residues")
+    offsets = offsets.astype(np.int32)
+    if scheme.unknown == "error" and codes.size:
+        bad = np.flatnonzero(codes == sentinel)
+        if bad.size:
+            which = int(np.searchsorted(offsets, int(bad[0]), side="right")) - 1
+            raise EncodeError(
+                f"sequence {ids[which]!r} contains an out-of-alphabet symbol "
+                f"(alphabet={scheme.eff_alphabet!r}); set unknown='mismatch' or 'zero'"
+            )
+    return Encoded(codes, offsets, ids, np.diff(offsets).astype(np.int32), scheme.eff_alphabet)
+
+
+def save_db(path, enc: "Encoded") -> None:
+    """Write an encoded sequence set as a directory: ``codes.npy`` (uint8),
+    ``offsets.npy`` (int32, n+1), ``ids.txt`` (one id per line) and ``meta.json``."""
+    import json
+
+    d = os.fspath(path)
+    os.makedirs(d, exist_ok=True)
+    if any(("\n" in i or "\r" in i) for i in map(str, enc.ids)):
+        raise EncodeError("sequence ids must not contain line breaks")
+    np.save(os.path.join(d, "codes.npy"), np.ascontiguousarray(enc.codes, dtype=np.uint8))
+    np.save(os.path.join(d, "offsets.npy"), np.ascontiguousarray(enc.offsets, dtype=np.int32))
+    with open(os.path.join(d, "ids.txt"), "w", encoding="utf-8") as fh:
+        fh.write("".join(f"{i}\n" for i in enc.ids))
+    with open(os.path.join(d, "meta.json"), "w") as fh:
+        json.dump({"format": "gsw-db-1", "alphabet": enc.alphabet, "n": enc.n,
+                   "residues": int(enc.offsets[-1]) if enc.offsets.size else 0}, fh)
+
+
+def load_db(path, scheme=None, mmap: bool = True) -> "Encoded":
+    """Read a directory written by :func:`save_db` (arrays memory-mapped, nothing
+    unpickled).  With ``scheme``, its alphabet must be the one the set was encoded for."""
+    import json
+
+    d = os.fspath(path)
+    with open(os.path.join(d, "meta.json")) as fh:
+        meta = json.load(fh)
+    if meta.get("format") != "gsw-db-1":
+        raise EncodeError(f"{d}: not a sequence database (format {meta.get('format')!r})")
+    if scheme is not None and meta["alphabet"] != scheme.eff_alphabet:
+        raise EncodeError(f"{d}: encoded for alphabet {meta['alphabet']!r}, scheme uses "
+                          f"{scheme.eff_alphabet!r}")
+    mode = "r" if mmap else None
+    codes = np.load(os.path.join(d, "codes.npy"), mmap_mode=mode, allow_pickle=False)
+    offsets = np.load(os.path.join(d, "offsets.npy"), mmap_mode=mode, allow_pickle=False)
+    with open(os.path.join(d, "ids.txt"), encoding="utf-8") as fh:
+        ids = fh.read().split("\n")[:-1]
+    if len(ids) != offsets.size - 1 or meta["n"] != len(ids):
+        raise EncodeError(f"{d}: ids, offsets and meta disagree on the sequence count")
+    return Encoded(codes, offsets, ids, np.diff(offsets).astype(np.int32), meta["alphabet"])
+
+
+def _is_db(obj) -> bool:
+    return isinstance(obj, (str, os.PathLike)) and os.path.isfile(os.path.join(os.fspath(obj), "meta.json"))
 
 
 def _records(obj, id_prefix: str) -> list[tuple[str, str]]:
@@ -180,6 +273,12 @@ def funnel(obj, scheme, *, id_prefix: str = "q") -> Encoded:
     ):
         return _from_preencoded(obj, scheme, id_prefix)
 
+    if _is_db(obj):
+        return load_db(obj, scheme)
+    if isinstance(obj, (str, os.PathLike)) and not str(obj).lstrip().startswith(">") and os.path.isfile(obj):
+        enc = encode_fasta(obj, scheme)   # None: not plain ASCII, take the record path
+        if enc is not None:
+            return enc
     records = _records(obj, id_prefix)
     ids = [r[0] for r in records]
     raw, lengths = _encode_texts([r[1] for r in records], scheme)

@@ -151,6 +151,18 @@ int gsw_set_tile_table(const int32_t *rules, int32_t n);
 int gsw_force_variant(int32_t G, int32_t R);
 int gsw_list_variants(int32_t *gr, int32_t cap, int32_t *n);   /* (G, R) pairs */
 
+/* ---- host FASTA funnel ------------------------------------------------------
+ * Replaces the funnel's FASTA reader (src/gpusw/encode.py:91-121, 164-220) for ASCII
+ * files: same records (lines split at the ASCII breaks of str.splitlines, stripped of
+ * str.strip whitespace, blank lines skipped, text before the first '>' dropped, id = the
+ * first word of the header).  gsw_fasta_scan sizes the output (GSW_ERR_INVALID on a byte
+ * >= 0x80); gsw_fasta_encode maps residues through lut[256] into codes[nres], writes
+ * offsets[nrec+1] and id_span[2*nrec] = (start, length) of each id in buf (length 0: the
+ * header has no word). */
+int gsw_fasta_scan(const uint8_t *buf, int64_t n, int64_t *nrec, int64_t *nres);
+int gsw_fasta_encode(const uint8_t *buf, int64_t n, const uint8_t *lut, uint8_t *codes,
+                     int64_t *offsets, int64_t *id_span);
+
 /* ---- serving -------------------------------------------------------------
  * No counterpart in the reference (its Aligner.top_k, src/gpusw/aligner.py:169-195,
  * is a synchronous batch loop).  A server keeps reference databases resident, admits
