@@ -208,9 +208,12 @@ int gsw_get_device(int32_t *device) {
 int gsw_device_name(int32_t device, char *buf, int32_t buflen) {
     int count = 0;
     GSW_TRY(init_runtime(&count));
+    if (device < 0 || device >= count) return fail(GSW_ERR_INVALID, "device %d out of range", device);
+    if (!buf || buflen <= 0) return fail(GSW_ERR_INVALID, "null name buffer");
     hipDeviceProp_t prop;
     GSW_HIP_TRY(hipGetDeviceProperties(&prop, device));
-    snprintf(buf, buflen, "%s (%s, %d CUs, %d MHz, LDS/block %zu KiB, %zu GiB)", prop.name, prop.gcnArchName,
+    snprintf(buf, buflen, "%s (%s, %d CUs, %d MHz, LDS/block %zu KiB, %zu GiB)",
+             prop.name[0] ? prop.name : "AMD GPU", prop.gcnArchName,
              prop.multiProcessorCount, prop.clockRate / 1000, prop.sharedMemPerBlock / 1024,
              prop.totalGlobalMem >> 30);
     return GSW_OK;
