@@ -18,7 +18,7 @@ from .backends import available_backends
 from .errors import EncodeError, GpuSWError, NoGpuError, OverflowRiskError, SchemeError
 from .matrix import Matrix
 from .matrix import available as available_matrices
-from .result import AlignResult
+from .result import AlignResult, Alignment
 from .scheme import Scheme
 
 __all__ = [
@@ -28,6 +28,7 @@ __all__ = [
     "Scheme",
     "Matrix",
     "AlignResult",
+    "Alignment",
     "schemes",
     "available_matrices",
     "gpu_available",

@@ -62,6 +62,12 @@ class GswServerStatus(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class GswAln(ctypes.Structure):
+    _fields_ = [("score", _i32), ("q_begin", _i32), ("q_end", _i32), ("r_begin", _i32), ("r_end", _i32),
+                ("matches", _i32), ("mismatches", _i32), ("ins", _i32), ("del_", _i32), ("gap_opens", _i32),
+                ("n_ops", _i32), ("ops_offset", _i64)]
+
+
 #: symbol -> argtypes (every function returns int status unless listed in _RESTYPE)
 _SIGNATURES = {
     "gsw_last_error": [],
@@ -98,6 +104,8 @@ _SIGNATURES = {
     "gsw_force_variant": [_i32, _i32],
     "gsw_list_variants": [_vp, _i32, ctypes.POINTER(_i32)],
     "gsw_fasta_scan": [_vp, _i64, ctypes.POINTER(_i64), ctypes.POINTER(_i64)],
+    "gsw_traceback_pairs": [ctypes.POINTER(GswScheme), _vp, _vp, _vp, _vp, _i64, _vp, _vp, _i64,
+                            ctypes.POINTER(GswStats)],
     "gsw_fasta_encode": [_vp, _i64, _vp, _vp, _vp, _vp],
 }
 _RESTYPE = {"gsw_last_error": ctypes.c_char_p, "gsw_version": ctypes.c_char_p}

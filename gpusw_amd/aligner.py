@@ -183,6 +183,39 @@ class Aligner:
         return AlignResult(None, self._queries.ids, self._refs.ids, self.scheme,
                            topk_idx=idx_out, topk_scores=sc_out)
 
+    # ------------------------------------------------------------- traceback
+    def align(self, qi, rj) -> list:
+        """The alignments behind the scores of pairs ``(qi[k], rj[k])`` (GPU traceback,
+        int32): a list of :class:`~gpusw_amd.result.Alignment`.  Each score equals the
+        pair's ``score_pairs`` score.  No counterpart in the reference, whose results are
+        score-only (its examples re-align top hits on the CPU)."""
+        from .backends.hip import run_traceback
+        from .result import Alignment, _cigar
+
+        if self._refs is None or self._queries is None:
+            raise ValueError("call index(refs) and set_queries(queries) first")
+        qi = np.ascontiguousarray(qi, dtype=np.int32).reshape(-1)
+        rj = np.ascontiguousarray(rj, dtype=np.int32).reshape(-1)
+        if qi.shape != rj.shape:
+            raise ValueError("qi and rj must have the same length")
+        with self._b.device_scope(self.device):
+            recs, ops, stats = run_traceback(self.scheme, self._d_qbuf, self._d_qoff, self._d_rbuf,
+                                             self._d_roff, qi, rj)
+        self._record(int(stats.cells), stats.kernel_ms / 1e3, "traceback (one thread per pair, int32)")
+        return [Alignment(self._queries.ids[int(q)], self._refs.ids[int(r)], a.score, a.q_begin, a.q_end,
+                          a.r_begin, a.r_end, _cigar(ops, a.ops_offset, a.n_ops), a.matches, a.mismatches,
+                          a.ins, a.del_, a.gap_opens)
+                for q, r, a in zip(qi, rj, recs)]
+
+    def top_k_alignments(self, queries=None, k: int = 5, *, query_batch: int = 512) -> list:
+        """``top_k`` followed by the traceback of every hit: ``[(query_id, [Alignment,
+        ...]), ...]`` best first."""
+        res = self.top_k(queries, k, query_batch=query_batch)
+        nq, kk = res.topk_idx.shape
+        qi = np.repeat(np.arange(nq, dtype=np.int32), kk)
+        alns = self.align(qi, res.topk_idx.reshape(-1))
+        return [(self._queries.ids[q], alns[q * kk:(q + 1) * kk]) for q in range(nq)]
+
     # ----------------------------------------------------------- introspection
     @property
     def backend(self) -> str:

@@ -156,6 +156,27 @@ class HipPlainBackend(HipBackend):
         return out.to_host(), stats.kernel_ms / 1e3, _describe("pairs-plain", scheme, stats, dtype)
 
 
+def run_traceback(scheme, qbuf, qoff, rbuf, roff, qi_host, rj_host):
+    """GPU traceback of pairs (``gsw_traceback_pairs``): ``(records, ops, stats)`` with
+    records a ``GswAln`` array and ops the concatenated CIGAR runs."""
+    qset = _hip.seqset_for(qbuf, qoff)
+    rset = _hip.seqset_for(rbuf, roff)
+    qi = np.ascontiguousarray(qi_host, dtype=np.int32)
+    rj = np.ascontiguousarray(rj_host, dtype=np.int32)
+    _check_pair_indices(qi, rj, qset.n, rset.n)
+    n = int(qi.size)
+    qlen = np.diff(qoff.host.astype(np.int64))
+    rlen = np.diff(roff.host.astype(np.int64))
+    cap = int((qlen[qi] + rlen[rj] + 1).sum()) if n else 0
+    recs = (_hip.GswAln * max(n, 1))()
+    ops = np.zeros(max(cap, 1), dtype=np.uint32)
+    s, _table = scheme_struct_cached(scheme)
+    stats = _hip.GswStats()
+    _hip.call("gsw_traceback_pairs", ctypes.byref(s), qset.handle, rset.handle, _hip.ptr(qi), _hip.ptr(rj),
+              n, ctypes.cast(recs, ctypes.c_void_p), _hip.ptr(ops), cap, ctypes.byref(stats))
+    return recs[:n], ops, stats
+
+
 def _check_pair_indices(qi, rj, nq, nr) -> None:
     if qi.size and (qi.min() < 0 or qi.max() >= nq):
         raise IndexError("query index out of range")

@@ -1116,9 +1116,13 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
     // (longest first), then (stable) by last-column register when it is tracked; pair
     // adjacent members of one reference; pad blocks of 64/G tasks at shift changes.
     struct Scratch { std::vector<int64_t> order, order2; std::vector<int32_t> cnt; };
-    auto plan = [&](const Batch &B, PairGraph &g, Scratch &S) -> int32_t {
+    std::vector<double> batch_cells(batches.size(), 0.0);
+    auto plan = [&](const Batch &B, PairGraph &g, Scratch &S, double *cells) -> int32_t {
         const Variant &v = *B.v;
         const int gpw = 64 / v.G;
+        double c = 0;
+        for (int64_t i = B.b0; i < B.b1; ++i) c += (double)qs.len[h_qi[members[i]]] * rs.len[h_rj[members[i]]];
+        *cells = c;
         S.cnt.assign((size_t)rs.n_nonempty + 1, 0);
         for (int64_t i = B.b0; i < B.b1; ++i) S.cnt[rs.rank[h_rj[members[i]]] + 1]++;
         for (int32_t r = 0; r < rs.n_nonempty; ++r) S.cnt[r + 1] += S.cnt[r];
@@ -1204,7 +1208,7 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
                     return;
                 }
             }
-            const int32_t nt = plan(B, g, S);
+            const int32_t nt = plan(B, g, S, &batch_cells[b]);
             std::lock_guard<std::mutex> lk(m);
             ntasks[b] = nt;
             state[b] = 1;
@@ -1281,11 +1285,8 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
             if (batches[b].v->id != dominant) continue;
             float bms = 0.f;
             GSW_HIP_TRY(hipEventElapsedTime(&bms, b ? ctx->marks[b - 1] : ctx->ev0, ctx->marks[b]));
-            double c = 0;
-            for (int64_t i = batches[b].b0; i < batches[b].b1; ++i)
-                c += (double)qs.len[h_qi[members[i]]] * rs.len[h_rj[members[i]]];
             sum_ms += bms;
-            sum_cells += c;
+            sum_cells += batch_cells[b];
             ++cnt;
         }
         if (cnt) {

@@ -151,6 +151,29 @@ int gsw_set_tile_table(const int32_t *rules, int32_t n);
 int gsw_force_variant(int32_t G, int32_t R);
 int gsw_list_variants(int32_t *gr, int32_t cap, int32_t *n);   /* (G, R) pairs */
 
+/* ---- traceback ------------------------------------------------------------
+ * No counterpart in the reference, which scores only and re-aligns its top hits on the
+ * CPU (docs/DESIGN.md:94-96, examples/legacy_16s_gpu_align.py:280-292).  For each pair
+ * (h_qi[k], h_rj[k]) the alignment behind its score, in int32, one thread per pair:
+ * coordinates are 0-based half-open; free end gaps (semiglobal) are left out of the CIGAR
+ * and show in the coordinates; h_ops receives the CIGAR runs of pair k at
+ * h_ops[out[k].ops_offset ..+ n_ops), each (length << 2) | op with op 0 '=' (identical
+ * residues), 1 'X', 2 'I' (query residue against a gap), 3 'D' (reference residue against
+ * a gap).  ops_cap must be >= sum over pairs of (lq + lr + 1).  Ties: H prefers zero
+ * (local start), then the diagonal, then E, then F; E and F prefer opening; the end cell
+ * is the first maximum in row-major order (local) or the corner, then the last column,
+ * then the last row (semiglobal). */
+typedef struct gsw_aln {
+    int32_t score;
+    int32_t q_begin, q_end, r_begin, r_end;
+    int32_t matches, mismatches, ins, del, gap_opens;
+    int32_t n_ops;
+    int64_t ops_offset;
+} gsw_aln;
+int gsw_traceback_pairs(const gsw_scheme *s, gsw_seqset queries, gsw_seqset refs, const int32_t *h_qi,
+                        const int32_t *h_rj, int64_t n, gsw_aln *h_out, uint32_t *h_ops, int64_t ops_cap,
+                        gsw_stats *stats);
+
 /* ---- host FASTA funnel ------------------------------------------------------
  * Replaces the funnel's FASTA reader (src/gpusw/encode.py:91-121, 164-220) for ASCII
  * files: same records (lines split at the ASCII breaks of str.splitlines, stripped of
