@@ -197,6 +197,18 @@ def run_dna_cross(args):
     plain_match = bool(np.array_equal(phost, host[psel]))
     log(f"plain: {plain_gcups:.1f} GCUPS over {len(psel)} queries, equal to batched: {plain_match}")
 
+    # ---- traceback of the top-5 hits of 512 queries (outside the timed region)
+    tq = np.repeat(np.arange(512, dtype=np.int32), 5)
+    tr = np.ascontiguousarray(idx[:512].reshape(-1), dtype=np.int32)
+    t0 = time.perf_counter()
+    alns = al.align(tq, tr)
+    tb_wall = time.perf_counter() - t0
+    tb_ok = [a.score for a in alns] == np.take_along_axis(host[:512], idx[:512], 1).reshape(-1).tolist()
+    traceback = {"pairs": int(tq.size), "wall_ms": round(tb_wall * 1e3, 2),
+                 "kernel_ms": round(al._last_seconds * 1e3, 2), "scores_equal_matrix": bool(tb_ok),
+                 "mean_identity": round(float(np.mean([a.identity for a in alns])), 4)}
+    log(f"traceback: {tq.size} alignments in {tb_wall * 1e3:.1f} ms, scores equal: {tb_ok}")
+
     cpu = None
     mism = 0
     if not args.no_cpu:
@@ -228,6 +240,7 @@ def run_dna_cross(args):
                   "speedup": round(value / plain_gcups, 2), "equal_to_batched": plain_match},
         "checks": {"topk_equals_stable_sort": topk_ok, "plain_equals_batched": plain_match,
                    "oracle_mismatches": mism, "rescored": int(stats.rescored)},
+        "traceback": traceback,
         "cpu_baseline": cpu,
     }
     return line
