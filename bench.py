@@ -93,8 +93,8 @@ def _kernel_name(variant: int, striped: int) -> str | None:
     g, r = _hip.list_variants()[variant // 4]
     elem = "short __vector(2)" if variant % 4 < 2 else "int"
     local = "true" if variant % 2 == 0 else "false"
-    kind = {0: "cross", 1: "stripe", 2: "pairs"}[int(striped)]
-    return f"void gsw::{kind}_kernel<{elem}, {g}, {r}, {local}, 8>(gsw::KArgs)"
+    kind, waves = {0: ("cross", 8), 1: ("stripe", 8), 2: ("pairs", 8), 3: ("pairs_tab", 4)}[int(striped)]
+    return f"void gsw::{kind}_kernel<{elem}, {g}, {r}, {local}, {waves}>(gsw::KArgs)"
 
 
 def _measured_traffic(workload: str, kernel: str | None):

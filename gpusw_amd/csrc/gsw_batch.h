@@ -52,11 +52,13 @@ struct KArgs {
     uint32_t *bnd;               // stripe kernel: boundary regions, one per workgroup,
     int64_t bnd_stride;          //   each an H plane then an E plane of bnd_stride words (>= item rows)
     const int64_t *bpos;         //   row offsets of the sorted references (n + 1)
+    int32_t tab_copies;          // pairs table kernel: interleaved copies of the table
 };
 
 
 constexpr int CROSS_WAVES = 8;
 constexpr int PAIRS_WAVES = 8;   // most waves of a pairs workgroup (LDS decides the actual count)
+constexpr int PAIRS_TAB_WAVES = 4;   // 4-wave workgroups: three fit a CU at 3 waves per SIMD
 
 typedef void (*LaunchFn)(const KArgs &, int grid, size_t lds, hipStream_t);
 
@@ -65,7 +67,12 @@ struct Variant {
     bool packed, local;
     const void *cross_fn, *pairs_fn, *stripe_fn;   // stripe_fn null: no stripe kernel
     LaunchFn cross, pairs, stripe;
+    const void *pairs_tab_fn;                       // pairs with the substitution table in LDS
 };
+// LDS of the pairs table kernel: ns rows of (ns+2)^2 packed words (int32 tiles: ns+2),
+// times pairs_tab_copies interleaved copies (32 when that stays within 48 KiB, else 1).
+int pairs_tab_copies(const Variant &v, int ns);
+size_t pairs_tab_lds_bytes(const Variant &v, int ns);
 
 std::vector<Variant> &variant_table();
 const Variant *pick_variant(int lq, double mean_lr, bool packed, bool local, int ns, size_t lds_limit,

@@ -156,12 +156,16 @@ struct SweepOut {
 // PF: for low occupancy (one or two waves per SIMD) the reference code of the next row is
 // loaded one step ahead, from a clamped index so the load needs no branch and nothing
 // waits for it until the next step's profile address.
-template <typename V, int G, int R, bool LOCAL, bool STRIPE = false, bool PF_ = false>
+// TAB: no profile; prof_lane is a substitution table of trow bytes per reference code
+// and toff[kk] the byte offset of column kk's (query A code, query B code) entry in a row,
+// so each column costs one LDS load (see the pairs table kernel).
+template <typename V, int G, int R, bool LOCAL, bool STRIPE = false, bool PF_ = false, bool TAB = false>
 __device__ __forceinline__ SweepOut sweep_group(const uint8_t *__restrict__ rseq, int lr, int nsteps,
                                                 const char *prof_lane, int t, const SweepConst &k,
                                                 int lqa, int lqb, int cbase, uint32_t *bh = nullptr,
                                                 uint32_t *be = nullptr, bool bnd_in = false,
-                                                bool bnd_out = false) {
+                                                bool bnd_out = false, const int *toff = nullptr,
+                                                int trow = 0) {
     constexpr bool PF = PF_ || STRIPE;   // stripes: no code-load wait right after the boundary store
     using O = VOps<V>;
     using L = ProfLayout<G, R>;
@@ -200,14 +204,20 @@ __device__ __forceinline__ SweepOut sweep_group(const uint8_t *__restrict__ rseq
     // out-of-range codes score as the sentinel
     auto code_at = [&](int r) -> int { return min((int)rseq[r], k.ns - 1); };
     auto load_row = [&](int c, V *dst) {
-        const char *pc = prof_lane + c * L::ROWB;
+        if constexpr (TAB) {
+            const char *pc = prof_lane + c * trow;
 #pragma unroll
-        for (int kc = 0; kc < KC; ++kc) {
-            const int4 x = *reinterpret_cast<const int4 *>(pc + kc * L::CH);
-            dst[4 * kc + 0] = O::from_bits(x.x);
-            dst[4 * kc + 1] = O::from_bits(x.y);
-            dst[4 * kc + 2] = O::from_bits(x.z);
-            dst[4 * kc + 3] = O::from_bits(x.w);
+            for (int kk = 0; kk < R; ++kk) dst[kk] = O::from_bits(*reinterpret_cast<const int *>(pc + toff[kk]));
+        } else {
+            const char *pc = prof_lane + c * L::ROWB;
+#pragma unroll
+            for (int kc = 0; kc < KC; ++kc) {
+                const int4 x = *reinterpret_cast<const int4 *>(pc + kc * L::CH);
+                dst[4 * kc + 0] = O::from_bits(x.x);
+                dst[4 * kc + 1] = O::from_bits(x.y);
+                dst[4 * kc + 2] = O::from_bits(x.z);
+                dst[4 * kc + 3] = O::from_bits(x.w);
+            }
         }
     };
     // column-0 boundary H(i, 0) of lane 0's next row, kept as a running (saturating) value

@@ -264,6 +264,74 @@ __global__ void __launch_bounds__(WAVES * 64) pairs_kernel(KArgs a) {
     }
 }
 
+// ---------------------------------------------------------------- pairs table kernel
+// Explicit pairs with the substitution table in LDS instead of per-lane profiles, for
+// alphabets whose profiles do not fit (protein beyond 160 columns) and to free LDS for
+// occupancy.  The workgroup builds T[c][a][b] = (sub[c][a] | sub[c][b] << 16) once, where
+// a, b run over the query codes plus two padding codes (NEG past a query's end, 0 for
+// front padding); int32 tiles use T[c][a].  Each lane keeps its R columns' byte offsets
+// into a table row in registers, so a column costs one LDS load and an address add.
+// Small tables are stored in pairs_tab_copies() interleaved copies (word w of copy c at
+// w*copies + c) and lane l reads copy l mod copies: the lanes of an LDS pass then hit
+// distinct banks whatever entries they look up.
+template <typename V, int G, int R, bool LOCAL, int WAVES>
+__global__ void __launch_bounds__(WAVES * 64) pairs_tab_kernel(KArgs a) {
+    using O = VOps<V>;
+    constexpr int GPW = 64 / G;
+    extern __shared__ __align__(16) char lds[];
+    const int lane = threadIdx.x & 63;
+    const int t = lane % G, g = lane / G;
+    const int ns = a.ns, nq = ns + 2, npad = ns, zpad = ns + 1;
+    const int row_words = O::PACKED ? nq * nq : nq;
+    const int copies = a.tab_copies;
+    for (int idx = threadIdx.x; idx < ns * row_words * copies; idx += blockDim.x) {
+        const int w = idx / copies;
+        const int c = w / row_words, rem = w - c * row_words;
+        const int qa = O::PACKED ? rem / nq : rem, qb = O::PACKED ? rem - (rem / nq) * nq : rem;
+        const int va = qa == npad ? O::NEG : qa == zpad ? 0 : a.sub[c * ns + qa];
+        const int vb = qb == npad ? O::NEG : qb == zpad ? 0 : a.sub[c * ns + qb];
+        reinterpret_cast<uint32_t *>(lds)[idx] = O::pack(va, vb);
+    }
+    __syncthreads();
+    const int trow = row_words * copies * 4;
+    const char *tab_lane = lds + (lane % copies) * 4;
+    const PairTask *tasks = static_cast<const PairTask *>(a.items);
+    const int nitems = a.nitems_ptr ? __builtin_amdgcn_readfirstlane(*a.nitems_ptr) : a.nitems;
+    const int nblocks = (nitems + GPW - 1) / GPW;
+    for (;;) {
+        int blk = 0;
+        if (lane == 0) blk = atomicAdd(a.counter, 1);
+        blk = __builtin_amdgcn_readfirstlane(blk);
+        if (blk >= nblocks) break;
+        const int ti = blk * GPW + g;
+        PairTask task{-1, -1, -1, 0, -1, -1};
+        if (ti < nitems) task = tasks[ti];
+        const int shift = __builtin_amdgcn_readfirstlane(tasks[blk * GPW].shift);
+        const bool valid = task.ref >= 0;
+        const int qa = valid ? task.qa : 0;
+        const int qb = (valid && task.qb >= 0) ? task.qb : qa;
+        const int qsa = valid ? a.qoff[qa] : 0, lqa = valid ? a.qoff[qa + 1] - qsa : 0;
+        const int qsb = valid ? a.qoff[qb] : 0, lqb = valid ? a.qoff[qb + 1] - qsb : 0;
+        const int rs = valid ? a.roff[task.ref] : 0;
+        const int lr = valid ? a.roff[task.ref + 1] - rs : 0;
+        int toff[R];
+#pragma unroll
+        for (int kk = 0; kk < R; ++kk) {
+            const int col = t * R + kk - shift;
+            const int ca = col >= lqa ? npad : col < 0 ? zpad : min((int)a.qcodes[qsa + col], ns - 1);
+            const int cb = col >= lqb ? npad : col < 0 ? zpad : min((int)a.qcodes[qsb + col], ns - 1);
+            toff[kk] = (O::PACKED ? ca * nq + cb : ca) * copies * 4;
+        }
+        const int nsteps = __builtin_amdgcn_readfirstlane(wave_max_i32(lr)) + G - 1;
+        const SweepOut res = sweep_group<V, G, R, LOCAL, false, true, true>(
+            a.rcodes + rs, lr, nsteps, tab_lane, t, a.k, lqa, lqb, -shift, nullptr, nullptr, false, false, toff, trow);
+        if (t == 0 && valid) {
+            emit(a, task.oa, qsa, lqa, task.ref, res.a);
+            if (task.ob >= 0) emit(a, task.ob, qsb, lqb, task.ref, res.b);
+        }
+    }
+}
+
 // ---------------------------------------------------------------- edge cells
 // Closed-form scores of pairs with an empty side, written to out[o[k]].
 __global__ void scatter_kernel(int32_t *out, const int64_t *idx, const int32_t *val, int64_t n) {
@@ -290,7 +358,8 @@ static void launch_stripe(const KArgs &a, int grid, size_t lds, hipStream_t st) 
 #define GSW_VARIANT(V, PK, G, R, L)                                                               \
     Variant{0, G, R, PK, L, reinterpret_cast<const void *>(&cross_kernel<V, G, R, L, CROSS_WAVES>), \
             reinterpret_cast<const void *>(&pairs_kernel<V, G, R, L, PAIRS_WAVES>), nullptr,      \
-            &launch_cross<V, G, R, L>, &launch_pairs<V, G, R, L>, nullptr}
+            &launch_cross<V, G, R, L>, &launch_pairs<V, G, R, L>, nullptr,                        \
+            reinterpret_cast<const void *>(&pairs_tab_kernel<V, G, R, L, PAIRS_TAB_WAVES>)}
 #define GSW_VARIANT4(G, R)                                                                        \
     GSW_VARIANT(s2, true, G, R, true), GSW_VARIANT(s2, true, G, R, false),                        \
         GSW_VARIANT(int, false, G, R, true), GSW_VARIANT(int, false, G, R, false)
@@ -299,7 +368,8 @@ static void launch_stripe(const KArgs &a, int grid, size_t lds, hipStream_t st) 
     Variant{0, G, R, PK, L, reinterpret_cast<const void *>(&cross_kernel<V, G, R, L, CROSS_WAVES>), \
             reinterpret_cast<const void *>(&pairs_kernel<V, G, R, L, PAIRS_WAVES>),               \
             reinterpret_cast<const void *>(&stripe_kernel<V, G, R, L, CROSS_WAVES>),              \
-            &launch_cross<V, G, R, L>, &launch_pairs<V, G, R, L>, &launch_stripe<V, G, R, L>}
+            &launch_cross<V, G, R, L>, &launch_pairs<V, G, R, L>, &launch_stripe<V, G, R, L>,     \
+            reinterpret_cast<const void *>(&pairs_tab_kernel<V, G, R, L, PAIRS_TAB_WAVES>)}
 #define GSW_SVARIANT4(G, R)                                                                       \
     GSW_SVARIANT(s2, true, G, R, true), GSW_SVARIANT(s2, true, G, R, false),                      \
         GSW_SVARIANT(int, false, G, R, true), GSW_SVARIANT(int, false, G, R, false)
@@ -444,6 +514,47 @@ size_t cross_lds_bytes(const Variant &v, int ns) {
 size_t pairs_wave_bytes(const Variant &v, int ns) { return (size_t)ns * ((v.R + 3) / 4) * 64 * 16; }
 int pairs_waves(const Variant &v, int ns, size_t lds_limit) {
     return (int)std::min<size_t>(PAIRS_WAVES, lds_limit / pairs_wave_bytes(v, ns));
+}
+int pairs_tab_copies(const Variant &v, int ns) {
+    const size_t nq = (size_t)ns + 2;
+    const size_t one = 4 * (size_t)ns * (v.packed ? nq * nq : nq);
+    return 32 * one <= 48 * 1024 ? 32 : 1;
+}
+size_t pairs_tab_lds_bytes(const Variant &v, int ns) {
+    const size_t nq = (size_t)ns + 2;
+    return 4 * (size_t)ns * (v.packed ? nq * nq : nq) * pairs_tab_copies(v, ns);
+}
+
+// Explicit pairs run the table kernel (measured faster than per-lane profiles on the
+// 1,000,000 read-pair workload: 3 waves per SIMD against 1-2).  GSW_PAIRS_TAB=0 prefers
+// per-lane profiles where they fit LDS.
+static bool pairs_prefer_profiles() {
+    const char *e = std::getenv("GSW_PAIRS_TAB");
+    return e && e[0] == '0';
+}
+
+// Tile of an explicit pair with query length lq; *tab: run it with the table kernel.
+static const Variant *pick_pairs_tile(int lq, double mean_lr, bool packed, bool local, int ns, size_t lds_limit,
+                                      bool *tab) {
+    *tab = false;
+    if (pairs_prefer_profiles()) {
+        const Variant *v = pick_variant(lq, mean_lr, packed, local, ns, lds_limit, true);
+        if (v) return v;
+    }
+    const Variant *best = nullptr;
+    double best_cost = 1e30;
+    for (const Variant &v : variant_table()) {
+        if (!v.pairs_tab_fn || v.packed != packed || v.local != local || v.G * v.R < lq) continue;
+        if (pairs_tab_lds_bytes(v, ns) > lds_limit) continue;
+        if (g_force_g && (v.G != g_force_g || v.R != g_force_r)) continue;
+        const double c = tile_cost(v.G, v.R, lq, mean_lr);
+        if (c < best_cost) {
+            best_cost = c;
+            best = &v;
+        }
+    }
+    *tab = best != nullptr;
+    return best;
 }
 
 int occupancy_grid(DeviceCtx *ctx, const void *fn, int threads, size_t lds, int *grid) {
@@ -900,13 +1011,13 @@ struct PairGraph {
 static std::mutex g_graph_mu;
 static std::map<std::tuple<int, int, int>, std::unique_ptr<PairGraph>> g_pair_graphs;
 
-static int pair_graph(DeviceCtx *ctx, const Variant &v, int slot, int ns, PairGraph **out) {
+static int pair_graph(DeviceCtx *ctx, const Variant &v, bool tab, int slot, int ns, PairGraph **out) {
     std::lock_guard<std::mutex> lk(g_graph_mu);
-    auto key = std::make_tuple(ctx->device, v.id, slot);
+    auto key = std::make_tuple(ctx->device, 2 * v.id + (tab ? 1 : 0), slot);
     auto &pg = g_pair_graphs[key];
-    const int waves = pairs_waves(v, ns, (size_t)ctx->max_smem);
+    const int waves = tab ? PAIRS_TAB_WAVES : pairs_waves(v, ns, (size_t)ctx->max_smem);
     if (waves == 0) return fail(GSW_ERR_LIMIT, "pairs tile (%d, %d) does not fit in LDS", v.G, v.R);
-    const size_t lds = (size_t)waves * pairs_wave_bytes(v, ns);
+    const size_t lds = tab ? pairs_tab_lds_bytes(v, ns) : (size_t)waves * pairs_wave_bytes(v, ns);
     if (pg && pg->lds == lds) {
         *out = pg.get();
         return GSW_OK;
@@ -925,14 +1036,15 @@ static int pair_graph(DeviceCtx *ctx, const Variant &v, int slot, int ns, PairGr
     }
     pg->lds = lds;
     pg->waves = waves;
-    GSW_TRY(occupancy_grid(ctx, v.pairs_fn, waves * 64, lds, &pg->grid));
+    const void *fn = tab ? v.pairs_tab_fn : v.pairs_fn;
+    GSW_TRY(occupancy_grid(ctx, fn, waves * 64, lds, &pg->grid));
     GSW_HIP_TRY(hipGraphCreate(&pg->graph, 0));
     GSW_HIP_TRY(hipGraphAddMemcpyNode1D(&pg->copy, pg->graph, nullptr, 0, pg->d_buf, pg->h_buf, pg->cap,
                                         hipMemcpyHostToDevice));
     KArgs a{};
     void *params[] = {&a};
     hipKernelNodeParams kp{};
-    kp.func = const_cast<void *>(v.pairs_fn);
+    kp.func = const_cast<void *>(fn);
     kp.gridDim = dim3(pg->grid);
     kp.blockDim = dim3(waves * 64);
     kp.sharedMemBytes = (unsigned)lds;
@@ -974,7 +1086,7 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
     const int wmax = max_width();
     const double mean_lr = rs.n_nonempty ? (double)rs.total / rs.n_nonempty : 1.0;
     const int T = n >= 65536 ? host_threads() : 1;
-    const size_t nv = variant_table().size();
+    const size_t nv = 2 * variant_table().size();   // kinds: 2 * variant id + (table kernel)
 
     // -- classify in parallel over contiguous index ranges: empty side (closed form), no
     //    tile (comparator), else a tile
@@ -992,7 +1104,7 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
         Part &P = parts[tid];
         P.count.assign(nv, 0);
         std::vector<const Variant *> vcache(2 * (wmax + 1), nullptr);
-        std::vector<char> vdone(2 * (wmax + 1), 0);
+        std::vector<char> vdone(2 * (wmax + 1), 0), vtab(2 * (wmax + 1), 0);
         const int64_t k0 = n * tid / T, k1 = n * (tid + 1) / T;
         for (int64_t k = k0; k < k1; ++k) {
             const int q = h_qi[k], r = h_rj[k];
@@ -1015,13 +1127,17 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
             }
             const bool use16 = dtype_policy != GSW_DTYPE_INT32 && fits_int16(s, lq, lr);
             const Variant *v = nullptr;
+            bool tab = false;
             if (lq <= wmax) {
                 const size_t ci = (size_t)(use16 ? 1 : 0) * (wmax + 1) + lq;
                 if (!vdone[ci]) {
-                    vcache[ci] = pick_variant(lq, mean_lr, use16, local, s.ns, (size_t)ctx->max_smem, true);
+                    bool tb = false;
+                    vcache[ci] = pick_pairs_tile(lq, mean_lr, use16, local, s.ns, (size_t)ctx->max_smem, &tb);
+                    vtab[ci] = tb;
                     vdone[ci] = 1;
                 }
                 v = vcache[ci];
+                tab = vtab[ci];
             }
             if (!v) {
                 P.p_qi.push_back(q);
@@ -1029,8 +1145,8 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
                 P.p_oi.push_back(k);
                 continue;
             }
-            vid[k] = (int16_t)v->id;
-            P.count[v->id]++;
+            vid[k] = (int16_t)(2 * v->id + (tab ? 1 : 0));
+            P.count[2 * v->id + (tab ? 1 : 0)]++;
         }
     });
     for (const Part &P : parts)   // ranges are in index order: the first error is the lowest index
@@ -1070,18 +1186,20 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
     });
 
     // -- batches of PAIR_BATCH members of one tile; a tile's batches cycle through its slots
-    struct Batch { const Variant *v; int64_t b0, b1; int bi, nslot; };
+    struct Batch { const Variant *v; bool tab; int kind; int64_t b0, b1; int bi, nslot; };
     std::vector<Batch> batches;
-    int dominant = -1;
+    int dominant = -1;   // kind with the most pairs
     int64_t dominant_n = 0;
-    for (const Variant &v : variant_table()) {
-        const int64_t cnt_v = vbeg[v.id + 1] - vbeg[v.id];
+    for (size_t kind = 0; kind < nv; ++kind) {
+        const int64_t cnt_v = vbeg[kind + 1] - vbeg[kind];
         if (cnt_v == 0) continue;
-        if (cnt_v > dominant_n) { dominant_n = cnt_v; dominant = v.id; }
+        if (cnt_v > dominant_n) { dominant_n = cnt_v; dominant = (int)kind; }
+        const Variant &v = variant_table()[kind / 2];
         const int nb = (int)((cnt_v + PAIR_BATCH - 1) / PAIR_BATCH);
         for (int bi = 0; bi < nb; ++bi)
-            batches.push_back(Batch{&v, vbeg[v.id] + bi * PAIR_BATCH, std::min(vbeg[v.id + 1], vbeg[v.id] + (bi + 1) * PAIR_BATCH),
-                                    bi, std::min(nb, MAX_PAIR_SLOTS)});
+            batches.push_back(Batch{&v, (kind & 1) != 0, (int)kind, vbeg[kind] + bi * PAIR_BATCH,
+                                    std::min(vbeg[kind + 1], vbeg[kind] + (bi + 1) * PAIR_BATCH), bi,
+                                    std::min(nb, MAX_PAIR_SLOTS)});
     }
 
     SatBuffers sb;
@@ -1091,8 +1209,8 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
     for (const Batch &B : batches) {
         if (B.bi >= B.nslot) continue;
         PairGraph *pg = nullptr;
-        GSW_TRY(pair_graph(ctx, *B.v, B.bi, s.ns, &pg));
-        slot_of[{B.v->id, B.bi}] = pg;
+        GSW_TRY(pair_graph(ctx, *B.v, B.tab, B.bi, s.ns, &pg));
+        slot_of[{B.kind, B.bi}] = pg;
         KArgs a = base_args(s, qs, rs, d_out);
         a.sat = sb.d_sat;
         a.sat_count = ctx->counters + 1;
@@ -1103,7 +1221,8 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
         a.counter = reinterpret_cast<int32_t *>(pg->d_buf) + 1;
         void *params[] = {&a};
         hipKernelNodeParams kp{};
-        kp.func = const_cast<void *>(B.v->pairs_fn);
+        a.tab_copies = B.tab ? pairs_tab_copies(*B.v, s.ns) : 1;
+        kp.func = const_cast<void *>(B.tab ? B.v->pairs_tab_fn : B.v->pairs_fn);
         kp.gridDim = dim3(pg->grid);
         kp.blockDim = dim3(pg->waves * 64);
         kp.sharedMemBytes = (unsigned)pg->lds;
@@ -1181,7 +1300,7 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
             const size_t b = next.fetch_add(1);
             if (b >= nb) return;
             const Batch &B = batches[b];
-            PairGraph &g = *slot_of.at({B.v->id, B.bi % B.nslot});
+            PairGraph &g = *slot_of.at({B.kind, B.bi % B.nslot});
             if (B.bi >= B.nslot) {
                 const size_t prev = b - B.nslot;   // same tile: its batches are contiguous
                 std::unique_lock<std::mutex> lk(m);
@@ -1227,7 +1346,7 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
         for (auto &x : workers) x.join();
         workers.clear();
     };
-    while (ctx->marks.size() < nb) {   // one timing event after each batch
+    while (ctx->marks.size() < 2 * nb) {   // timing events before and after each batch
         hipEvent_t e;
         if (hipEventCreate(&e) != hipSuccess) {
             stop();
@@ -1247,13 +1366,14 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
             }
         }
         const Batch &B = batches[b];
-        PairGraph &g = *slot_of.at({B.v->id, B.bi % B.nslot});
+        PairGraph &g = *slot_of.at({B.kind, B.bi % B.nslot});
         hipError_t e = hipGraphExecMemcpyNodeSetParams1D(g.exec, g.copy, g.d_buf, g.h_buf,
                                                          PAIR_HDR + sizeof(PairTask) * (size_t)ntasks[b],
                                                          hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipEventRecord(ctx->marks[2 * b], ctx->stream);
         if (e == hipSuccess) e = hipGraphLaunch(g.exec, ctx->stream);
         if (e == hipSuccess) e = hipEventRecord(g.done, ctx->stream);
-        if (e == hipSuccess) e = hipEventRecord(ctx->marks[b], ctx->stream);
+        if (e == hipSuccess) e = hipEventRecord(ctx->marks[2 * b + 1], ctx->stream);
         if (e != hipSuccess) {
             rc = fail(GSW_ERR_HIP, "pairs batch launch: %s", hipGetErrorString(e));
             break;
@@ -1275,16 +1395,16 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
     float ms = 0.f;
     GSW_HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     res->kernel_ms += ms;
-    res->variant = dominant;
-    // the dominant tile's batches: mean time between consecutive batch events (the graph's
-    // task copy and kernel) and mean cells per batch
+    res->variant = dominant >= 0 ? dominant / 2 : -1;
+    // the dominant tile's batches: mean device time of a batch (the graph's task copy and
+    // kernel, between events recorded right before and after its launch) and mean cells
     {
         double sum_ms = 0, sum_cells = 0;
         int cnt = 0;
         for (size_t b = 0; b < nb; ++b) {
-            if (batches[b].v->id != dominant) continue;
+            if (batches[b].kind != dominant) continue;
             float bms = 0.f;
-            GSW_HIP_TRY(hipEventElapsedTime(&bms, b ? ctx->marks[b - 1] : ctx->ev0, ctx->marks[b]));
+            GSW_HIP_TRY(hipEventElapsedTime(&bms, ctx->marks[2 * b], ctx->marks[2 * b + 1]));
             sum_ms += bms;
             sum_cells += batch_cells[b];
             ++cnt;
@@ -1292,8 +1412,8 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
         if (cnt) {
             res->top_ms = sum_ms / cnt;
             res->top_cells = (int64_t)(sum_cells / cnt);
-            res->top_variant = dominant;
-            res->top_striped = 2;   // pairs kernel
+            res->top_variant = dominant / 2;
+            res->top_striped = (dominant & 1) ? 3 : 2;   // pairs kernel (3: table kernel)
         }
     }
 

@@ -63,3 +63,41 @@ def test_many_batches_two_tiles_slot_reuse():
     pick = rng.choice(n, 2000, replace=False)
     want = native.score_encoded_pairs(al._queries, al._refs, qi[pick], rj[pick], scheme)
     assert np.array_equal(got[pick].astype(np.int64), want)
+
+
+@pytest.mark.parametrize("mode", ["local", "global", "semiglobal"])
+def test_protein_pairs_table_kernel(mode):
+    """Protein queries past 160 columns have no profile tile that fits LDS: the table
+    kernel scores them (no comparator), bit-exact."""
+    from tests.helpers import PROT, rand_seqs
+
+    rng = np.random.default_rng(41)
+    scheme = Scheme.preset("blosum62").replace(mode=mode)
+    queries = rand_seqs(rng, 60, 150, 900, PROT)
+    refs = rand_seqs(rng, 40, 50, 700, PROT) + [queries[0][20:500]]
+    qi = rng.integers(0, len(queries), 2000).astype(np.int32)
+    rj = rng.integers(0, len(refs), 2000).astype(np.int32)
+    qi[0], rj[0] = 0, len(refs) - 1
+    al = Aligner(scheme, backend="hip").index(refs).set_queries(queries)
+    got = al.score_pairs(qi, rj)
+    assert al._b.last_stats.plain_pairs == 0
+    want = native.score_encoded_pairs(al._queries, al._refs, qi, rj, scheme)
+    assert np.array_equal(got.astype(np.int64), want)
+
+
+@pytest.mark.parametrize("mode", ["local", "global", "semiglobal"])
+@pytest.mark.parametrize("tab", ["0", "1"])
+def test_dna_pairs_both_kernels(mode, tab, monkeypatch):
+    """DNA pairs through the table kernel (default) and through per-lane profiles."""
+    monkeypatch.setenv("GSW_PAIRS_TAB", tab)
+    rng = np.random.default_rng(42)
+    scheme = Scheme.preset("dna").replace(mode=mode)
+    queries = [rng.integers(0, 4, int(L)).astype(np.uint8) for L in rng.integers(1, 400, 300)]
+    refs = [rng.integers(0, 4, int(L)).astype(np.uint8) for L in rng.integers(1, 500, 80)]
+    qi = rng.integers(0, 300, 5000).astype(np.int32)
+    rj = rng.integers(0, 80, 5000).astype(np.int32)
+    al = Aligner(scheme, backend="hip").index(synth.pack(refs)).set_queries(synth.pack(queries))
+    got = al.score_pairs(qi, rj)
+    assert al._b.last_stats.top_striped == (3 if tab == "1" else 2)
+    want = native.score_encoded_pairs(al._queries, al._refs, qi, rj, scheme)
+    assert np.array_equal(got.astype(np.int64), want)
