@@ -103,6 +103,7 @@ _SIGNATURES = {
     "gsw_set_tile_table": [_vp, _i32],
     "gsw_force_variant": [_i32, _i32],
     "gsw_list_variants": [_vp, _i32, ctypes.POINTER(_i32)],
+    "gsw_list_stripe_variants": [_vp, _i32, ctypes.POINTER(_i32)],
     "gsw_fasta_scan": [_vp, _i64, ctypes.POINTER(_i64), ctypes.POINTER(_i64)],
     "gsw_traceback_pairs": [ctypes.POINTER(GswScheme), _vp, _vp, _vp, _vp, _i64, _vp, _vp, _i64,
                             ctypes.POINTER(GswStats)],
@@ -284,6 +285,13 @@ def list_variants() -> list[tuple[int, int]]:
     buf = np.zeros(2 * 256, dtype=np.int32)
     n = _i32(0)
     call("gsw_list_variants", ptr(buf), 256, ctypes.byref(n))
+    return [(int(buf[2 * i]), int(buf[2 * i + 1])) for i in range(min(n.value, 256))]
+
+
+def list_stripe_variants() -> list[tuple[int, int]]:
+    buf = np.zeros(2 * 256, dtype=np.int32)
+    n = _i32(0)
+    call("gsw_list_stripe_variants", ptr(buf), 256, ctypes.byref(n))
     return [(int(buf[2 * i]), int(buf[2 * i + 1])) for i in range(min(n.value, 256))]
 
 

@@ -1,9 +1,10 @@
 """Measured tile autotuner for the batched wavefront kernels.
 
 The library is compiled for a set of tiles: G lanes per reference (lanes-per-pair) and R
-query columns per lane (query tile height W = G*R).  Its built-in cost model picks a
-tile per query length from padding, per-step overhead and pipeline fill; this module
-replaces the model with measurements.  For every query-length bin it times each tile
+query columns per lane (query tile height W = G*R); some tiles also run wider queries as
+stripes of W columns.  Its built-in cost model picks a tile (or stripes) per query length
+from padding, per-step overhead and pipeline fill; this module replaces the model with
+measurements.  For every query-length bin it times each tile
 that covers the bin on a synthetic workload shaped like the caller's database (reference
 lengths drawn from the real ones), for the int16 and int32 element types the scheme can
 use, keeps the fastest, and installs the rules with ``gsw_set_tile_table``.  Results are
@@ -43,11 +44,15 @@ def clear() -> None:
 
 
 def _time_tile(al: Aligner, G: int, R: int, reps: int) -> float:
+    """Best kernel time of the bin's queries on tile (G, R); inf when the tile cannot run
+    them (its profile does not fit LDS and the comparator would score them)."""
     _hip.force_variant(G, R)
     try:
         best = float("inf")
         for _ in range(reps):
             al.score_cross()
+            if al._b.last_stats.plain_pairs:
+                return float("inf")
             best = min(best, al._b.last_stats.kernel_ms)
         return best
     finally:
@@ -68,6 +73,7 @@ def tune(refs, scheme="dna", bins=None, n_queries: int = 128, n_refs: int = 4000
     renc = funnel(refs, sch, id_prefix="r")
     rlen = renc.lengths[renc.lengths > 0]
     tiles = _hip.list_variants()
+    stripe_tiles = set(_hip.list_stripe_variants())
     wmax = max(g * r for g, r in tiles)
     if bins is None:
         bins = [(lo, min(lo + 31, wmax)) for lo in range(1, wmax + 1, 32)]
@@ -87,16 +93,15 @@ def tune(refs, scheme="dna", bins=None, n_queries: int = 128, n_refs: int = 4000
             al = Aligner(s2, backend="hip").index(pack(refs_syn)).set_queries(queries)
             timings = []
             for G, R in tiles:
-                if G * R < hi:
-                    continue
-                try:
-                    ms = _time_tile(al, G, R, reps)
-                except Exception:     # tile does not fit (LDS) for this alphabet
-                    continue
-                timings.append((ms, G, R))
+                if G * R < hi and (G, R) not in stripe_tiles:
+                    continue   # neither covers the bin nor runs it as stripes
+                ms = _time_tile(al, G, R, reps)
+                if ms != float("inf"):
+                    timings.append((ms, G, R))
             if not timings:
                 continue
             ms, G, R = min(timings)
+            # a rule names a tile; one narrower than the bin means stripes of that tile
             rules.append({"lo": int(lo), "hi": int(hi), "packed": dtype == "int16", "local": local,
                           "G": int(G), "R": int(R), "ms": round(ms, 3),
                           "candidates": [[g, r, round(t, 3)] for t, g, r in sorted(timings)]})

@@ -389,7 +389,13 @@ std::vector<Variant> &variant_table() {
 }
 
 // Forced variant (G, R) for tuning; 0 = automatic.
-static int g_force_g = 0, g_force_r = 0;
+// Packed (G << 16 | R) in one atomic: planner threads read it while a caller may set it.
+static std::atomic<int> g_force{0};
+static inline void forced_tile(int *g, int *r) {
+    const int f = g_force.load(std::memory_order_relaxed);
+    *g = f >> 16;
+    *r = f & 0xFFFF;
+}
 
 // Measured tile table (from the autotuner): for a query length in [lo, hi] and the given
 // element type / mode, use tile (G, R) when it is compiled, covers the query and fits.
@@ -399,15 +405,15 @@ static std::vector<TileRule> g_rules;
 
 // Estimated relative cost per cell of a (G, R) tile for a query of length lq against
 // references of mean length lr: padded columns, per-step overhead and pipeline fill.
+// (Measured on BLOSUM62 top-k, 1,000 queries x 100,000 references: padded-column rates of
+// the tiles from (8,20) to (32,48) differ by no more than the fill term explains, so wider
+// R, despite fewer waves per SIMD, carries no penalty.)
 static double tile_cost(int G, int R, int lq, double lr) {
     const double width = (double)G * R;
     const double pad = width / std::max(lq, 1);
     const double step = (9.0 * R + 14.0) / (9.0 * R);
     const double fill = (lr + G - 1) / std::max(lr, 1.0);
-    double regs = 1.0;                 // fewer waves per SIMD above R = 32 (more registers)
-    if (R > 44) regs = 1.08;           // 2 waves
-    else if (R > 32) regs = 1.03;      // 3 waves
-    return pad * step * fill * regs;
+    return pad * step * fill;
 }
 
 
@@ -417,6 +423,8 @@ const Variant *pick_variant(int lq, double mean_lr, bool packed, bool local, int
                                    size_t lds_limit, bool for_pairs) {
     const Variant *best = nullptr;
     double best_cost = 1e30;
+    int g_force_g, g_force_r;
+    forced_tile(&g_force_g, &g_force_r);
     if (!g_force_g) {
         std::lock_guard<std::mutex> lk(g_rules_mu);
         for (const TileRule &r : g_rules) {
@@ -443,12 +451,13 @@ const Variant *pick_variant(int lq, double mean_lr, bool packed, bool local, int
 
 // Relative cost per cell of running a query of length lq as nstr stripes of a (G, R)
 // tile: padding of the last stripe, per-step overhead including the boundary load and
-// store, pipeline fill per stripe, and the per-stripe profile build and barrier.
+// store, pipeline fill per stripe, and the per-stripe profile build, boundary traffic
+// and barrier (measured: about 12 % more per padded column than a single tile).
 static double stripe_cost(int G, int R, int lq, double lr, int nstr) {
     const double pad = (double)nstr * G * R / std::max(lq, 1);
     const double step = (9.0 * R + 17.0) / (9.0 * R);
     const double fill = (lr + G - 1) / std::max(lr, 1.0);
-    return pad * step * fill * 1.03;
+    return pad * step * fill * 1.10;
 }
 
 static int stripes_for(const Variant &v, int lq, bool track_last_col) {
@@ -460,6 +469,8 @@ static int stripes_for(const Variant &v, int lq, bool track_last_col) {
 const Variant *pick_cross_tile(int lq, double mean_lr, bool packed, bool local, int ns, size_t lds_limit,
                                bool track_last_col, int *nstr) {
     *nstr = 1;
+    int g_force_g, g_force_r;
+    forced_tile(&g_force_g, &g_force_r);
     auto usable = [&](const Variant &v, int G, int R) {
         return v.G == G && v.R == R && v.packed == packed && v.local == local && cross_lds_bytes(v, ns) <= lds_limit;
     };
@@ -543,6 +554,8 @@ static const Variant *pick_pairs_tile(int lq, double mean_lr, bool packed, bool 
     }
     const Variant *best = nullptr;
     double best_cost = 1e30;
+    int g_force_g, g_force_r;
+    forced_tile(&g_force_g, &g_force_r);
     for (const Variant &v : variant_table()) {
         if (!v.pairs_tab_fn || v.packed != packed || v.local != local || v.G * v.R < lq) continue;
         if (pairs_tab_lds_bytes(v, ns) > lds_limit) continue;
@@ -750,6 +763,36 @@ void expand_cross_items(const CrossUnit *u0, const CrossUnit *u1, const SeqSet &
     });
 }
 
+void expand_cross_items_balanced(const CrossUnit *u0, const CrossUnit *u1, const SeqSet &rs, int grid,
+                                 int per_wg, int64_t min_rows, int gpw, std::vector<CrossItem> *items) {
+    const int64_t nref = rs.n_nonempty;
+    const int64_t nu = (int64_t)(u1 - u0);
+    const int64_t unit_rows = rs.bpos[nref];
+    const int64_t want = std::max<int64_t>(nu, (int64_t)per_wg * std::max(grid, 1));
+    const int64_t max_pieces = std::max<int64_t>(1, unit_rows / std::max<int64_t>(min_rows, 1));
+    const size_t first = items->size();
+    for (int64_t i = 0; i < nu; ++i) {
+        const CrossUnit *u = u0 + i;
+        // `want` pieces over the units, the first `want % nu` units one more
+        const int64_t pieces = std::min<int64_t>(max_pieces, want / nu + (i < want % nu ? 1 : 0));
+        int64_t rb = 0;
+        for (int64_t j = 1; j <= pieces && rb < nref; ++j) {
+            // cut at the first whole block of gpw references reaching j/pieces of the rows
+            const int64_t goal = unit_rows * j / pieces;
+            int64_t re = std::min<int64_t>(nref, rb + gpw);
+            while (re < nref && rs.bpos[re] < goal) re = std::min<int64_t>(nref, re + gpw);
+            if (j == pieces) re = nref;
+            items->push_back(CrossItem{u->a.start, u->a.len, u->has_b ? u->b.start : 0, u->has_b ? u->b.len : -1,
+                                       u->a.out, u->has_b ? u->b.out : -1, (int32_t)rb, (int32_t)re, u->shift,
+                                       u->nstr});
+            rb = re;
+        }
+    }
+    std::stable_sort(items->begin() + first, items->end(), [&](const CrossItem &x, const CrossItem &y) {
+        return rs.bpos[x.rend] - rs.bpos[x.rbeg] > rs.bpos[y.rend] - rs.bpos[y.rbeg];
+    });
+}
+
 int64_t stripe_item_rows(double mean_lr, int max_lr) {
     // about 16 blocks of 64/G references per wave and stripe, so the waves reach a stripe's
     // barrier within about one block of each other; at most 2^19 rows (4 MiB of boundaries
@@ -759,10 +802,16 @@ int64_t stripe_item_rows(double mean_lr, int max_lr) {
 }
 
 int64_t expand_stripe_items(const CrossUnit *u0, const CrossUnit *u1, const SeqSet &rs, int64_t max_rows,
-                            std::vector<CrossItem> *items) {
+                            std::vector<CrossItem> *items, int grid, int64_t min_rows) {
     const int64_t nref = rs.n_nonempty;
     const size_t first = items->size();
     int64_t widest = 0;
+    // launches of few long queries: items down to min_rows so that there are about two
+    // per workgroup, rather than leaving most of the GPU idle
+    if (grid > 0 && min_rows < max_rows) {
+        const int64_t rows = rs.bpos[nref] * (int64_t)(u1 - u0);
+        max_rows = std::max<int64_t>(min_rows, std::min<int64_t>(max_rows, rows / (2 * (int64_t)grid)));
+    }
     for (const CrossUnit *u = u0; u < u1; ++u) {
         for (int64_t rb = 0; rb < nref;) {
             int64_t re = rb + 1;
@@ -853,13 +902,13 @@ int batch_cross(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
         const int64_t nref = rs.n_nonempty;
         if (striped) {
             const int64_t rows = expand_stripe_items(units.data() + u0, units.data() + u1, rs,
-                                                     stripe_item_rows(mean_lr, rs.max_len), &items);
+                                                     stripe_item_rows(mean_lr, rs.max_len), &items, g.grid,
+                                                     (int64_t)(4.0 * gpw * CROSS_WAVES * mean_lr));
             bnd_entries = std::max(bnd_entries, rows * std::min<int64_t>(g.grid, (int64_t)(items.size() - g.i0)));
         } else {
-            // >= 4 items per workgroup, each at least one block per wave
-            int64_t rows = (rs.total * (int64_t)(u1 - u0) + 4LL * g.grid - 1) / (4LL * g.grid);
-            rows = std::max<int64_t>(rows, (int64_t)(gpw * CROSS_WAVES * mean_lr));
-            expand_cross_items(units.data() + u0, units.data() + u1, rs, rows, gpw, &items);
+            // 8 items per workgroup of about equal rows, each at least one block per wave
+            expand_cross_items_balanced(units.data() + u0, units.data() + u1, rs, g.grid, 8,
+                                        (int64_t)(gpw * CROSS_WAVES * mean_lr), gpw, &items);
             (void)nref;
         }
         g.i1 = items.size();
@@ -1525,13 +1574,12 @@ extern "C" int gsw_score_pairs(const gsw_scheme *s, int32_t dtype_policy, gsw_se
 // Tuning hook: force the (G, R) tile of the batched kernels (0, 0 = automatic).
 extern "C" int gsw_force_variant(int32_t G, int32_t R) {
     if (G == 0 && R == 0) {
-        g_force_g = g_force_r = 0;
+        g_force.store(0);
         return GSW_OK;
     }
     for (const Variant &v : variant_table())
         if (v.G == G && v.R == R) {
-            g_force_g = G;
-            g_force_r = R;
+            g_force.store((G << 16) | R);
             return GSW_OK;
         }
     return fail(GSW_ERR_INVALID, "no compiled variant G=%d R=%d", G, R);
@@ -1552,6 +1600,24 @@ extern "C" int gsw_set_tile_table(const int32_t *rules, int32_t n) {
 }
 
 // List compiled (G, R) tiles: writes up to cap (G, R) pairs, returns the count in *n.
+// (G, R) of the tiles that also have a stripe kernel (queries wider than the tile).
+extern "C" int gsw_list_stripe_variants(int32_t *gr, int32_t cap, int32_t *n) {
+    if (!n || (cap > 0 && !gr)) return fail(GSW_ERR_INVALID, "null argument");
+    int cnt = 0, last_g = -1, last_r = -1;
+    for (const Variant &v : variant_table()) {
+        if (!v.stripe_fn || (v.G == last_g && v.R == last_r)) continue;
+        last_g = v.G;
+        last_r = v.R;
+        if (cnt < cap) {
+            gr[2 * cnt] = v.G;
+            gr[2 * cnt + 1] = v.R;
+        }
+        ++cnt;
+    }
+    *n = cnt;
+    return GSW_OK;
+}
+
 extern "C" int gsw_list_variants(int32_t *gr, int32_t cap, int32_t *n) {
     int cnt = 0;
     int last_g = -1, last_r = -1;

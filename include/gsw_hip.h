@@ -150,6 +150,9 @@ int gsw_topk_rows(const int32_t *d_scores, int32_t nrows, int32_t ncols, int32_t
 int gsw_set_tile_table(const int32_t *rules, int32_t n);
 int gsw_force_variant(int32_t G, int32_t R);
 int gsw_list_variants(int32_t *gr, int32_t cap, int32_t *n);   /* (G, R) pairs */
+/* Tiles that also run queries wider than G*R as stripes (forcing one of them stripes
+ * every query it does not cover). */
+int gsw_list_stripe_variants(int32_t *gr, int32_t cap, int32_t *n);
 
 /* ---- traceback ------------------------------------------------------------
  * No counterpart in the reference, which scores only and re-aligns its top hits on the
