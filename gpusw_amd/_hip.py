@@ -151,7 +151,8 @@ def check(rc: int) -> None:
     if rc == GSW_ERR_NO_DEVICE:
         raise NoGpuError(msg)
     if rc == GSW_ERR_INVALID:
-        raise ValueError(msg)
+        # an index past the end of a sequence set, as numpy indexing would report it
+        raise (IndexError if "index out of range" in msg else ValueError)(msg)
     if rc == GSW_ERR_OVERFLOW:
         raise OverflowRiskError(msg)
     if rc == GSW_ERR_TIMEOUT:

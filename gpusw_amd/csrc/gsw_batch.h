@@ -112,15 +112,18 @@ const Variant *pick_cross_tile(int lq, double mean_lr, bool packed, bool local, 
 // target_rows rows); the appended items are ordered longest first.
 void expand_cross_items(const CrossUnit *u0, const CrossUnit *u1, const SeqSet &rs, int64_t target_rows,
                         int gpw, std::vector<CrossItem> *items);
-// The same for one persistent launch of `grid` workgroups: per_wg x grid items in all
-// (at least one per unit, none under min_rows), spread evenly over the units and cut at
+// The same for one persistent launch of `grid` workgroups: per_wg x grid items in all (at
+// least one per unit, rounded up to a multiple of grid, none under min_rows where the
+// unit allows), spread evenly over the units and cut at
 // equal row fractions, so every workgroup gets per_wg items of about the same work and the
 // launch has no partly filled last round.  Sorted longest first.
 void expand_cross_items_balanced(const CrossUnit *u0, const CrossUnit *u1, const SeqSet &rs, int grid,
                                  int per_wg, int64_t min_rows, int gpw, std::vector<CrossItem> *items);
-// Striped units: one item per (unit, run of sorted references of at most max_rows total
-// length, or one reference).  With grid > 0, max_rows shrinks (not below min_rows) until
-// there are about two items per workgroup.  Returns the largest item's row count.
+// Striped units: with grid == 0, one item per (unit, run of sorted references of at most
+// max_rows total length, or one reference); with grid > 0, a whole number (at least two)
+// of items per workgroup of the launch, each unit cut at equal row fractions into pieces of
+// about max_rows or less and, where the unit allows, at least min_rows.  Returns the
+// largest item's row count.
 int64_t stripe_item_rows(double mean_lr, int max_lr);
 int64_t expand_stripe_items(const CrossUnit *u0, const CrossUnit *u1, const SeqSet &rs, int64_t max_rows,
                             std::vector<CrossItem> *items, int grid = 0, int64_t min_rows = 0);

@@ -405,15 +405,23 @@ static std::vector<TileRule> g_rules;
 
 // Estimated relative cost per cell of a (G, R) tile for a query of length lq against
 // references of mean length lr: padded columns, per-step overhead and pipeline fill.
-// (Measured on BLOSUM62 top-k, 1,000 queries x 100,000 references: padded-column rates of
-// the tiles from (8,20) to (32,48) differ by no more than the fill term explains, so wider
-// R, despite fewer waves per SIMD, carries no penalty.)
 static double tile_cost(int G, int R, int lq, double lr) {
     const double width = (double)G * R;
     const double pad = width / std::max(lq, 1);
     const double step = (9.0 * R + 14.0) / (9.0 * R);
     const double fill = (lr + G - 1) / std::max(lr, 1.0);
     return pad * step * fill;
+}
+
+size_t cross_lds_bytes(const Variant &v, int ns);
+
+// Occupancy of a cross or stripe tile relative to the 4 waves per SIMD of R <= 32: R 36-44
+// run 3 waves, R 48 two, and a profile above 80 KiB leaves one workgroup per CU.  (Measured
+// on BLOSUM62 top-k, 1,000 x 100,000: padded-column rates 5-8 % below the R <= 32 tiles.)
+static double tile_occupancy_cost(const Variant &v, int ns) {
+    double f = v.R > 44 ? 1.08 : v.R > 32 ? 1.05 : 1.0;
+    if (cross_lds_bytes(v, ns) > 80 * 1024) f *= 1.05;
+    return f;
 }
 
 
@@ -440,7 +448,7 @@ const Variant *pick_variant(int lq, double mean_lr, bool packed, bool local, int
         if (v.G * v.R < lq) continue;
         if (for_pairs ? pairs_waves(v, ns, lds_limit) == 0 : cross_lds_bytes(v, ns) > lds_limit) continue;
         if (g_force_g && (v.G != g_force_g || v.R != g_force_r)) continue;
-        const double c = tile_cost(v.G, v.R, lq, mean_lr);
+        const double c = tile_cost(v.G, v.R, lq, mean_lr) * (for_pairs ? 1.0 : tile_occupancy_cost(v, ns));
         if (c < best_cost) {
             best_cost = c;
             best = &v;
@@ -494,12 +502,12 @@ const Variant *pick_cross_tile(int lq, double mean_lr, bool packed, bool local, 
         }
     }
     const Variant *best = pick_variant(lq, mean_lr, packed, local, ns, lds_limit, false);
-    double best_cost = best ? tile_cost(best->G, best->R, lq, mean_lr) : 1e30;
+    double best_cost = best ? tile_cost(best->G, best->R, lq, mean_lr) * tile_occupancy_cost(*best, ns) : 1e30;
     for (const Variant &v : variant_table()) {
         if (!v.stripe_fn || v.packed != packed || v.local != local || v.G * v.R >= lq) continue;
         if (cross_lds_bytes(v, ns) > lds_limit) continue;
         const int n = stripes_for(v, lq, track_last_col);
-        const double c = stripe_cost(v.G, v.R, lq, mean_lr, n);
+        const double c = stripe_cost(v.G, v.R, lq, mean_lr, n) * tile_occupancy_cost(v, ns);
         if (c < best_cost) {
             best_cost = c;
             best = &v;
@@ -768,7 +776,8 @@ void expand_cross_items_balanced(const CrossUnit *u0, const CrossUnit *u1, const
     const int64_t nref = rs.n_nonempty;
     const int64_t nu = (int64_t)(u1 - u0);
     const int64_t unit_rows = rs.bpos[nref];
-    const int64_t want = std::max<int64_t>(nu, (int64_t)per_wg * std::max(grid, 1));
+    const int64_t g = std::max(grid, 1);
+    const int64_t want = (std::max<int64_t>(nu, (int64_t)per_wg * g) + g - 1) / g * g;
     const int64_t max_pieces = std::max<int64_t>(1, unit_rows / std::max<int64_t>(min_rows, 1));
     const size_t first = items->size();
     for (int64_t i = 0; i < nu; ++i) {
@@ -804,23 +813,53 @@ int64_t stripe_item_rows(double mean_lr, int max_lr) {
 int64_t expand_stripe_items(const CrossUnit *u0, const CrossUnit *u1, const SeqSet &rs, int64_t max_rows,
                             std::vector<CrossItem> *items, int grid, int64_t min_rows) {
     const int64_t nref = rs.n_nonempty;
+    const int64_t nu = (int64_t)(u1 - u0);
+    const int64_t unit_rows = rs.bpos[nref];
     const size_t first = items->size();
     int64_t widest = 0;
-    // launches of few long queries: items down to min_rows so that there are about two
-    // per workgroup, rather than leaving most of the GPU idle
-    if (grid > 0 && min_rows < max_rows) {
-        const int64_t rows = rs.bpos[nref] * (int64_t)(u1 - u0);
-        max_rows = std::max<int64_t>(min_rows, std::min<int64_t>(max_rows, rows / (2 * (int64_t)grid)));
-    }
-    for (const CrossUnit *u = u0; u < u1; ++u) {
-        for (int64_t rb = 0; rb < nref;) {
-            int64_t re = rb + 1;
-            while (re < nref && rs.bpos[re + 1] - rs.bpos[rb] <= max_rows) ++re;
-            widest = std::max(widest, rs.bpos[re] - rs.bpos[rb]);
-            items->push_back(CrossItem{u->a.start, u->a.len, u->has_b ? u->b.start : 0, u->has_b ? u->b.len : -1,
-                                       u->a.out, u->has_b ? u->b.out : -1, (int32_t)rb, (int32_t)re, u->shift,
-                                       u->nstr});
-            rb = re;
+    auto push = [&](const CrossUnit *u, int64_t rb, int64_t re) {
+        widest = std::max(widest, rs.bpos[re] - rs.bpos[rb]);
+        items->push_back(CrossItem{u->a.start, u->a.len, u->has_b ? u->b.start : 0, u->has_b ? u->b.len : -1,
+                                   u->a.out, u->has_b ? u->b.out : -1, (int32_t)rb, (int32_t)re, u->shift,
+                                   u->nstr});
+    };
+    if (grid <= 0) {
+        // runs of at most max_rows rows (or one reference)
+        for (const CrossUnit *u = u0; u < u1; ++u) {
+            for (int64_t rb = 0; rb < nref;) {
+                int64_t re = rb + 1;
+                while (re < nref && rs.bpos[re + 1] - rs.bpos[rb] <= max_rows) ++re;
+                push(u, rb, re);
+                rb = re;
+            }
+        }
+    } else {
+        // a whole number of items per workgroup (at least two); unit i gets a share of them
+        // in proportion to its stripes, so items carry about equal rows x stripes, cut at
+        // equal row fractions, enough pieces for max_rows and pieces of at least min_rows
+        const int64_t pmin = (unit_rows + max_rows - 1) / std::max<int64_t>(max_rows, 1);
+        const int64_t pmax = std::max(pmin, unit_rows / std::max<int64_t>(min_rows, 1));
+        int64_t sum_str = 0;
+        for (const CrossUnit *u = u0; u < u1; ++u) sum_str += u->nstr;
+        int64_t want = std::max<int64_t>(2 * (int64_t)grid, nu * pmin);
+        want = (want + grid - 1) / grid * grid;
+        double carry = 0;   // running remainder of the proportional shares
+        for (int64_t i = 0; i < nu; ++i) {
+            const CrossUnit *u = u0 + i;
+            const double share = (double)want * u->nstr / (double)sum_str + carry;
+            int64_t quota = (int64_t)std::floor(share + 1e-9);
+            carry = share - (double)quota;
+            const int64_t pieces = std::min(pmax, std::max(pmin, std::max<int64_t>(quota, 1)));
+            int64_t rb = 0;
+            for (int64_t j = 1; j <= pieces && rb < nref; ++j) {
+                int64_t re = nref;
+                if (j < pieces) {   // first reference boundary at or past j/pieces of the rows
+                    const int64_t goal = unit_rows * j / pieces;
+                    re = std::lower_bound(rs.bpos.begin() + rb + 1, rs.bpos.begin() + nref, goal) - rs.bpos.begin();
+                }
+                push(u, rb, re);
+                rb = re;
+            }
         }
     }
     // most work (rows x stripes) first
@@ -1039,20 +1078,22 @@ int batch_cross(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
 // ---------------------------------------------------------------- pairs
 // Explicit pairs run in batches of PAIR_BATCH pairs.  Each batch is planned on the host
 // in O(n) (counting sorts by reference length rank, then by last-column register) into a
-// pinned buffer, and replayed through a HIP graph [copy header+tasks -> persistent pairs
-// kernel]; the header carries the task count and a zeroed work counter, so the graph
-// needs no memset node.  Two graph slots alternate so that planning batch b+1 overlaps
-// the device work of batch b.
+// pinned buffer of its slot, copied to the slot's device buffer on the copy stream, and
+// run by replaying the slot's HIP graph (the persistent pairs kernel with this call's
+// arguments) on the library stream once the copy is done.  The header carries the task
+// count and a zeroed work counter, so no memset is needed.  Slots alternate: planning
+// batch b+1 and copying its tasks overlap the kernel of batch b.
 constexpr int64_t PAIR_BATCH = 131072;
 constexpr size_t PAIR_HDR = 64;
 
 struct PairGraph {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    hipGraphNode_t copy = nullptr, kern = nullptr;
+    hipGraphNode_t kern = nullptr;
     char *d_buf = nullptr, *h_buf = nullptr;
     size_t cap = 0;
-    hipEvent_t done = nullptr;
+    hipEvent_t copied = nullptr;     // the copy has read h_buf and written d_buf
+    hipEvent_t consumed = nullptr;   // the kernel that read d_buf has finished
     int grid = 0, waves = 0;
     size_t lds = 0;
 };
@@ -1081,15 +1122,14 @@ static int pair_graph(DeviceCtx *ctx, const Variant &v, bool tab, int slot, int 
         GSW_HIP_TRY(hipMalloc(&pg->d_buf, pg->cap));
         GSW_HIP_TRY(hipHostMalloc(&pg->h_buf, pg->cap, hipHostMallocDefault));
         std::memset(pg->h_buf, 0, PAIR_HDR);
-        GSW_HIP_TRY(hipEventCreateWithFlags(&pg->done, hipEventDisableTiming));
+        GSW_HIP_TRY(hipEventCreateWithFlags(&pg->copied, hipEventDisableTiming));
+        GSW_HIP_TRY(hipEventCreateWithFlags(&pg->consumed, hipEventDisableTiming));
     }
     pg->lds = lds;
     pg->waves = waves;
     const void *fn = tab ? v.pairs_tab_fn : v.pairs_fn;
     GSW_TRY(occupancy_grid(ctx, fn, waves * 64, lds, &pg->grid));
     GSW_HIP_TRY(hipGraphCreate(&pg->graph, 0));
-    GSW_HIP_TRY(hipGraphAddMemcpyNode1D(&pg->copy, pg->graph, nullptr, 0, pg->d_buf, pg->h_buf, pg->cap,
-                                        hipMemcpyHostToDevice));
     KArgs a{};
     void *params[] = {&a};
     hipKernelNodeParams kp{};
@@ -1099,7 +1139,7 @@ static int pair_graph(DeviceCtx *ctx, const Variant &v, bool tab, int slot, int 
     kp.sharedMemBytes = (unsigned)lds;
     kp.kernelParams = params;
     kp.extra = nullptr;
-    GSW_HIP_TRY(hipGraphAddKernelNode(&pg->kern, pg->graph, &pg->copy, 1, &kp));
+    GSW_HIP_TRY(hipGraphAddKernelNode(&pg->kern, pg->graph, nullptr, 0, &kp));
     GSW_HIP_TRY(hipGraphInstantiate(&pg->exec, pg->graph, nullptr, nullptr, 0));
     *out = pg.get();
     return GSW_OK;
@@ -1356,7 +1396,7 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
                 cv.wait(lk, [&] { return state[prev] == 2 || abort; });
                 if (abort) return;
                 lk.unlock();
-                const hipError_t e = hipEventSynchronize(g.done);
+                const hipError_t e = hipEventSynchronize(g.copied);
                 if (e != hipSuccess) {
                     std::lock_guard<std::mutex> lk2(m);
                     abort = true;
@@ -1366,7 +1406,7 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
                     return;
                 }
             } else {
-                const hipError_t e = hipEventSynchronize(g.done);   // an earlier call's replay
+                const hipError_t e = hipEventSynchronize(g.copied);   // an earlier call's copy
                 if (e != hipSuccess) {
                     std::lock_guard<std::mutex> lk2(m);
                     abort = true;
@@ -1416,13 +1456,17 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
         }
         const Batch &B = batches[b];
         PairGraph &g = *slot_of.at({B.kind, B.bi % B.nslot});
-        hipError_t e = hipGraphExecMemcpyNodeSetParams1D(g.exec, g.copy, g.d_buf, g.h_buf,
-                                                         PAIR_HDR + sizeof(PairTask) * (size_t)ntasks[b],
-                                                         hipMemcpyHostToDevice);
+        // copy once the slot's previous kernel is done with d_buf; run once copied
+        hipError_t e = hipStreamWaitEvent(ctx->copy_stream, g.consumed, 0);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(g.d_buf, g.h_buf, PAIR_HDR + sizeof(PairTask) * (size_t)ntasks[b],
+                               hipMemcpyHostToDevice, ctx->copy_stream);
+        if (e == hipSuccess) e = hipEventRecord(g.copied, ctx->copy_stream);
+        if (e == hipSuccess) e = hipStreamWaitEvent(ctx->stream, g.copied, 0);
         if (e == hipSuccess) e = hipEventRecord(ctx->marks[2 * b], ctx->stream);
         if (e == hipSuccess) e = hipGraphLaunch(g.exec, ctx->stream);
-        if (e == hipSuccess) e = hipEventRecord(g.done, ctx->stream);
         if (e == hipSuccess) e = hipEventRecord(ctx->marks[2 * b + 1], ctx->stream);
+        if (e == hipSuccess) e = hipEventRecord(g.consumed, ctx->stream);
         if (e != hipSuccess) {
             rc = fail(GSW_ERR_HIP, "pairs batch launch: %s", hipGetErrorString(e));
             break;
@@ -1445,8 +1489,8 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
     GSW_HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     res->kernel_ms += ms;
     res->variant = dominant >= 0 ? dominant / 2 : -1;
-    // the dominant tile's batches: mean device time of a batch (the graph's task copy and
-    // kernel, between events recorded right before and after its launch) and mean cells
+    // the dominant tile's batches: mean device time of a batch's kernel (events recorded
+    // right before and after its graph launch, after the wait for its copy) and mean cells
     {
         double sum_ms = 0, sum_cells = 0;
         int cnt = 0;

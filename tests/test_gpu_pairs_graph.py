@@ -101,3 +101,23 @@ def test_dna_pairs_both_kernels(mode, tab, monkeypatch):
     assert al._b.last_stats.top_striped == (3 if tab == "1" else 2)
     want = native.score_encoded_pairs(al._queries, al._refs, qi, rj, scheme)
     assert np.array_equal(got.astype(np.int64), want)
+
+
+@pytest.mark.parametrize("n", [1000, 200_000])
+def test_pair_index_out_of_range_raises_index_error(n):
+    """The library checks pair indices while it plans (single- and multi-threaded) and the
+    host maps its error to IndexError, as indexing the sequence sets would."""
+    rng = np.random.default_rng(5)
+    refs = [rng.integers(0, 4, 50).astype(np.uint8) for _ in range(20)]
+    qs = [rng.integers(0, 4, 30).astype(np.uint8) for _ in range(10)]
+    al = Aligner(Scheme.preset("dna"), backend="hip").index(synth.pack(refs)).set_queries(synth.pack(qs))
+    qi = rng.integers(0, 10, n).astype(np.int32)
+    rj = rng.integers(0, 20, n).astype(np.int32)
+    assert al.score_pairs(qi, rj).shape == (n,)
+    for bad_q, bad_r in ((10, 0), (-1, 0), (0, 20), (0, -3)):
+        q2, r2 = qi.copy(), rj.copy()
+        q2[n // 2], r2[n // 2] = bad_q, bad_r
+        with pytest.raises(IndexError):
+            al.score_pairs(q2, r2)
+    # the device state is usable after the errors
+    assert np.array_equal(al.score_pairs(qi, rj), al.score_pairs(qi, rj))
