@@ -11,6 +11,7 @@ is the device top-k (``gsw_topk_rows``).  ``dt`` is device time from HIP events.
 from __future__ import annotations
 
 import ctypes
+import threading
 from contextlib import contextmanager
 
 import numpy as np
@@ -52,6 +53,20 @@ class HipBackend(Backend):
 
     name = "hip"
     history = None   # set to a list to collect every call's GswStats (benchmarks)
+    _scratch = threading.local()   # per thread and device: the pair scores' device buffer
+
+    def _pair_out(self, n: int) -> _hip.DeviceBuffer:
+        """A grow-only device buffer of at least ``n`` int32 for pair scores (one per
+        thread and device, so concurrent callers never share one)."""
+        bufs = getattr(self._scratch, "bufs", None)
+        if bufs is None:
+            bufs = self._scratch.bufs = {}
+        dev = _hip.get_device()
+        buf = bufs.get(dev)
+        if buf is None or buf.nbytes < 4 * n:
+            bufs[dev] = None   # free the old buffer first
+            buf = bufs[dev] = _hip.DeviceBuffer(4 * max(n, 1), np.int32)
+        return buf
 
     def available(self) -> bool:
         return _hip.gpu_available()
@@ -94,14 +109,17 @@ class HipBackend(Backend):
         qi = np.ascontiguousarray(qi_host, dtype=np.int32)
         rj = np.ascontiguousarray(rj_host, dtype=np.int32)
         # indices are checked by the library while it plans (IndexError), not in extra passes
-        out = _hip.DeviceBuffer(4 * n, np.int32, (n,))
+        buf = self._pair_out(n)
         s, table = scheme_struct_cached(scheme)
         stats = _hip.GswStats()
         _hip.call("gsw_score_pairs", ctypes.byref(s), _policy(scheme), qset.handle, rset.handle,
-                  _hip.ptr(qi), _hip.ptr(rj), int(n), out.ptr, ctypes.byref(stats))
+                  _hip.ptr(qi), _hip.ptr(rj), int(n), buf.ptr, ctypes.byref(stats))
         self._note(stats)
         self.last_cells = int(stats.cells)   # counted by the library while planning
-        return out.to_host(), stats.kernel_ms / 1e3, _describe("pairs", scheme, stats, dtype)
+        host = np.empty(n, dtype=np.int32)
+        if n:
+            _hip.call("gsw_memcpy_d2h", _hip.ptr(host), buf.ptr, 4 * n)
+        return host, stats.kernel_ms / 1e3, _describe("pairs", scheme, stats, dtype)
 
     def as_host(self, scores_native):
         return scores_native.to_host().astype(np.int32, copy=False)

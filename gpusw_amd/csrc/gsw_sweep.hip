@@ -13,6 +13,7 @@
 #include <tuple>
 #include <atomic>
 #include <condition_variable>
+#include <functional>
 #include <thread>
 #include <string>
 
@@ -1156,12 +1157,82 @@ static int host_threads() {
     return t;
 }
 
+// Host threads for planning, started on first use and kept: starting and joining 7
+// threads costs 0.2-0.4 ms, several times per call.  run(T, fn) runs fn(1..T-1) on pool
+// threads and fn(0) on the caller, and returns when all have returned.  One run at a time;
+// a caller that finds the pool busy starts threads of its own instead.
+class HostPool {
+  public:
+    template <typename F> void run(int T, F &&fn) {
+        std::unique_lock<std::mutex> call(call_mu_, std::try_to_lock);
+        if (!call.owns_lock()) {
+            std::vector<std::thread> th;
+            for (int i = 1; i < T; ++i) th.emplace_back([&fn, i] { fn(i); });
+            fn(0);
+            for (auto &x : th) x.join();
+            return;
+        }
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            while ((int)threads_.size() < T - 1) {
+                const int w = (int)threads_.size() + 1;
+                threads_.emplace_back([this, w] { loop(w); });
+            }
+            job_ = [&fn](int i) { fn(i); };
+            njobs_ = T;
+            remaining_ = T - 1;
+            ++gen_;
+        }
+        cv_.notify_all();
+        fn(0);
+        std::unique_lock<std::mutex> lk(mu_);
+        done_.wait(lk, [&] { return remaining_ == 0; });
+        job_ = nullptr;
+    }
+    ~HostPool() {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            quit_ = true;
+        }
+        cv_.notify_all();
+        for (auto &t : threads_) t.join();
+    }
+
+  private:
+    void loop(int w) {
+        uint64_t seen = 0;
+        for (;;) {
+            std::function<void(int)> f;
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [&] { return quit_ || gen_ != seen; });
+                if (quit_) return;
+                seen = gen_;
+                if (w >= njobs_) continue;
+                f = job_;
+            }
+            f(w);
+            std::lock_guard<std::mutex> lk(mu_);
+            if (--remaining_ == 0) done_.notify_all();
+        }
+    }
+    std::mutex call_mu_, mu_;
+    std::condition_variable cv_, done_;
+    std::vector<std::thread> threads_;
+    std::function<void(int)> job_;
+    int njobs_ = 0, remaining_ = 0;
+    uint64_t gen_ = 0;
+    bool quit_ = false;
+};
+
 // fn(tid) on T threads (tid 0 on the calling thread).
 template <typename F> static void run_threads(int T, F &&fn) {
-    std::vector<std::thread> th;
-    for (int i = 1; i < T; ++i) th.emplace_back(fn, i);
-    fn(0);
-    for (auto &x : th) x.join();
+    if (T <= 1) {
+        fn(0);
+        return;
+    }
+    static HostPool pool;
+    pool.run(T, fn);
 }
 
 constexpr int MAX_PAIR_SLOTS = 8;   // pinned buffers + graphs per tile
@@ -1170,6 +1241,11 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
                 const SeqSet &rs, const int32_t *h_qi, const int32_t *h_rj, int64_t n,
                 int32_t *d_out, BatchResult *res) {
     if (n == 0) return GSW_OK;
+    // GSW_TRACE_LAUNCHES=1: host phases of the call to stderr
+    static const bool trace = [] { const char *e = std::getenv("GSW_TRACE_LAUNCHES"); return e && *e == '1'; }();
+    using clk = std::chrono::steady_clock;
+    const clk::time_point t_in = clk::now();
+    clk::time_point t_members, t_first;
     const bool local = s.mode == GSW_MODE_LOCAL;
     const bool track_last_col = s.mode == GSW_MODE_SEMIGLOBAL && s.free_q;
     const int wmax = max_width();
@@ -1188,7 +1264,22 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
         int bad_code = GSW_OK;
     };
     std::vector<Part> parts(T);
-    std::vector<int16_t> vid(n, -1);
+    // per-pair tile and tile members: kept between calls (grow-only), so that a call does
+    // not fault in fresh pages for them
+    // (local references: the planning threads must use the calling thread's vectors)
+    static thread_local std::vector<int16_t> tl_vid;
+    static thread_local std::vector<int64_t> tl_members;
+    std::vector<int16_t> &vid = tl_vid;
+    std::vector<int64_t> &members = tl_members;
+    struct Trim {   // calls of more than 2^24 pairs give their buffers back
+        std::vector<int16_t> &a;
+        std::vector<int64_t> &b;
+        ~Trim() {
+            if (a.size() > ((size_t)1 << 24)) std::vector<int16_t>().swap(a);
+            if (b.size() > ((size_t)1 << 24)) std::vector<int64_t>().swap(b);
+        }
+    } trim{vid, members};
+    if (vid.size() < (size_t)n) vid.resize((size_t)n);
     run_threads(T, [&](int tid) {
         Part &P = parts[tid];
         P.count.assign(nv, 0);
@@ -1197,6 +1288,7 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
         const int64_t k0 = n * tid / T, k1 = n * (tid + 1) / T;
         for (int64_t k = k0; k < k1; ++k) {
             const int q = h_qi[k], r = h_rj[k];
+            vid[k] = -1;
             if (q < 0 || q >= qs.n || r < 0 || r >= rs.n) {
                 P.bad = k;
                 P.bad_code = GSW_ERR_INVALID;
@@ -1266,13 +1358,15 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
         }
         vbeg[v + 1] = p;
     }
-    std::vector<int64_t> members((size_t)vbeg[nv]);
+    if (members.size() < (size_t)vbeg[nv]) members.resize((size_t)vbeg[nv]);
     run_threads(T, [&](int tid) {
         const int64_t k0 = n * tid / T, k1 = n * (tid + 1) / T;
         std::vector<int64_t> &ps = pos[tid];
         for (int64_t k = k0; k < k1; ++k)
             if (vid[k] >= 0) members[ps[vid[k]]++] = k;
     });
+
+    t_members = clk::now();
 
     // -- batches of PAIR_BATCH members of one tile; a tile's batches cycle through its slots
     struct Batch { const Variant *v; bool tab; int kind; int64_t b0, b1; int bi, nslot; };
@@ -1284,11 +1378,18 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
         if (cnt_v == 0) continue;
         if (cnt_v > dominant_n) { dominant_n = cnt_v; dominant = (int)kind; }
         const Variant &v = variant_table()[kind / 2];
-        const int nb = (int)((cnt_v + PAIR_BATCH - 1) / PAIR_BATCH);
-        for (int bi = 0; bi < nb; ++bi)
-            batches.push_back(Batch{&v, (kind & 1) != 0, (int)kind, vbeg[kind] + bi * PAIR_BATCH,
-                                    std::min(vbeg[kind + 1], vbeg[kind] + (bi + 1) * PAIR_BATCH), bi,
-                                    std::min(nb, MAX_PAIR_SLOTS)});
+        // the call's first batches ramp up (1/8, 1/4, 1/2 of a batch), so that the device
+        // starts after a short plan and later plans overlap its work
+        const bool ramp = batches.empty();
+        const size_t first = batches.size();
+        int bi = 0;
+        for (int64_t b0 = vbeg[kind]; b0 < vbeg[kind + 1]; ++bi) {
+            const int64_t size = ramp && bi < 3 ? (PAIR_BATCH >> (3 - bi)) : PAIR_BATCH;
+            const int64_t b1 = std::min(vbeg[kind + 1], b0 + size);
+            batches.push_back(Batch{&v, (kind & 1) != 0, (int)kind, b0, b1, bi, 0});
+            b0 = b1;
+        }
+        for (size_t i = first; i < batches.size(); ++i) batches[i].nslot = std::min(bi, MAX_PAIR_SLOTS);
     }
 
     SatBuffers sb;
@@ -1424,78 +1525,90 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
         }
     };
     const int W = (int)std::min<size_t>((size_t)T, nb);
-    std::vector<std::thread> workers;
-    for (int i = 0; i < W; ++i) workers.emplace_back(worker);
-    auto stop = [&]() {
-        {
-            std::lock_guard<std::mutex> lk(m);
-            abort = true;
-        }
-        cv.notify_all();
-        for (auto &x : workers) x.join();
-        workers.clear();
-    };
     while (ctx->marks.size() < 2 * nb) {   // timing events before and after each batch
         hipEvent_t e;
-        if (hipEventCreate(&e) != hipSuccess) {
-            stop();
-            return fail(GSW_ERR_HIP, "event create failed");
-        }
+        GSW_HIP_TRY(hipEventCreate(&e));
         ctx->marks.push_back(e);
     }
     int rc = GSW_OK;
-    if (hipEventRecord(ctx->ev0, ctx->stream) != hipSuccess) rc = fail(GSW_ERR_HIP, "event record failed");
-    for (size_t b = 0; b < nb && rc == GSW_OK; ++b) {
-        {
-            std::unique_lock<std::mutex> lk(m);
-            cv.wait(lk, [&] { return state[b] >= 1 || abort; });
-            if (abort) {
-                rc = fail(werr != GSW_OK ? werr : GSW_ERR_HIP, "pairs planning failed: %s", werr_msg.c_str());
+    auto launcher = [&]() {
+        if (hipEventRecord(ctx->ev0, ctx->stream) != hipSuccess) rc = fail(GSW_ERR_HIP, "event record failed");
+        for (size_t b = 0; b < nb && rc == GSW_OK; ++b) {
+            {
+                std::unique_lock<std::mutex> lk(m);
+                cv.wait(lk, [&] { return state[b] >= 1 || abort; });
+                if (abort) {
+                    rc = fail(werr != GSW_OK ? werr : GSW_ERR_HIP, "pairs planning failed: %s", werr_msg.c_str());
+                    break;
+                }
+            }
+            const Batch &B = batches[b];
+            PairGraph &g = *slot_of.at({B.kind, B.bi % B.nslot});
+            // copy once the slot's previous kernel is done with d_buf; run once copied
+            hipError_t e = hipStreamWaitEvent(ctx->copy_stream, g.consumed, 0);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(g.d_buf, g.h_buf, PAIR_HDR + sizeof(PairTask) * (size_t)ntasks[b],
+                                   hipMemcpyHostToDevice, ctx->copy_stream);
+            if (e == hipSuccess) e = hipEventRecord(g.copied, ctx->copy_stream);
+            if (e == hipSuccess) e = hipStreamWaitEvent(ctx->stream, g.copied, 0);
+            if (e == hipSuccess) e = hipEventRecord(ctx->marks[2 * b], ctx->stream);
+            if (e == hipSuccess) e = hipGraphLaunch(g.exec, ctx->stream);
+            if (e == hipSuccess) e = hipEventRecord(ctx->marks[2 * b + 1], ctx->stream);
+            if (e == hipSuccess) e = hipEventRecord(g.consumed, ctx->stream);
+            if (e != hipSuccess) {
+                rc = fail(GSW_ERR_HIP, "pairs batch launch: %s", hipGetErrorString(e));
                 break;
             }
+            res->launches++;
+            if (b == 0) t_first = clk::now();
+            {
+                std::lock_guard<std::mutex> lk(m);
+                state[b] = 2;
+            }
+            cv.notify_all();
         }
-        const Batch &B = batches[b];
-        PairGraph &g = *slot_of.at({B.kind, B.bi % B.nslot});
-        // copy once the slot's previous kernel is done with d_buf; run once copied
-        hipError_t e = hipStreamWaitEvent(ctx->copy_stream, g.consumed, 0);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(g.d_buf, g.h_buf, PAIR_HDR + sizeof(PairTask) * (size_t)ntasks[b],
-                               hipMemcpyHostToDevice, ctx->copy_stream);
-        if (e == hipSuccess) e = hipEventRecord(g.copied, ctx->copy_stream);
-        if (e == hipSuccess) e = hipStreamWaitEvent(ctx->stream, g.copied, 0);
-        if (e == hipSuccess) e = hipEventRecord(ctx->marks[2 * b], ctx->stream);
-        if (e == hipSuccess) e = hipGraphLaunch(g.exec, ctx->stream);
-        if (e == hipSuccess) e = hipEventRecord(ctx->marks[2 * b + 1], ctx->stream);
-        if (e == hipSuccess) e = hipEventRecord(g.consumed, ctx->stream);
-        if (e != hipSuccess) {
-            rc = fail(GSW_ERR_HIP, "pairs batch launch: %s", hipGetErrorString(e));
-            break;
+        if (rc != GSW_OK) {   // release planners waiting for a launch
+            {
+                std::lock_guard<std::mutex> lk(m);
+                abort = true;
+            }
+            cv.notify_all();
         }
-        res->launches++;
-        {
-            std::lock_guard<std::mutex> lk(m);
-            state[b] = 2;
-        }
-        cv.notify_all();
-    }
-    if (rc != GSW_OK) {
-        stop();
-        return rc;
-    }
-    for (auto &x : workers) x.join();
+    };
+    // the launcher on the calling thread, W planners beside it
+    run_threads(W + 1, [&](int tid) {
+        if (tid == 0)
+            launcher();
+        else
+            worker();
+    });
+    if (rc != GSW_OK) return rc;
+    const clk::time_point t_launched = clk::now();
     GSW_HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
     GSW_HIP_TRY(hipEventSynchronize(ctx->ev1));
     float ms = 0.f;
     GSW_HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     res->kernel_ms += ms;
+    if (trace) {
+        auto d = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
+        std::fprintf(stderr, "[gsw] pairs n=%lld batches=%zu: classify+members %.2f ms, first launch +%.2f ms, "
+                             "all launched +%.2f ms, device done +%.2f ms (device %.2f ms)\n",
+                     (long long)n, nb, d(t_in, t_members), d(t_members, t_first), d(t_first, t_launched),
+                     d(t_launched, clk::now()), ms);
+    }
     res->variant = dominant >= 0 ? dominant / 2 : -1;
-    // the dominant tile's batches: mean device time of a batch's kernel (events recorded
-    // right before and after its graph launch, after the wait for its copy) and mean cells
+    // the dominant tile's full batches (all of its batches if none is full): mean device
+    // time of a batch's kernel (events recorded right before and after its graph launch,
+    // after the wait for its copy) and mean cells
     {
+        bool any_full = false;
+        for (size_t b = 0; b < nb; ++b)
+            any_full |= batches[b].kind == dominant && batches[b].b1 - batches[b].b0 == PAIR_BATCH;
         double sum_ms = 0, sum_cells = 0;
         int cnt = 0;
         for (size_t b = 0; b < nb; ++b) {
             if (batches[b].kind != dominant) continue;
+            if (any_full && batches[b].b1 - batches[b].b0 != PAIR_BATCH) continue;
             float bms = 0.f;
             GSW_HIP_TRY(hipEventElapsedTime(&bms, ctx->marks[2 * b], ctx->marks[2 * b + 1]));
             sum_ms += bms;
