@@ -38,7 +38,9 @@ struct DeviceCtx {
     int num_cus = 0;
     int max_smem = 0;
     hipStream_t stream = nullptr;
-    hipStream_t copy_stream = nullptr;   // host-to-device task copies that overlap kernels
+    // second stream (default priority): pair task copies that overlap kernels, and every
+    // other cross launch so that consecutive launches overlap at their ends
+    hipStream_t stream2 = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // grow-only scratch buffers owned by the library
     void *plain_ws = nullptr;  size_t plain_ws_bytes = 0;

@@ -66,7 +66,7 @@ int ctx_get(DeviceCtx **out) {
     ctx->max_smem = (int)prop.sharedMemPerBlock;
     // Default-priority stream (non-default priorities are avoided on purpose).
     GSW_HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
-    GSW_HIP_TRY(hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking));
+    GSW_HIP_TRY(hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking));
     GSW_HIP_TRY(hipEventCreate(&ctx->ev0));
     GSW_HIP_TRY(hipEventCreate(&ctx->ev1));
     GSW_HIP_TRY(hipMalloc(&ctx->counters, 64 * sizeof(int32_t)));

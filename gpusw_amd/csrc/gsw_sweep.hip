@@ -955,25 +955,35 @@ int batch_cross(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
         groups.push_back(g);
         u0 = u1;
     }
-    if (bnd_entries > 0) GSW_TRY(ensure_buffer(&ctx->bnd, &ctx->bnd_bytes, 2 * sizeof(uint32_t) * (size_t)bnd_entries));
+    // two boundary regions: consecutive striped launches may run at the same time
+    if (bnd_entries > 0) GSW_TRY(ensure_buffer(&ctx->bnd, &ctx->bnd_bytes, 4 * sizeof(uint32_t) * (size_t)bnd_entries));
     if (groups.size() > 60) return fail(GSW_ERR_LIMIT, "too many tile groups (%zu)", groups.size());
-    // every launch is timed with an event after it (the longest one is reported);
-    // GSW_TRACE_LAUNCHES=1 also prints tile, cells and GCUPS of each launch to stderr
+    // Launches alternate between the library stream and the second stream, so each one's
+    // workgroups take the CUs as the previous launch's last workgroups finish instead of
+    // waiting for all of them.  Every launch is timed by events right before and after it
+    // on its stream (the longest is reported; the interval can include the end of the
+    // previous launch, which it shares the GPU with).  GSW_TRACE_LAUNCHES=1 also prints
+    // tile, cells and GCUPS of each launch to stderr.
     static const bool trace = [] { const char *e = std::getenv("GSW_TRACE_LAUNCHES"); return e && *e == '1'; }();
-    while (ctx->marks.size() < groups.size()) {
+    while (ctx->marks.size() < 2 * groups.size() + 2) {
         hipEvent_t e;
         GSW_HIP_TRY(hipEventCreate(&e));
         ctx->marks.push_back(e);
     }
+    hipEvent_t fork = ctx->marks[2 * groups.size()], join = ctx->marks[2 * groups.size() + 1];
+    const hipStream_t streams[2] = {ctx->stream, ctx->stream2};
     GSW_HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
     if (!groups.empty()) {
         GSW_TRY(ensure_buffer(&ctx->meta2, &ctx->meta2_bytes, sizeof(CrossItem) * items.size()));
         GSW_HIP_TRY(hipMemcpyAsync(ctx->meta2, items.data(), sizeof(CrossItem) * items.size(),
                                    hipMemcpyHostToDevice, ctx->stream));
         GSW_HIP_TRY(hipMemsetAsync(ctx->counters + 2, 0, sizeof(int32_t) * groups.size(), ctx->stream));
+        GSW_HIP_TRY(hipEventRecord(fork, ctx->stream));
+        GSW_HIP_TRY(hipStreamWaitEvent(ctx->stream2, fork, 0));
     }
     for (size_t gi = 0; gi < groups.size(); ++gi) {
         const Group &g = groups[gi];
+        const hipStream_t st = streams[gi & 1];
         const Variant &v = *units[g.u0].v;
         KArgs a = base_args(s, qs, rs, d_out);
         a.ldo = nr;
@@ -985,27 +995,37 @@ int batch_cross(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
         a.nitems = (int32_t)(g.i1 - g.i0);
         a.counter = ctx->counters + 2 + gi;
         const int grid = std::min<int>(g.grid, a.nitems);
+        GSW_HIP_TRY(hipEventRecord(ctx->marks[2 * gi], st));
         if (g.striped) {
-            a.bnd = static_cast<uint32_t *>(ctx->bnd);
+            // region gi & 1: the launches of one stream run one after the other
+            a.bnd = static_cast<uint32_t *>(ctx->bnd) + (gi & 1) * 2 * (size_t)bnd_entries;
             a.bnd_stride = bnd_entries / grid;
             a.bpos = rs.d_bpos;
-            v.stripe(a, grid, g.lds, ctx->stream);
+            v.stripe(a, grid, g.lds, st);
         } else {
-            v.cross(a, grid, g.lds, ctx->stream);
+            v.cross(a, grid, g.lds, st);
         }
         GSW_HIP_TRY(hipGetLastError());
         res->launches++;
-        GSW_HIP_TRY(hipEventRecord(ctx->marks[gi], ctx->stream));
+        GSW_HIP_TRY(hipEventRecord(ctx->marks[2 * gi + 1], st));
+    }
+    if (groups.size() > 1) {   // the library stream waits for the second one
+        GSW_HIP_TRY(hipEventRecord(join, ctx->stream2));
+        GSW_HIP_TRY(hipStreamWaitEvent(ctx->stream, join, 0));
     }
     GSW_HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
     GSW_HIP_TRY(hipEventSynchronize(ctx->ev1));
     float ms = 0.f;
     GSW_HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    // A launch's time runs from its start event to its end event on its stream; when the
+    // other stream's launches run beside it, it includes the time it shared the GPU (or
+    // waited for CUs).  The launch with the most cells is reported.
+    int64_t top_cells = -1;
     for (size_t gi = 0; gi < groups.size(); ++gi) {
         const Group &g = groups[gi];
         const Variant &v = *units[g.u0].v;
         float gms = 0.f;
-        GSW_HIP_TRY(hipEventElapsedTime(&gms, gi ? ctx->marks[gi - 1] : ctx->ev0, ctx->marks[gi]));
+        GSW_HIP_TRY(hipEventElapsedTime(&gms, ctx->marks[2 * gi], ctx->marks[2 * gi + 1]));
         int64_t qcols = 0;
         double width = 0;
         for (size_t u = g.u0; u < g.u1; ++u) {
@@ -1013,7 +1033,8 @@ int batch_cross(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
             width += (double)v.G * v.R * units[u].nstr * (v.packed ? 2 : 1) * (double)rs.total;
         }
         const int64_t cells = qcols * rs.total;
-        if (gms > res->top_ms) {
+        if (cells > top_cells && gms > 0.f) {
+            top_cells = cells;
             res->top_ms = gms;
             res->top_cells = cells;
             res->top_variant = v.id;
@@ -1079,7 +1100,7 @@ int batch_cross(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
 // ---------------------------------------------------------------- pairs
 // Explicit pairs run in batches of PAIR_BATCH pairs.  Each batch is planned on the host
 // in O(n) (counting sorts by reference length rank, then by last-column register) into a
-// pinned buffer of its slot, copied to the slot's device buffer on the copy stream, and
+// pinned buffer of its slot, copied to the slot's device buffer on the second stream, and
 // run by replaying the slot's HIP graph (the persistent pairs kernel with this call's
 // arguments) on the library stream once the copy is done.  The header carries the task
 // count and a zeroed work counter, so no memset is needed.  Slots alternate: planning
@@ -1330,6 +1351,7 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
             P.count[2 * v->id + (tab ? 1 : 0)]++;
         }
     });
+    const clk::time_point t_classified = clk::now();
     for (const Part &P : parts)   // ranges are in index order: the first error is the lowest index
         if (P.bad >= 0)
             return fail(P.bad_code, P.bad_code == GSW_ERR_INVALID ? "pair %lld index out of range"
@@ -1545,11 +1567,11 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
             const Batch &B = batches[b];
             PairGraph &g = *slot_of.at({B.kind, B.bi % B.nslot});
             // copy once the slot's previous kernel is done with d_buf; run once copied
-            hipError_t e = hipStreamWaitEvent(ctx->copy_stream, g.consumed, 0);
+            hipError_t e = hipStreamWaitEvent(ctx->stream2, g.consumed, 0);
             if (e == hipSuccess)
                 e = hipMemcpyAsync(g.d_buf, g.h_buf, PAIR_HDR + sizeof(PairTask) * (size_t)ntasks[b],
-                                   hipMemcpyHostToDevice, ctx->copy_stream);
-            if (e == hipSuccess) e = hipEventRecord(g.copied, ctx->copy_stream);
+                                   hipMemcpyHostToDevice, ctx->stream2);
+            if (e == hipSuccess) e = hipEventRecord(g.copied, ctx->stream2);
             if (e == hipSuccess) e = hipStreamWaitEvent(ctx->stream, g.copied, 0);
             if (e == hipSuccess) e = hipEventRecord(ctx->marks[2 * b], ctx->stream);
             if (e == hipSuccess) e = hipGraphLaunch(g.exec, ctx->stream);
@@ -1591,10 +1613,10 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
     res->kernel_ms += ms;
     if (trace) {
         auto d = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
-        std::fprintf(stderr, "[gsw] pairs n=%lld batches=%zu: classify+members %.2f ms, first launch +%.2f ms, "
-                             "all launched +%.2f ms, device done +%.2f ms (device %.2f ms)\n",
-                     (long long)n, nb, d(t_in, t_members), d(t_members, t_first), d(t_first, t_launched),
-                     d(t_launched, clk::now()), ms);
+        std::fprintf(stderr, "[gsw] pairs n=%lld batches=%zu: classify %.2f ms, members +%.2f ms, first launch "
+                             "+%.2f ms, all launched +%.2f ms, device done +%.2f ms (device %.2f ms)\n",
+                     (long long)n, nb, d(t_in, t_classified), d(t_classified, t_members), d(t_members, t_first),
+                     d(t_first, t_launched), d(t_launched, clk::now()), ms);
     }
     res->variant = dominant >= 0 ? dominant / 2 : -1;
     // the dominant tile's full batches (all of its batches if none is full): mean device

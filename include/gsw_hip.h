@@ -67,8 +67,11 @@ typedef struct gsw_stats {
     int64_t plain_pairs;  /* pairs routed to the one-thread-per-pair kernel      */
     int32_t launches;     /* kernel launches issued                              */
     int32_t variant;      /* dominant batched variant id (-1: none)              */
-    double  top_ms;       /* cross: device time of the longest batched launch;   */
-                          /* pairs: mean time of the dominant tile's batches     */
+    double  top_ms;       /* cross: device time of the batched launch with the   */
+                          /*   most cells (events before and after it on its    */
+                          /*   stream; includes time shared with launches of    */
+                          /*   the other stream);                               */
+                          /* pairs: mean time of the dominant tile's full batches */
     int64_t top_cells;    /*   cells of that launch (pairs: mean per batch)      */
     int32_t top_variant;  /*   its variant id (-1: none)                         */
     int32_t top_striped;  /*   kernel: 0 cross, 1 stripe, 2 pairs, 3 pairs table */
