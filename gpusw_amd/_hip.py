@@ -9,6 +9,7 @@ from __future__ import annotations
 
 import ctypes
 import os
+import re
 import threading
 from pathlib import Path
 
@@ -152,7 +153,7 @@ def check(rc: int) -> None:
         raise NoGpuError(msg)
     if rc == GSW_ERR_INVALID:
         # an index past the end of a sequence set, as numpy indexing would report it
-        raise (IndexError if "index out of range" in msg else ValueError)(msg)
+        raise (IndexError if re.search(r"\bindex\b.*\bout of range\b", msg) else ValueError)(msg)
     if rc == GSW_ERR_OVERFLOW:
         raise OverflowRiskError(msg)
     if rc == GSW_ERR_TIMEOUT:

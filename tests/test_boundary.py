@@ -44,6 +44,34 @@ def test_status_mapping():
         _hip.check(_hip.GSW_ERR_HIP)
 
 
+def test_index_errors_map_to_index_error(monkeypatch):
+    """The library's 'index out of range' failures surface as IndexError (what numpy
+    indexing raises); other invalid-argument failures stay ValueError."""
+    _hip.load()
+    monkeypatch.setattr(_hip, "last_error", lambda: "pair 12 index out of range")
+    with pytest.raises(IndexError):
+        _hip.check(_hip.GSW_ERR_INVALID)
+    monkeypatch.setattr(_hip, "last_error", lambda: "query index 7 out of range")
+    with pytest.raises(IndexError):
+        _hip.check(_hip.GSW_ERR_INVALID)
+    monkeypatch.setattr(_hip, "last_error", lambda: "device 3 out of range (have 1)")
+    with pytest.raises(ValueError) as ei:
+        _hip.check(_hip.GSW_ERR_INVALID)
+    assert not isinstance(ei.value, IndexError)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    import subprocess
+    import sys
+    from pathlib import Path
+
+    root = Path(__file__).resolve().parents[1]
+    proc = subprocess.run([sys.executable, str(root / "bench.py"), "--gpus", "2"], capture_output=True,
+                          text=True, timeout=120)
+    assert proc.returncode != 0
+    assert "one GPU" in (proc.stdout + proc.stderr)
+
+
 def test_unknown_backend_name():
     with pytest.raises(ValueError):
         resolve_backend("cuda")
