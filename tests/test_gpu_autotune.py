@@ -43,3 +43,26 @@ def test_every_forced_tile_exact(mode):
         finally:
             _hip.force_variant(0, 0)
         assert np.array_equal(got.astype(np.int64), want), (G, R)
+
+
+@pytest.mark.parametrize("mode", ["local", "global", "semiglobal"])
+@pytest.mark.parametrize("tab", ["1", "0"])
+def test_every_forced_tile_exact_pairs(mode, tab, monkeypatch):
+    """Explicit pairs on every compiled tile, with the table kernel (default) and with
+    per-lane profiles (GSW_PAIRS_TAB=0, where the profiles fit LDS)."""
+    monkeypatch.setenv("GSW_PAIRS_TAB", tab)
+    rng = np.random.default_rng(29)
+    scheme = Scheme(mode=mode)
+    refs = rand_seqs(rng, 60, 1, 300, "ACGT")
+    queries = rand_seqs(rng, 50, 1, 64, "ACGT")
+    qi = rng.integers(0, len(queries), 3000).astype(np.int32)
+    rj = rng.integers(0, len(refs), 3000).astype(np.int32)
+    want = native.score_encoded_pairs(funnel(queries, scheme), funnel(refs, scheme), qi, rj, scheme)
+    for G, R in _hip.list_variants():
+        _hip.force_variant(G, R)
+        try:
+            al = Aligner(scheme, backend="hip").index(refs).set_queries(queries)
+            got = al.score_pairs(qi, rj)
+        finally:
+            _hip.force_variant(0, 0)
+        assert np.array_equal(got.astype(np.int64), want), (G, R)
