@@ -146,8 +146,10 @@ struct SweepOut {
 //                padding columns whose profile is 0 carry exactly H = 0 and E = gap_open,
 //                the column-0 boundary, so shifting the query by (R - lq mod R) mod R puts
 //                its last column on the last register of a lane.
-// STRIPE: the group covers query columns [cbase, cbase + G*R) of a longer query.  bh[i]
-// and be[i] hold H and E of row i+1 at the column left of the stripe when bnd_in (else
+// STRIPE: the group covers query columns [cbase, cbase + G*R) of a longer query.  bh and be
+// are the workgroup's (wave-uniform) boundary planes and boff the group's first row in
+// them, so an access is a uniform base plus a 32-bit offset.  bh[boff+i]
+// and be[boff+i] hold H and E of row i+1 at the column left of the stripe when bnd_in (else
 // the column-0 boundary applies), and receive those of the stripe's last column when
 // bnd_out; the
 // row is read one step before lane 0 needs it and written G-1 steps after, so one buffer
@@ -165,7 +167,7 @@ __device__ __forceinline__ SweepOut sweep_group(const uint8_t *__restrict__ rseq
                                                 int lqa, int lqb, int cbase, uint32_t *bh = nullptr,
                                                 uint32_t *be = nullptr, bool bnd_in = false,
                                                 bool bnd_out = false, const int *toff = nullptr,
-                                                int trow = 0) {
+                                                int trow = 0, uint32_t boff = 0) {
     constexpr bool PF = PF_ || STRIPE;   // stripes: no code-load wait right after the boundary store
     using O = VOps<V>;
     using L = ProfLayout<G, R>;
@@ -193,9 +195,13 @@ __device__ __forceinline__ SweepOut sweep_group(const uint8_t *__restrict__ rseq
     V din = LOCAL ? ZERO : O::splat(row0_value(k, cbase));   // H(0, left column), lane 0
     V ein = LOCAL ? ZERO : O::splat(O::NEG);
     V eout = ein;
+    // boundary word i of a plane: uniform base + 32-bit byte offset (SGPR-base addressing)
+    auto bnd = [](uint32_t *plane, uint32_t i) -> uint32_t * {
+        return reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(plane) + (uint32_t)(i * 4u));
+    };
     if (STRIPE && bnd_in && t == 0 && lr > 0) {
-        hin = O::from_bits((int)bh[0]);
-        ein = O::from_bits((int)be[0]);
+        hin = O::from_bits((int)*bnd(bh, boff));
+        ein = O::from_bits((int)*bnd(be, boff));
     }
     V best = ZERO;
     V endcol = O::splat(O::NEG);
@@ -225,11 +231,13 @@ __device__ __forceinline__ SweepOut sweep_group(const uint8_t *__restrict__ rseq
     V c0v = ladder ? O::splat(col0_value(k, 2)) : ZERO;
     const int rlast = max(lr - 1, 0);   // idle groups (lr = 0) read their row 0 harmlessly
     int c_raw = PF ? (int)rseq[min(max(-t, 0), rlast)] : 0;
+    // lane 0's boundary words of its next row (stale when there is none: lane 0 is idle then)
+    uint2 nb{0u, 0u};
 
     for (int s = 0; s < nsteps; ++s) {
         const int row = s - t;
-        uint2 nb{0u, 0u};
-        if (STRIPE && bnd_in && t == 0 && row + 1 < lr) nb = uint2{bh[row + 1], be[row + 1]};   // lane 0's next row
+        if (STRIPE && bnd_in && t == 0 && row + 1 < lr)   // lane 0's next row
+            nb = uint2{*bnd(bh, boff + (uint32_t)(row + 1)), *bnd(be, boff + (uint32_t)(row + 1))};
         int c_nxt = 0;
         if constexpr (PF) c_nxt = (int)rseq[min(max(row + 1, 0), rlast)];
         if ((unsigned)row < (unsigned)lr) {
@@ -277,8 +285,8 @@ __device__ __forceinline__ SweepOut sweep_group(const uint8_t *__restrict__ rseq
                 // rewritten until late in the next step); one 64-bit store would need an
                 // adjacent copy that the next step's first profile load overwrites,
                 // forcing an early wait
-                bh[row] = (uint32_t)O::bits(hout);
-                be[row] = (uint32_t)O::bits(eout);
+                *bnd(bh, boff + (uint32_t)row) = (uint32_t)O::bits(hout);
+                *bnd(be, boff + (uint32_t)row) = (uint32_t)O::bits(eout);
             }
         }
         din = hin;

@@ -166,11 +166,11 @@ __attribute__((amdgpu_waves_per_eu(WavesPerEU<LOCAL, G, R>::value))) stripe_kern
                 const int ref = rpos < item.rend ? a.rorder[rpos] : -1;
                 const int rs = ref >= 0 ? a.roff[ref] : 0;
                 const int lr = ref >= 0 ? a.roff[ref + 1] - rs : 0;
-                uint32_t *bh = bnd_wg + (ref >= 0 ? a.bpos[rpos] - row_base : 0);
+                const uint32_t boff = ref >= 0 ? (uint32_t)(a.bpos[rpos] - row_base) : 0u;
                 const int nsteps = __builtin_amdgcn_readfirstlane(wave_max_i32(lr)) + G - 1;
                 const SweepOut res = sweep_group<V, G, R, LOCAL, true>(a.rcodes + rs, lr, nsteps, prof_lane, t, a.k,
-                                                                       lqa, lqb, cbase, bh, bh + a.bnd_stride,
-                                                                       !first, !last);
+                                                                       lqa, lqb, cbase, bnd_wg, bnd_wg + a.bnd_stride,
+                                                                       !first, !last, nullptr, 0, boff);
                 if (t == 0 && ref >= 0) {
                     int va = res.a, vb = res.b;
                     if (!first) {   // written by this workgroup in an earlier stripe
