@@ -1,13 +1,15 @@
 // Serving loop: resident reference databases, memory-budget admission and continuous
 // batching of top-k requests into the persistent cross kernel.
 //
-// A worker thread owns a private stream.  Submitted requests wait in FIFO order until
+// A worker thread owns a private stream and one stream per launch slot.  Submitted
+// requests wait in FIFO order until
 // their device memory (query codes + score rows + top-k output) fits in the server's
 // arenas; admitted requests are planned into cross work items (query pair x chunk of
 // length-sorted references).  Every launch takes items round-robin from all admitted
 // requests of one (database, tile), up to a cell budget, so requests that arrive while
 // others run join the next launch and share the device with them.  Two launches are kept
-// in flight.  When a request's last item completes, a device top-k of its score rows
+// in flight on the two slots' streams, so the second takes the CUs as the first one's last
+// workgroups finish.  When a request's last item completes, a device top-k of its score rows
 // (score descending, ties to the lower reference index) is copied back and its memory is
 // released.  Saturated int16 results are rescored in int32 before a launch is retired.
 #include <algorithm>
@@ -103,6 +105,9 @@ struct Request {
 };
 
 struct Slot {
+    hipStream_t st = nullptr;      // the slot's own stream (default priority): the two slots'
+                                   // launches overlap, each one's workgroups taking CUs as the
+                                   // other's finish
     hipEvent_t ev = nullptr;
     void *items = nullptr;
     size_t items_cap = 0;
@@ -122,6 +127,7 @@ constexpr int NSLOTS = 2;
 
 struct Server {
     DeviceCtx ctx;                       // private stream, events and scratch
+    hipEvent_t codes_ready = nullptr;    // after the latest query-code upload on ctx.stream
     uint8_t *code_base = nullptr;
     uint8_t *score_base = nullptr;
     Arena code_arena, score_arena;
@@ -184,22 +190,27 @@ int Server::plan_locked(Request *r) {
     std::vector<CrossQuery> plain;
     GSW_TRY(plan_cross_units(db.s, GSW_DTYPE_AUTO, cq, rs.max_len, db.mean_lr, (size_t)ctx.max_smem, &units,
                              &plain));
-    // chunk references so an item is ~1e8 cells: enough items per launch to fill the GPU
+    // chunk references into small items: a launch gathers items of one tile from the
+    // admitted requests, and small requests or rare tiles must still give it more items than
+    // workgroups (with 1e8-cell items, 29 of 120 launches of the serving mix had fewer)
     const int64_t nref = rs.n_nonempty;
     for (size_t u0 = 0; u0 < units.size();) {
         size_t u1 = u0;
         const bool striped = units[u0].nstr > 1;
         while (u1 < units.size() && units[u1].v == units[u0].v && (units[u1].nstr > 1) == striped) ++u1;
         const Variant &v = *units[u0].v;
+        const int gpw = 64 / v.G;
         if (striped) {
-            expand_stripe_items(units.data() + u0, units.data() + u1, rs, stripe_item_rows(db.mean_lr, rs.max_len),
-                                &r->items);
+            // a quarter of the batch path's item, at least four blocks per wave per stripe
+            const int64_t rows = std::max<int64_t>(stripe_item_rows(db.mean_lr, rs.max_len) / 4,
+                                                   (int64_t)(4.0 * gpw * CROSS_WAVES * db.mean_lr));
+            expand_stripe_items(units.data() + u0, units.data() + u1, rs, rows, &r->items);
         } else {
-            // items of ~1e8 cells, at least one block of references per wave
-            const int gpw = 64 / v.G;
+            // items of ~5e7 cells, at least two blocks of references per wave (the waves of a
+            // workgroup then finish an item within about a block of each other)
             const double qcols = (double)(units[u0].a.len + std::max(units[u0].b.len, 0));
-            int64_t rows = (int64_t)(1e8 / std::max(qcols, 1.0));
-            rows = std::max<int64_t>(rows, (int64_t)(gpw * CROSS_WAVES * db.mean_lr));
+            int64_t rows = (int64_t)(5e7 / std::max(qcols, 1.0));
+            rows = std::max<int64_t>(rows, (int64_t)(2.0 * gpw * CROSS_WAVES * db.mean_lr));
             expand_cross_items(units.data() + u0, units.data() + u1, rs, rows, gpw, &r->items);
             (void)nref;
         }
@@ -264,9 +275,12 @@ int Server::admit_locked() {
         r->codes_at = ca;
         r->scores_at = sa;
         r->t_admit = now_ms();
-        if (!r->codes.empty())
+        if (!r->codes.empty()) {
             GSW_HIP_TRY(hipMemcpyAsync(code_base + ca, r->codes.data(), r->codes.size(), hipMemcpyHostToDevice,
                                        ctx.stream));
+            // the slots' streams wait for it before their next launch
+            GSW_HIP_TRY(hipEventRecord(codes_ready, ctx.stream));
+        }
         int rc = plan_locked(r);
         if (rc != GSW_OK) {
             r->status = rc;
@@ -337,11 +351,18 @@ int Server::launch_locked(int si, Request *head) {
     });
     const size_t bytes = sizeof(CrossItem) * sl.host_items.size();
     GSW_TRY(ensure_buffer(&sl.items, &sl.items_cap, bytes));
-    GSW_HIP_TRY(hipMemcpyAsync(sl.items, sl.host_items.data(), bytes, hipMemcpyHostToDevice, ctx.stream));
-    GSW_HIP_TRY(hipMemsetAsync(sl.counters, 0, 2 * sizeof(int32_t), ctx.stream));
+    GSW_HIP_TRY(hipStreamWaitEvent(sl.st, codes_ready, 0));   // this launch's query codes are uploaded
+    GSW_HIP_TRY(hipMemcpyAsync(sl.items, sl.host_items.data(), bytes, hipMemcpyHostToDevice, sl.st));
+    GSW_HIP_TRY(hipMemsetAsync(sl.counters, 0, 2 * sizeof(int32_t), sl.st));
     const size_t lds = cross_lds_bytes(v, d.s.ns);
     int grid = 0;
     GSW_TRY(occupancy_grid(&ctx, striped ? v.stripe_fn : v.cross_fn, CROSS_WAVES * 64, lds, &grid));
+    // GSW_TRACE_LAUNCHES=1: one line per launch to stderr
+    static const bool trace = [] { const char *e = std::getenv("GSW_TRACE_LAUNCHES"); return e && *e == '1'; }();
+    if (trace)
+        std::fprintf(stderr, "[gsw] serve launch %lld: db=%d G=%d R=%d%s requests=%zu items=%zu grid=%d cells=%.3e\n",
+                     (long long)launches, db, v.G, v.R, striped ? " striped" : "", sl.reqs.size(),
+                     sl.host_items.size(), grid, cells);
     grid = std::min<int>(grid, (int)sl.host_items.size());
     int64_t rows = 0;   // boundary entries per workgroup (striped)
     if (striped) {
@@ -369,12 +390,12 @@ int Server::launch_locked(int si, Request *head) {
         a.bnd = static_cast<uint32_t *>(sl.bnd);
         a.bnd_stride = rows;
         a.bpos = d.rs->d_bpos;
-        v.stripe(a, grid, lds, ctx.stream);
+        v.stripe(a, grid, lds, sl.st);
     } else {
-        v.cross(a, grid, lds, ctx.stream);
+        v.cross(a, grid, lds, sl.st);
     }
     GSW_HIP_TRY(hipGetLastError());
-    GSW_HIP_TRY(hipEventRecord(sl.ev, ctx.stream));
+    GSW_HIP_TRY(hipEventRecord(sl.ev, sl.st));
     launches++;
     inflight.push_back(si);
     return GSW_OK;
@@ -525,6 +546,7 @@ int gsw_server_create(size_t mem_budget_bytes, double launch_cells, gsw_server *
     GSW_HIP_TRY(hipStreamCreateWithFlags(&srv->ctx.stream, hipStreamNonBlocking));
     GSW_HIP_TRY(hipEventCreate(&srv->ctx.ev0));
     GSW_HIP_TRY(hipEventCreate(&srv->ctx.ev1));
+    GSW_HIP_TRY(hipEventCreateWithFlags(&srv->codes_ready, hipEventDisableTiming));
     GSW_HIP_TRY(hipMalloc(&srv->ctx.counters, 64 * sizeof(int32_t)));
     if (launch_cells > 0) srv->launch_cells = launch_cells;
     // query codes are addressed with 32-bit offsets: at most 1 GiB of them
@@ -536,6 +558,7 @@ int gsw_server_create(size_t mem_budget_bytes, double launch_cells, gsw_server *
     srv->code_arena.init(code_cap);
     srv->score_arena.init(score_cap);
     for (int i = 0; i < NSLOTS; ++i) {
+        GSW_HIP_TRY(hipStreamCreateWithFlags(&srv->slots[i].st, hipStreamNonBlocking));
         GSW_HIP_TRY(hipEventCreateWithFlags(&srv->slots[i].ev, hipEventDisableTiming));
         GSW_HIP_TRY(hipMalloc(&srv->slots[i].counters, 2 * sizeof(int32_t)));
         GSW_HIP_TRY(hipMalloc(&srv->slots[i].sat, sizeof(SatEntry) * SAT_CAP));
@@ -662,6 +685,10 @@ int gsw_server_destroy(gsw_server server) {
     if (srv->worker.joinable()) srv->worker.join();
     (void)hipStreamSynchronize(srv->ctx.stream);
     for (int i = 0; i < NSLOTS; ++i) {
+        if (srv->slots[i].st) {
+            (void)hipStreamSynchronize(srv->slots[i].st);
+            (void)hipStreamDestroy(srv->slots[i].st);
+        }
         (void)hipEventDestroy(srv->slots[i].ev);
         (void)hipFree(srv->slots[i].counters);
         (void)hipFree(srv->slots[i].sat);
@@ -676,6 +703,7 @@ int gsw_server_destroy(gsw_server server) {
     if (srv->ctx.meta2) (void)hipFree(srv->ctx.meta2);
     (void)hipEventDestroy(srv->ctx.ev0);
     (void)hipEventDestroy(srv->ctx.ev1);
+    if (srv->codes_ready) (void)hipEventDestroy(srv->codes_ready);
     (void)hipStreamDestroy(srv->ctx.stream);
     delete srv;
     return GSW_OK;
