@@ -1246,14 +1246,20 @@ class HostPool {
     bool quit_ = false;
 };
 
+// The library's one pool (a function-local static in the template below would give each
+// caller's lambda type a pool of its own).
+static HostPool &host_pool() {
+    static HostPool pool;
+    return pool;
+}
+
 // fn(tid) on T threads (tid 0 on the calling thread).
 template <typename F> static void run_threads(int T, F &&fn) {
     if (T <= 1) {
         fn(0);
         return;
     }
-    static HostPool pool;
-    pool.run(T, fn);
+    host_pool().run(T, fn);
 }
 
 constexpr int MAX_PAIR_SLOTS = 8;   // pinned buffers + graphs per tile
