@@ -1553,14 +1553,23 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
         }
     };
     const int W = (int)std::min<size_t>((size_t)T, nb);
-    while (ctx->marks.size() < 2 * nb) {   // timing events before and after each batch
+    while (ctx->marks.size() < 2 * nb + 2) {   // before/after each batch, fork and join
         hipEvent_t e;
         GSW_HIP_TRY(hipEventCreate(&e));
         ctx->marks.push_back(e);
     }
     int rc = GSW_OK;
+    // Batches alternate between the library stream and the second one; on its stream a
+    // batch's task copy follows the kernel two batches back, so it overlaps the other
+    // stream's kernel, and its kernel takes the CUs as that kernel's last waves finish.
+    const hipStream_t streams[2] = {ctx->stream, ctx->stream2};
+    hipEvent_t fork = ctx->marks[2 * nb], join = ctx->marks[2 * nb + 1];
     auto launcher = [&]() {
         if (hipEventRecord(ctx->ev0, ctx->stream) != hipSuccess) rc = fail(GSW_ERR_HIP, "event record failed");
+        // the second stream starts after the saturation counter's reset
+        if (rc == GSW_OK && (hipEventRecord(fork, ctx->stream) != hipSuccess ||
+                             hipStreamWaitEvent(ctx->stream2, fork, 0) != hipSuccess))
+            rc = fail(GSW_ERR_HIP, "event record failed");
         for (size_t b = 0; b < nb && rc == GSW_OK; ++b) {
             {
                 std::unique_lock<std::mutex> lk(m);
@@ -1572,17 +1581,18 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
             }
             const Batch &B = batches[b];
             PairGraph &g = *slot_of.at({B.kind, B.bi % B.nslot});
-            // copy once the slot's previous kernel is done with d_buf; run once copied
-            hipError_t e = hipStreamWaitEvent(ctx->stream2, g.consumed, 0);
+            // copy once the slot's previous kernel (on either stream) is done with d_buf, then
+            // run; planners reuse h_buf once `copied` has passed
+            const hipStream_t st = streams[b & 1];
+            hipError_t e = hipStreamWaitEvent(st, g.consumed, 0);
             if (e == hipSuccess)
                 e = hipMemcpyAsync(g.d_buf, g.h_buf, PAIR_HDR + sizeof(PairTask) * (size_t)ntasks[b],
-                                   hipMemcpyHostToDevice, ctx->stream2);
-            if (e == hipSuccess) e = hipEventRecord(g.copied, ctx->stream2);
-            if (e == hipSuccess) e = hipStreamWaitEvent(ctx->stream, g.copied, 0);
-            if (e == hipSuccess) e = hipEventRecord(ctx->marks[2 * b], ctx->stream);
-            if (e == hipSuccess) e = hipGraphLaunch(g.exec, ctx->stream);
-            if (e == hipSuccess) e = hipEventRecord(ctx->marks[2 * b + 1], ctx->stream);
-            if (e == hipSuccess) e = hipEventRecord(g.consumed, ctx->stream);
+                                   hipMemcpyHostToDevice, st);
+            if (e == hipSuccess) e = hipEventRecord(g.copied, st);
+            if (e == hipSuccess) e = hipEventRecord(ctx->marks[2 * b], st);
+            if (e == hipSuccess) e = hipGraphLaunch(g.exec, st);
+            if (e == hipSuccess) e = hipEventRecord(ctx->marks[2 * b + 1], st);
+            if (e == hipSuccess) e = hipEventRecord(g.consumed, st);
             if (e != hipSuccess) {
                 rc = fail(GSW_ERR_HIP, "pairs batch launch: %s", hipGetErrorString(e));
                 break;
@@ -1610,8 +1620,14 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
         else
             worker();
     });
-    if (rc != GSW_OK) return rc;
+    if (rc != GSW_OK) {   // nothing may still run when the caller sees the error
+        (void)hipStreamSynchronize(ctx->stream2);
+        (void)hipStreamSynchronize(ctx->stream);
+        return rc;
+    }
     const clk::time_point t_launched = clk::now();
+    GSW_HIP_TRY(hipEventRecord(join, ctx->stream2));
+    GSW_HIP_TRY(hipStreamWaitEvent(ctx->stream, join, 0));
     GSW_HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
     GSW_HIP_TRY(hipEventSynchronize(ctx->ev1));
     float ms = 0.f;
@@ -1625,26 +1641,29 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
                      d(t_first, t_launched), d(t_launched, clk::now()), ms);
     }
     res->variant = dominant >= 0 ? dominant / 2 : -1;
-    // the dominant tile's full batches (all of its batches if none is full): mean device
-    // time of a batch's kernel (events recorded right before and after its graph launch,
-    // after the wait for its copy) and mean cells
+    // the dominant tile's full batches (all of its batches if none is full): device time
+    // from the first one's start event to the last one's end event, per batch (consecutive
+    // batches overlap on the two streams, so one batch's own interval would count shared
+    // time), and mean cells
     {
         bool any_full = false;
         for (size_t b = 0; b < nb; ++b)
             any_full |= batches[b].kind == dominant && batches[b].b1 - batches[b].b0 == PAIR_BATCH;
-        double sum_ms = 0, sum_cells = 0;
+        double sum_cells = 0, t_lo = 1e30, t_hi = -1e30;
         int cnt = 0;
         for (size_t b = 0; b < nb; ++b) {
             if (batches[b].kind != dominant) continue;
             if (any_full && batches[b].b1 - batches[b].b0 != PAIR_BATCH) continue;
-            float bms = 0.f;
-            GSW_HIP_TRY(hipEventElapsedTime(&bms, ctx->marks[2 * b], ctx->marks[2 * b + 1]));
-            sum_ms += bms;
+            float t0 = 0.f, t1 = 0.f;
+            GSW_HIP_TRY(hipEventElapsedTime(&t0, ctx->ev0, ctx->marks[2 * b]));
+            GSW_HIP_TRY(hipEventElapsedTime(&t1, ctx->ev0, ctx->marks[2 * b + 1]));
+            t_lo = std::min(t_lo, (double)t0);
+            t_hi = std::max(t_hi, (double)t1);
             sum_cells += batch_cells[b];
             ++cnt;
         }
-        if (cnt) {
-            res->top_ms = sum_ms / cnt;
+        if (cnt && t_hi > t_lo) {
+            res->top_ms = (t_hi - t_lo) / cnt;
             res->top_cells = (int64_t)(sum_cells / cnt);
             res->top_variant = dominant / 2;
             res->top_striped = (dominant & 1) ? 3 : 2;   // pairs kernel (3: table kernel)
