@@ -71,7 +71,8 @@ typedef struct gsw_stats {
                           /*   most cells (events before and after it on its    */
                           /*   stream; includes time shared with launches of    */
                           /*   the other stream);                               */
-                          /* pairs: mean time of the dominant tile's full batches */
+                          /* pairs: device time of the dominant tile's full     */
+                          /*   batches (first start to last end) per batch      */
     int64_t top_cells;    /*   cells of that launch (pairs: mean per batch)      */
     int32_t top_variant;  /*   its variant id (-1: none)                         */
     int32_t top_striped;  /*   kernel: 0 cross, 1 stripe, 2 pairs, 3 pairs table */
