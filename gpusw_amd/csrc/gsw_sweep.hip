@@ -1307,11 +1307,25 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
         }
     } trim{vid, members};
     if (vid.size() < (size_t)n) vid.resize((size_t)n);
+    // The int16 and int32 range checks are monotone in the reference length for a given
+    // query length: the largest reference length each allows is found once per query
+    // length (binary search) and a pair costs one comparison per check.
+    auto lr_limit = [&](bool (*fits)(const SchemeDev &, int, int), int lq) -> int {
+        if (!fits(s, lq, 1)) return 0;
+        int lo = 1, hi = INT32_MAX;   // fits(lo) holds
+        while (lo < hi) {
+            const int mid = lo + (hi - lo + 1) / 2;
+            if (fits(s, lq, mid)) lo = mid;
+            else hi = mid - 1;
+        }
+        return lo;
+    };
     run_threads(T, [&](int tid) {
         Part &P = parts[tid];
         P.count.assign(nv, 0);
         std::vector<const Variant *> vcache(2 * (wmax + 1), nullptr);
         std::vector<char> vdone(2 * (wmax + 1), 0), vtab(2 * (wmax + 1), 0);
+        std::vector<int> lim16(wmax + 1, -1), lim32(wmax + 1, -1);   // -1: not computed
         const int64_t k0 = n * tid / T, k1 = n * (tid + 1) / T;
         for (int64_t k = k0; k < k1; ++k) {
             const int q = h_qi[k], r = h_rj[k];
@@ -1328,12 +1342,24 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
                 P.e_val.push_back(empty_edge_score(s, lq, lr));
                 continue;
             }
-            if (!fits_int32(s, lq, lr)) {
+            bool ok32, ok16;
+            if (lq <= wmax) {
+                if (lim32[lq] < 0) {
+                    lim32[lq] = lr_limit(fits_int32, lq);
+                    lim16[lq] = lr_limit(fits_int16, lq);
+                }
+                ok32 = lr <= lim32[lq];
+                ok16 = lr <= lim16[lq];
+            } else {
+                ok32 = fits_int32(s, lq, lr);
+                ok16 = fits_int16(s, lq, lr);
+            }
+            if (!ok32) {
                 P.bad = k;
                 P.bad_code = GSW_ERR_OVERFLOW;
                 return;
             }
-            const bool use16 = dtype_policy != GSW_DTYPE_INT32 && fits_int16(s, lq, lr);
+            const bool use16 = dtype_policy != GSW_DTYPE_INT32 && ok16;
             const Variant *v = nullptr;
             bool tab = false;
             if (lq <= wmax) {
