@@ -379,9 +379,10 @@ std::vector<Variant> &variant_table() {
     static std::vector<Variant> table = [] {
         std::vector<Variant> v = {
             GSW_VARIANT4(8, 8),   GSW_VARIANT4(8, 12),  GSW_VARIANT4(8, 16),  GSW_SVARIANT4(8, 20),
-            GSW_SVARIANT4(8, 24), GSW_SVARIANT4(8, 32), GSW_VARIANT4(8, 36),  GSW_VARIANT4(8, 40),
-            GSW_VARIANT4(8, 44),  GSW_VARIANT4(8, 48),  GSW_VARIANT4(16, 24), GSW_VARIANT4(16, 32),
-            GSW_VARIANT4(16, 48), GSW_VARIANT4(32, 32), GSW_VARIANT4(32, 48), GSW_VARIANT4(64, 32),
+            GSW_SVARIANT4(8, 24), GSW_SVARIANT4(8, 32), GSW_VARIANT4(8, 34),  GSW_VARIANT4(8, 36),
+            GSW_VARIANT4(8, 40),  GSW_VARIANT4(8, 44),  GSW_VARIANT4(8, 48),  GSW_VARIANT4(16, 24),
+            GSW_VARIANT4(16, 32), GSW_VARIANT4(16, 48), GSW_VARIANT4(32, 32), GSW_VARIANT4(32, 48),
+            GSW_VARIANT4(64, 32),
         };
         for (size_t i = 0; i < v.size(); ++i) v[i].id = (int)i;
         return v;
@@ -416,7 +417,7 @@ static double tile_cost(int G, int R, int lq, double lr) {
 
 size_t cross_lds_bytes(const Variant &v, int ns);
 
-// Occupancy of a cross or stripe tile relative to the 4 waves per SIMD of R <= 32: R 36-44
+// Occupancy of a cross or stripe tile relative to the 4 waves per SIMD of R <= 32: R 34-44
 // run 3 waves, R 48 two, and a profile above 80 KiB leaves one workgroup per CU.  (Measured
 // on BLOSUM62 top-k, 1,000 x 100,000: padded-column rates 5-8 % below the R <= 32 tiles.)
 static double tile_occupancy_cost(const Variant &v, int ns) {
