@@ -1496,10 +1496,15 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
         if (track_last_col) {
             S.cnt.assign(v.R + 1, 0);
             for (int64_t k : S.order) S.cnt[shift_of(k) + 1]++;
-            for (int r = 0; r < v.R; ++r) S.cnt[r + 1] += S.cnt[r];
-            S.order2.resize(S.order.size());
-            for (int64_t k : S.order) S.order2[S.cnt[shift_of(k)]++] = k;
-            S.order.swap(S.order2);
+            int distinct = 0;
+            for (int r = 1; r <= v.R; ++r) distinct += S.cnt[r] > 0;
+            // a second (stable) pass by front padding, unless the whole batch shares one
+            if (distinct > 1) {
+                for (int r = 0; r < v.R; ++r) S.cnt[r + 1] += S.cnt[r];
+                S.order2.resize(S.order.size());
+                for (int64_t k : S.order) S.order2[S.cnt[shift_of(k)]++] = k;
+                S.order.swap(S.order2);
+            }
         }
         PairTask *tasks = reinterpret_cast<PairTask *>(g.h_buf + PAIR_HDR);
         int32_t nt = 0;
