@@ -302,9 +302,12 @@ def run_plain_dna(args):
     (qc, qo), (rc, ro) = synth.plain_dna(seed=0)
     scheme = Scheme.preset("dna").replace(dtype="int32")
     res = {}
+    # a call is under a millisecond: at least three untimed calls (the first calls of a new
+    # aligner still settle kernel attributes, events and buffers)
+    warmup = max(args.warmup, 3)
     for backend in ("hip", "hip-plain"):
         al = Aligner(scheme, backend=backend).index((rc, ro)).set_queries((qc, qo))
-        for _ in range(args.warmup):
+        for _ in range(warmup):
             al.score_cross()
         gpusw_amd._hip.call("gsw_synchronize")
         t0 = time.perf_counter()
@@ -324,7 +327,7 @@ def run_plain_dna(args):
     value = cells / hip_wall / 1e9
     return {
         "metric": METRIC, "value": round(value, 2), "unit": "GCUPS", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(hip_wall * 1e3, 3), "higher_is_better": True,
+        "warmup": warmup, "ms_per_step": round(hip_wall * 1e3, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "int32", "data": "synthetic",
         "config": {"workload": "plain DNA: 64 x 256 (lengths 100-300, seed 0), dna local, int32; host-side "
                                "call including the 64 KB result copy", "cells": cells},

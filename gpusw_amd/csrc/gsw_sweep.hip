@@ -10,6 +10,7 @@
 #include <map>
 #include <memory>
 #include <numeric>
+#include <set>
 #include <tuple>
 #include <atomic>
 #include <condition_variable>
@@ -879,6 +880,8 @@ int batch_cross(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
     if (nq == 0 || nr == 0) return GSW_OK;
     const bool local = s.mode == GSW_MODE_LOCAL;
     double mean_lr = rs.n_nonempty ? (double)rs.total / rs.n_nonempty : 1.0;
+    using clk = std::chrono::steady_clock;
+    const clk::time_point t_in = clk::now();
 
     // -- empty rows/columns: closed forms
     std::vector<int64_t> e_idx;
@@ -911,6 +914,29 @@ int batch_cross(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
     std::vector<CrossUnit> units;
     std::vector<CrossQuery> plain_cq;
     GSW_TRY(plan_cross_units(s, dtype_policy, cq, rs.max_len, mean_lr, (size_t)ctx->max_smem, &units, &plain_cq));
+    // Small calls: below ~1e10 cells a call is bound by launch latency (each tile's launch
+    // is a few short items), so the single-stripe units of each element type take the
+    // widest tile chosen for that type (it covers them all) and the call has one launch per
+    // type.  Not with last-column tracking, whose front padding depends on R.
+    if (!(s.mode == GSW_MODE_SEMIGLOBAL && s.free_q)) {
+        double cells = 0;
+        const Variant *wide[2] = {nullptr, nullptr};   // int32, int16x2
+        std::set<const Variant *> used;
+        for (const CrossUnit &u : units) {
+            cells += (double)(u.a.len + (u.has_b ? u.b.len : 0)) * (double)rs.total;
+            if (u.nstr != 1) continue;
+            used.insert(u.v);
+            const Variant *&w = wide[u.v->packed ? 1 : 0];
+            if (!w || u.v->G * u.v->R > w->G * w->R) w = u.v;
+        }
+        if (cells < 1e10 && used.size() > (size_t)((wide[0] != nullptr) + (wide[1] != nullptr))) {
+            for (CrossUnit &u : units)
+                if (u.nstr == 1) u.v = wide[u.v->packed ? 1 : 0];
+            std::stable_sort(units.begin(), units.end(), [](const CrossUnit &x, const CrossUnit &y) {
+                return std::make_pair(x.nstr == 1, x.v->id) < std::make_pair(y.nstr == 1, y.v->id);
+            });
+        }
+    }
     std::vector<int32_t> plain_rows, plain_q;
     for (const CrossQuery &x : plain_cq) {
         plain_rows.push_back((int32_t)(x.out / nr));
@@ -962,10 +988,10 @@ int batch_cross(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
     // Launches alternate between the library stream and the second stream, so each one's
     // workgroups take the CUs as the previous launch's last workgroups finish instead of
     // waiting for all of them.  Every launch is timed by events right before and after it
-    // on its stream (the longest is reported; the interval can include the end of the
-    // previous launch, which it shares the GPU with).  GSW_TRACE_LAUNCHES=1 also prints
-    // tile, cells and GCUPS of each launch to stderr.
+    // on its stream (the launch with the most cells is reported).  GSW_TRACE_LAUNCHES=1
+    // also prints tile, cells and GCUPS of each launch, and the call's host phases, to stderr.
     static const bool trace = [] { const char *e = std::getenv("GSW_TRACE_LAUNCHES"); return e && *e == '1'; }();
+    const clk::time_point t_planned = clk::now();
     while (ctx->marks.size() < 2 * groups.size() + 2) {
         hipEvent_t e;
         GSW_HIP_TRY(hipEventCreate(&e));
@@ -1014,10 +1040,17 @@ int batch_cross(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
         GSW_HIP_TRY(hipEventRecord(join, ctx->stream2));
         GSW_HIP_TRY(hipStreamWaitEvent(ctx->stream, join, 0));
     }
+    const clk::time_point t_launched = clk::now();
     GSW_HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
     GSW_HIP_TRY(hipEventSynchronize(ctx->ev1));
     float ms = 0.f;
     GSW_HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    if (trace) {
+        auto d = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
+        std::fprintf(stderr, "[gsw] cross nq=%d nr=%d launches=%zu: plan %.2f ms, launch +%.2f ms, device done +%.2f ms "
+                             "(device %.2f ms)\n", nq, nr, groups.size(), d(t_in, t_planned), d(t_planned, t_launched),
+                     d(t_launched, clk::now()), ms);
+    }
     // A launch's time runs from its start event to its end event on its stream; when the
     // other stream's launches run beside it, it includes the time it shared the GPU (or
     // waited for CUs).  The launch with the most cells is reported.

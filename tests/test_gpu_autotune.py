@@ -29,7 +29,7 @@ def test_tune_installs_rules_and_stays_exact(tmp_path):
         autotune.clear()
 
 
-@pytest.mark.parametrize("mode", ["local", "semiglobal"])
+@pytest.mark.parametrize("mode", ["local", "global", "semiglobal"])
 def test_every_forced_tile_exact(mode):
     rng = np.random.default_rng(13)
     scheme = Scheme(mode=mode)

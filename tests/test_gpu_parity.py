@@ -106,7 +106,9 @@ def test_int32_promotion(backend):
 
 @pytest.mark.parametrize("mode", ["local", "global", "semi_qr", "semi_q", "semi_r"])
 def test_cross_all_tile_widths(mode):
-    """Query lengths 1..700 hit every compiled tile width; references are ragged."""
+    """Query lengths 1..700 across the compiled tile widths (a call this small runs its
+    single-stripe queries on the widest chosen tile; every tile on its own is covered by
+    the forced-tile tests), stripes beyond; references are ragged."""
     kw = {"local": dict(mode="local"), "global": dict(mode="global"),
           "semi_qr": dict(mode="semiglobal"),
           "semi_q": dict(mode="semiglobal", free_end_gaps_query=True, free_end_gaps_ref=False),
