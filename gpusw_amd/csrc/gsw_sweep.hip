@@ -581,10 +581,28 @@ static const Variant *pick_pairs_tile(int lq, double mean_lr, bool packed, bool 
     return best;
 }
 
+// Persistent grid of a kernel: resident workgroups per CU x CUs.  The kernel's dynamic-LDS
+// limit is raised only when a launch needs more than it was set to, and occupancies are
+// cached per (device, kernel, threads, LDS bytes): the runtime queries cost tens of
+// microseconds per launch group, which small calls notice.
 int occupancy_grid(DeviceCtx *ctx, const void *fn, int threads, size_t lds, int *grid) {
+    static std::mutex mu;
+    static std::map<std::pair<int, const void *>, size_t> lds_set;
+    static std::map<std::tuple<int, const void *, int, size_t>, int> occ;
+    std::lock_guard<std::mutex> lk(mu);
+    auto oit = occ.find(std::make_tuple(ctx->device, fn, threads, lds));
     int per_cu = 0;
-    GSW_HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    GSW_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, lds));
+    if (oit != occ.end()) {
+        per_cu = oit->second;
+    } else {
+        size_t &set = lds_set[std::make_pair(ctx->device, fn)];
+        if (lds > set) {
+            GSW_HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            set = lds;
+        }
+        GSW_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, lds));
+        occ[std::make_tuple(ctx->device, fn, threads, lds)] = per_cu;
+    }
     if (per_cu <= 0) return fail(GSW_ERR_LIMIT, "kernel does not fit on a CU (lds %zu bytes)", lds);
     *grid = per_cu * ctx->num_cus;
     return GSW_OK;
