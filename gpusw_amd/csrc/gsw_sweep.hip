@@ -382,7 +382,8 @@ std::vector<Variant> &variant_table() {
             GSW_VARIANT4(8, 8),   GSW_VARIANT4(8, 12),  GSW_VARIANT4(8, 16),  GSW_SVARIANT4(8, 20),
             GSW_SVARIANT4(8, 24), GSW_SVARIANT4(8, 32), GSW_VARIANT4(8, 34),  GSW_VARIANT4(8, 36),
             GSW_VARIANT4(8, 40),  GSW_VARIANT4(8, 44),  GSW_VARIANT4(8, 48),  GSW_VARIANT4(16, 24),
-            GSW_VARIANT4(16, 32), GSW_VARIANT4(16, 48), GSW_VARIANT4(32, 32), GSW_VARIANT4(32, 48),
+            GSW_VARIANT4(16, 28), GSW_VARIANT4(16, 32), GSW_VARIANT4(16, 36), GSW_VARIANT4(16, 40),
+            GSW_VARIANT4(16, 44), GSW_VARIANT4(16, 48), GSW_VARIANT4(32, 32), GSW_VARIANT4(32, 48),
             GSW_VARIANT4(64, 32),
         };
         for (size_t i = 0; i < v.size(); ++i) v[i].id = (int)i;
