@@ -121,7 +121,7 @@ def _roofline(history, ops_per_cell: int, workload: str) -> dict:
     if not top:
         return {"bound": "valu", "achieved": None, "peak": PEAK_INT16_TOPS, "unit": "TOP/s (int16)",
                 "frac": None, "traffic": None}
-    # the most frequent (variant, striped) among the calls' longest launches
+    # the most frequent (variant, striped) among the calls' dominant launches (most cells)
     keys = [(h.top_variant, h.top_striped) for h in top]
     var, striped = max(set(keys), key=keys.count)
     sel = [h for h in top if (h.top_variant, h.top_striped) == (var, striped)]
