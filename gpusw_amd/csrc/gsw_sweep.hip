@@ -1153,11 +1153,11 @@ int batch_cross(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
 // ---------------------------------------------------------------- pairs
 // Explicit pairs run in batches of PAIR_BATCH pairs.  Each batch is planned on the host
 // in O(n) (counting sorts by reference length rank, then by last-column register) into a
-// pinned buffer of its slot, copied to the slot's device buffer on the second stream, and
-// run by replaying the slot's HIP graph (the persistent pairs kernel with this call's
-// arguments) on the library stream once the copy is done.  The header carries the task
-// count and a zeroed work counter, so no memset is needed.  Slots alternate: planning
-// batch b+1 and copying its tasks overlap the kernel of batch b.
+// pinned buffer of its slot, copied to the slot's device buffer and run by replaying the
+// slot's HIP graph (the persistent pairs kernel with this call's arguments), both on one
+// of two streams that batches alternate between.  The header carries the task count and a
+// zeroed work counter, so no memset is needed.  Planning later batches and copying their
+// tasks overlap the kernels of earlier ones.
 constexpr int64_t PAIR_BATCH = 131072;
 constexpr size_t PAIR_HDR = 64;
 
@@ -1325,16 +1325,24 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
     static const bool trace = [] { const char *e = std::getenv("GSW_TRACE_LAUNCHES"); return e && *e == '1'; }();
     using clk = std::chrono::steady_clock;
     const clk::time_point t_in = clk::now();
-    clk::time_point t_members, t_first;
+    clk::time_point t_first{}, t_head{}, t_rest{};
     const bool local = s.mode == GSW_MODE_LOCAL;
     const bool track_last_col = s.mode == GSW_MODE_SEMIGLOBAL && s.free_q;
     const int wmax = max_width();
     const double mean_lr = rs.n_nonempty ? (double)rs.total / rs.n_nonempty : 1.0;
     const int T = n >= 65536 ? host_threads() : 1;
     const size_t nv = 2 * variant_table().size();   // kinds: 2 * variant id + (table kernel)
+    // The pair list runs in segments.  A call of at least four batches classifies, plans and
+    // launches a head of 21/8 batches first and classifies the rest while the device runs
+    // the head (classifying 1,000,000 pairs and listing each tile's members take about
+    // 1 ms, which otherwise precedes the first launch); a smaller call is one segment.  The
+    // call's first tile ramps up (1/8, 1/4, 1/2, 3/4 of a batch, then full batches), so the
+    // device starts after a short plan and has work while the first full batch is planned.
+    static constexpr int64_t RAMP[4] = {PAIR_BATCH / 8, PAIR_BATCH / 4, PAIR_BATCH / 2, PAIR_BATCH * 3 / 4};
+    const int64_t head = n >= 4 * PAIR_BATCH ? PAIR_BATCH * 21 / 8 : n;
 
-    // -- classify in parallel over contiguous index ranges: empty side (closed form), no
-    //    tile (comparator), else a tile
+    // -- classify a segment in parallel over contiguous index ranges: empty side (closed
+    //    form), no tile (comparator), else a tile
     struct Part {
         std::vector<int64_t> e_idx, p_oi;
         std::vector<int32_t> e_val, p_qi, p_rj;
@@ -1373,14 +1381,22 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
         }
         return lo;
     };
-    run_threads(T, [&](int tid) {
-        Part &P = parts[tid];
+    std::vector<int64_t> e_idx, p_oi;
+    std::vector<int32_t> e_val, p_qi, p_rj;
+    std::vector<int64_t> vbeg(nv + 1, 0);
+    std::vector<std::vector<int64_t>> pos(T, std::vector<int64_t>(nv));
+    res->cells = 0;
+    // one thread's share [a, b) of a segment
+    auto classify_part = [&](Part &P, int64_t a, int64_t b) {
+        P.e_idx.clear(); P.e_val.clear(); P.p_qi.clear(); P.p_rj.clear(); P.p_oi.clear();
         P.count.assign(nv, 0);
+        P.cells = 0;
+        P.bad = -1;
+        P.bad_code = GSW_OK;
         std::vector<const Variant *> vcache(2 * (wmax + 1), nullptr);
         std::vector<char> vdone(2 * (wmax + 1), 0), vtab(2 * (wmax + 1), 0);
         std::vector<int> lim16(wmax + 1, -1), lim32(wmax + 1, -1);   // -1: not computed
-        const int64_t k0 = n * tid / T, k1 = n * (tid + 1) / T;
-        for (int64_t k = k0; k < k1; ++k) {
+        for (int64_t k = a; k < b; ++k) {
             const int q = h_qi[k], r = h_rj[k];
             vid[k] = -1;
             if (q < 0 || q >= qs.n || r < 0 || r >= rs.n) {
@@ -1435,149 +1451,177 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
             vid[k] = (int16_t)(2 * v->id + (tab ? 1 : 0));
             P.count[2 * v->id + (tab ? 1 : 0)]++;
         }
-    });
-    const clk::time_point t_classified = clk::now();
-    for (const Part &P : parts)   // ranges are in index order: the first error is the lowest index
-        if (P.bad >= 0)
-            return fail(P.bad_code, P.bad_code == GSW_ERR_INVALID ? "pair %lld index out of range"
-                                                                  : "score range of pair %lld exceeds the int32 kernels",
-                        (long long)P.bad);
-    std::vector<int64_t> e_idx, p_oi;
-    std::vector<int32_t> e_val, p_qi, p_rj;
-    res->cells = 0;
-    for (Part &P : parts) {
-        res->cells += P.cells;
-        e_idx.insert(e_idx.end(), P.e_idx.begin(), P.e_idx.end());
-        e_val.insert(e_val.end(), P.e_val.begin(), P.e_val.end());
-        p_qi.insert(p_qi.end(), P.p_qi.begin(), P.p_qi.end());
-        p_rj.insert(p_rj.end(), P.p_rj.begin(), P.p_rj.end());
-        p_oi.insert(p_oi.end(), P.p_oi.begin(), P.p_oi.end());
-    }
-
-    // -- members of each tile, in index order (parallel scatter)
-    std::vector<int64_t> vbeg(nv + 1, 0);
-    std::vector<std::vector<int64_t>> pos(T, std::vector<int64_t>(nv));
-    for (size_t v = 0; v < nv; ++v) {
-        int64_t p = vbeg[v];
-        for (int t = 0; t < T; ++t) {
-            pos[t][v] = p;
-            p += parts[t].count[v];
+    };
+    // Classify pairs [k0, k1) and list each tile's members of the segment, in index order,
+    // in members[vbeg[kind], vbeg[kind + 1]).
+    auto classify = [&](int64_t k0, int64_t k1) -> int {
+        const int64_t m = k1 - k0;
+        const int Tc = m >= 65536 ? T : 1;
+        auto lo = [&](int t) { return k0 + m * t / Tc; };
+        run_threads(Tc, [&](int tid) { classify_part(parts[tid], lo(tid), lo(tid + 1)); });
+        for (int t = 0; t < Tc; ++t) {   // ranges are in index order: the first error is the lowest index
+            const Part &P = parts[t];
+            if (P.bad >= 0)
+                return fail(P.bad_code, P.bad_code == GSW_ERR_INVALID ? "pair %lld index out of range"
+                                                                      : "score range of pair %lld exceeds the int32 kernels",
+                            (long long)P.bad);
         }
-        vbeg[v + 1] = p;
-    }
-    if (members.size() < (size_t)vbeg[nv]) members.resize((size_t)vbeg[nv]);
-    run_threads(T, [&](int tid) {
-        const int64_t k0 = n * tid / T, k1 = n * (tid + 1) / T;
-        std::vector<int64_t> &ps = pos[tid];
-        for (int64_t k = k0; k < k1; ++k)
-            if (vid[k] >= 0) members[ps[vid[k]]++] = k;
-    });
+        for (int t = 0; t < Tc; ++t) {
+            const Part &P = parts[t];
+            res->cells += P.cells;
+            e_idx.insert(e_idx.end(), P.e_idx.begin(), P.e_idx.end());
+            e_val.insert(e_val.end(), P.e_val.begin(), P.e_val.end());
+            p_qi.insert(p_qi.end(), P.p_qi.begin(), P.p_qi.end());
+            p_rj.insert(p_rj.end(), P.p_rj.begin(), P.p_rj.end());
+            p_oi.insert(p_oi.end(), P.p_oi.begin(), P.p_oi.end());
+        }
+        // members of each tile (parallel scatter)
+        for (size_t v = 0; v < nv; ++v) {
+            int64_t p = vbeg[v];
+            for (int t = 0; t < Tc; ++t) {
+                pos[t][v] = p;
+                p += parts[t].count[v];
+            }
+            vbeg[v + 1] = p;
+        }
+        if (members.size() < (size_t)vbeg[nv]) members.resize((size_t)vbeg[nv]);
+        run_threads(Tc, [&](int tid) {
+            std::vector<int64_t> &ps = pos[tid];
+            for (int64_t k = lo(tid); k < lo(tid + 1); ++k)
+                if (vid[k] >= 0) members[ps[vid[k]]++] = k;
+        });
+        return GSW_OK;
+    };
 
-    t_members = clk::now();
-
-    // -- batches of PAIR_BATCH members of one tile; a tile's batches cycle through its slots
-    struct Batch { const Variant *v; bool tab; int kind; int64_t b0, b1; int bi, nslot; };
+    // -- a segment's batches: PAIR_BATCH members of one tile each (the call's first tile
+    //    ramps up); a tile's batches cycle through its MAX_PAIR_SLOTS slots, and a batch
+    //    records the one that used its slot before it (-1: none in this call)
+    struct Batch { const Variant *v; bool tab; int kind; int64_t b0, b1; int slot; int64_t prev; };
     std::vector<Batch> batches;
-    int dominant = -1;   // kind with the most pairs
-    int64_t dominant_n = 0;
-    for (size_t kind = 0; kind < nv; ++kind) {
-        const int64_t cnt_v = vbeg[kind + 1] - vbeg[kind];
-        if (cnt_v == 0) continue;
-        if (cnt_v > dominant_n) { dominant_n = cnt_v; dominant = (int)kind; }
-        const Variant &v = variant_table()[kind / 2];
-        // the call's first batches ramp up (1/8, 1/4, 1/2, 3/4 of a batch), so that the device
-        // starts after a short plan and has work (about 1.6 batches) while the first full
-        // batch is planned beside them (a trace showed a 1.1 ms idle gap with three ramp
-        // batches of 7/8 of a batch in all)
-        const bool ramp = batches.empty();
-        const size_t first = batches.size();
-        int bi = 0;
-        static constexpr int64_t RAMP[4] = {PAIR_BATCH / 8, PAIR_BATCH / 4, PAIR_BATCH / 2, PAIR_BATCH * 3 / 4};
-        for (int64_t b0 = vbeg[kind]; b0 < vbeg[kind + 1]; ++bi) {
-            const int64_t size = ramp && bi < 4 ? RAMP[bi] : PAIR_BATCH;
-            const int64_t b1 = std::min(vbeg[kind + 1], b0 + size);
-            batches.push_back(Batch{&v, (kind & 1) != 0, (int)kind, b0, b1, bi, 0});
-            b0 = b1;
+    std::vector<int64_t> kind_pairs(nv, 0), kind_batches(nv, 0), last_use(nv * MAX_PAIR_SLOTS, -1);
+    auto add_batches = [&]() {
+        for (size_t kind = 0; kind < nv; ++kind) {
+            if (vbeg[kind + 1] == vbeg[kind]) continue;
+            kind_pairs[kind] += vbeg[kind + 1] - vbeg[kind];
+            const Variant &v = variant_table()[kind / 2];
+            const bool ramp = batches.empty();
+            int bi = 0;
+            for (int64_t b0 = vbeg[kind]; b0 < vbeg[kind + 1]; ++bi) {
+                const int64_t size = ramp && bi < 4 ? RAMP[bi] : PAIR_BATCH;
+                const int64_t b1 = std::min(vbeg[kind + 1], b0 + size);
+                const int slot = (int)(kind_batches[kind]++ % MAX_PAIR_SLOTS);
+                int64_t &last = last_use[kind * MAX_PAIR_SLOTS + slot];
+                batches.push_back(Batch{&v, (kind & 1) != 0, (int)kind, b0, b1, slot, last});
+                last = (int64_t)batches.size() - 1;
+                b0 = b1;
+            }
         }
-        for (size_t i = first; i < batches.size(); ++i) batches[i].nslot = std::min(bi, MAX_PAIR_SLOTS);
-    }
+    };
 
     SatBuffers sb;
     GSW_TRY(prepare_sat(ctx, &sb, 1 << 20));
-    // graphs of every slot, with this call's pointers in their kernel nodes
+    // graphs of the slots a segment's batches use, with this call's pointers in their kernel
+    // nodes (a slot is set up once per call), and timing events: fork and join in marks[0]
+    // and marks[1], batch b's start and end in marks[2 + 2b] and marks[3 + 2b]
     std::map<std::pair<int, int>, PairGraph *> slot_of;
-    for (const Batch &B : batches) {
-        if (B.bi >= B.nslot) continue;
-        PairGraph *pg = nullptr;
-        GSW_TRY(pair_graph(ctx, *B.v, B.tab, B.bi, s.ns, &pg));
-        slot_of[{B.kind, B.bi}] = pg;
-        KArgs a = base_args(s, qs, rs, d_out);
-        a.sat = sb.d_sat;
-        a.sat_count = ctx->counters + 1;
-        a.sat_cap = sb.cap;
-        a.sat_check = (B.v->packed && local) ? 1 : 0;
-        a.items = pg->d_buf + PAIR_HDR;
-        a.nitems_ptr = reinterpret_cast<const int32_t *>(pg->d_buf);
-        a.counter = reinterpret_cast<int32_t *>(pg->d_buf) + 1;
-        void *params[] = {&a};
-        hipKernelNodeParams kp{};
-        a.tab_copies = B.tab ? pairs_tab_copies(*B.v, s.ns) : 1;
-        kp.func = const_cast<void *>(B.tab ? B.v->pairs_tab_fn : B.v->pairs_fn);
-        kp.gridDim = dim3(pg->grid);
-        kp.blockDim = dim3(pg->waves * 64);
-        kp.sharedMemBytes = (unsigned)pg->lds;
-        kp.kernelParams = params;
-        kp.extra = nullptr;
-        GSW_HIP_TRY(hipGraphExecKernelNodeSetParams(pg->exec, pg->kern, &kp));
-    }
+    auto configure = [&](size_t nb0, size_t nb1) -> int {
+        while (ctx->marks.size() < 2 * nb1 + 2) {
+            hipEvent_t e;
+            GSW_HIP_TRY(hipEventCreate(&e));
+            ctx->marks.push_back(e);
+        }
+        for (size_t bb = nb0; bb < nb1; ++bb) {
+            const Batch &B = batches[bb];
+            if (slot_of.count({B.kind, B.slot})) continue;
+            PairGraph *pg = nullptr;
+            GSW_TRY(pair_graph(ctx, *B.v, B.tab, B.slot, s.ns, &pg));
+            slot_of[{B.kind, B.slot}] = pg;
+            KArgs a = base_args(s, qs, rs, d_out);
+            a.sat = sb.d_sat;
+            a.sat_count = ctx->counters + 1;
+            a.sat_cap = sb.cap;
+            a.sat_check = (B.v->packed && local) ? 1 : 0;
+            a.items = pg->d_buf + PAIR_HDR;
+            a.nitems_ptr = reinterpret_cast<const int32_t *>(pg->d_buf);
+            a.counter = reinterpret_cast<int32_t *>(pg->d_buf) + 1;
+            void *params[] = {&a};
+            hipKernelNodeParams kp{};
+            a.tab_copies = B.tab ? pairs_tab_copies(*B.v, s.ns) : 1;
+            kp.func = const_cast<void *>(B.tab ? B.v->pairs_tab_fn : B.v->pairs_fn);
+            kp.gridDim = dim3(pg->grid);
+            kp.blockDim = dim3(pg->waves * 64);
+            kp.sharedMemBytes = (unsigned)pg->lds;
+            kp.kernelParams = params;
+            kp.extra = nullptr;
+            GSW_HIP_TRY(hipGraphExecKernelNodeSetParams(pg->exec, pg->kern, &kp));
+        }
+        return GSW_OK;
+    };
 
     // Plan one batch into its slot's pinned buffer: counting sort by reference length rank
     // (longest first), then (stable) by last-column register when it is tracked; pair
-    // adjacent members of one reference; pad blocks of 64/G tasks at shift changes.
-    struct Scratch { std::vector<int64_t> order, order2; std::vector<int32_t> cnt; };
-    std::vector<double> batch_cells(batches.size(), 0.0);
+    // adjacent members of one reference; pad blocks of 64/G tasks at shift changes.  Each
+    // member's query, reference and shift are gathered once, reading the pair list in index
+    // order, into the sorted records, which the later passes read in sequence.
+    struct Member { int64_t k; int32_t q, r, shift; };
+    struct Scratch {
+        std::vector<Member> a, b;
+        std::vector<int32_t> cnt;
+        std::vector<int32_t> shift_of_len;   // front padding per query length (0..wmax)
+        int shift_r = -1;                    // the R it holds (0: last column not tracked)
+    };
+    std::vector<double> batch_cells;
     auto plan = [&](const Batch &B, PairGraph &g, Scratch &S, double *cells) -> int32_t {
         const Variant &v = *B.v;
         const int gpw = 64 / v.G;
-        double c = 0;
-        for (int64_t i = B.b0; i < B.b1; ++i) c += (double)qs.len[h_qi[members[i]]] * rs.len[h_rj[members[i]]];
-        *cells = c;
+        const size_t nm = (size_t)(B.b1 - B.b0);
+        if (S.shift_r != (track_last_col ? v.R : 0)) {   // a tile's members are at most wmax long
+            S.shift_r = track_last_col ? v.R : 0;
+            S.shift_of_len.assign((size_t)wmax + 1, 0);
+            if (track_last_col)
+                for (int lq = 0; lq <= wmax; ++lq) S.shift_of_len[lq] = (v.R - lq % v.R) % v.R;
+        }
         S.cnt.assign((size_t)rs.n_nonempty + 1, 0);
         for (int64_t i = B.b0; i < B.b1; ++i) S.cnt[rs.rank[h_rj[members[i]]] + 1]++;
         for (int32_t r = 0; r < rs.n_nonempty; ++r) S.cnt[r + 1] += S.cnt[r];
-        S.order.resize(B.b1 - B.b0);
-        for (int64_t i = B.b0; i < B.b1; ++i) S.order[S.cnt[rs.rank[h_rj[members[i]]]]++] = members[i];
-        auto shift_of = [&](int64_t k) { return track_last_col ? (v.R - qs.len[h_qi[k]] % v.R) % v.R : 0; };
+        S.a.resize(nm);
+        double c = 0;
+        for (int64_t i = B.b0; i < B.b1; ++i) {
+            const int64_t k = members[i];
+            const int32_t q = h_qi[k], r = h_rj[k];
+            const int lq = qs.len[q];
+            c += (double)lq * rs.len[r];
+            S.a[S.cnt[rs.rank[r]]++] = Member{k, q, r, S.shift_of_len[lq]};
+        }
+        *cells = c;
         if (track_last_col) {
             S.cnt.assign(v.R + 1, 0);
-            for (int64_t k : S.order) S.cnt[shift_of(k) + 1]++;
+            for (const Member &x : S.a) S.cnt[x.shift + 1]++;
             int distinct = 0;
             for (int r = 1; r <= v.R; ++r) distinct += S.cnt[r] > 0;
             // a second (stable) pass by front padding, unless the whole batch shares one
             if (distinct > 1) {
                 for (int r = 0; r < v.R; ++r) S.cnt[r + 1] += S.cnt[r];
-                S.order2.resize(S.order.size());
-                for (int64_t k : S.order) S.order2[S.cnt[shift_of(k)]++] = k;
-                S.order.swap(S.order2);
+                S.b.resize(nm);
+                for (const Member &x : S.a) S.b[S.cnt[x.shift]++] = x;
+                S.a.swap(S.b);
             }
         }
         PairTask *tasks = reinterpret_cast<PairTask *>(g.h_buf + PAIR_HDR);
         int32_t nt = 0;
         int cur_shift = -1;
-        for (size_t i = 0; i < S.order.size();) {
-            const int64_t ka = S.order[i];
-            const int shift = shift_of(ka);
-            if (shift != cur_shift) {
+        for (size_t i = 0; i < nm;) {
+            const Member &x = S.a[i];
+            if (x.shift != cur_shift) {
                 while (nt % gpw) tasks[nt++] = PairTask{-1, -1, -1, cur_shift, -1, -1};
-                cur_shift = shift;
+                cur_shift = x.shift;
             }
-            if (v.packed && i + 1 < S.order.size() && h_rj[S.order[i + 1]] == h_rj[ka] &&
-                shift_of(S.order[i + 1]) == shift) {
-                const int64_t kb = S.order[i + 1];
-                tasks[nt++] = PairTask{h_qi[ka], h_qi[kb], h_rj[ka], shift, ka, kb};
+            if (v.packed && i + 1 < nm && S.a[i + 1].r == x.r && S.a[i + 1].shift == x.shift) {
+                const Member &y = S.a[i + 1];
+                tasks[nt++] = PairTask{x.q, y.q, x.r, x.shift, x.k, y.k};
                 i += 2;
             } else {
-                tasks[nt++] = PairTask{h_qi[ka], -1, h_rj[ka], shift, ka, -1};
+                tasks[nt++] = PairTask{x.q, -1, x.r, x.shift, x.k, -1};
                 i += 1;
             }
         }
@@ -1588,11 +1632,12 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
         return nt;
     };
 
-    // -- planners (worker threads) and the launching thread.  A planner reuses a slot's
-    //    pinned buffer only after the batch that last used it was launched and finished.
-    const size_t nb = batches.size();
-    std::vector<int> state(nb, 0);   // 0 pending, 1 planned, 2 launched
-    std::vector<int32_t> ntasks(nb, 0);
+    // -- planners (worker threads) and the launching thread of a segment, batches
+    //    [seg_begin, seg_end).  A planner reuses a slot's pinned buffer only after the batch
+    //    that last used it was launched and its copy has read the buffer.
+    std::vector<int> state;   // per batch: 0 pending, 1 planned, 2 launched
+    std::vector<int32_t> ntasks;
+    size_t seg_begin = 0, seg_end = 0;
     std::mutex m;
     std::condition_variable cv;
     bool abort = false;
@@ -1603,34 +1648,23 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
         Scratch S;
         for (;;) {
             const size_t b = next.fetch_add(1);
-            if (b >= nb) return;
+            if (b >= seg_end) return;
             const Batch &B = batches[b];
-            PairGraph &g = *slot_of.at({B.kind, B.bi % B.nslot});
-            if (B.bi >= B.nslot) {
-                const size_t prev = b - B.nslot;   // same tile: its batches are contiguous
+            PairGraph &g = *slot_of.at({B.kind, B.slot});
+            if (B.prev >= 0) {   // launched already when it belongs to an earlier segment
                 std::unique_lock<std::mutex> lk(m);
-                cv.wait(lk, [&] { return state[prev] == 2 || abort; });
+                cv.wait(lk, [&] { return state[B.prev] == 2 || abort; });
                 if (abort) return;
-                lk.unlock();
-                const hipError_t e = hipEventSynchronize(g.copied);
-                if (e != hipSuccess) {
-                    std::lock_guard<std::mutex> lk2(m);
-                    abort = true;
-                    werr = GSW_ERR_HIP;
-                    werr_msg = hipGetErrorString(e);
-                    cv.notify_all();
-                    return;
-                }
-            } else {
-                const hipError_t e = hipEventSynchronize(g.copied);   // an earlier call's copy
-                if (e != hipSuccess) {
-                    std::lock_guard<std::mutex> lk2(m);
-                    abort = true;
-                    werr = GSW_ERR_HIP;
-                    werr_msg = hipGetErrorString(e);
-                    cv.notify_all();
-                    return;
-                }
+            }
+            // the slot's last copy (of this call or an earlier one) has read h_buf
+            const hipError_t e = hipEventSynchronize(g.copied);
+            if (e != hipSuccess) {
+                std::lock_guard<std::mutex> lk2(m);
+                abort = true;
+                werr = GSW_ERR_HIP;
+                werr_msg = hipGetErrorString(e);
+                cv.notify_all();
+                return;
             }
             const int32_t nt = plan(B, g, S, &batch_cells[b]);
             std::lock_guard<std::mutex> lk(m);
@@ -1639,25 +1673,13 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
             cv.notify_all();
         }
     };
-    const int W = (int)std::min<size_t>((size_t)T, nb);
-    while (ctx->marks.size() < 2 * nb + 2) {   // before/after each batch, fork and join
-        hipEvent_t e;
-        GSW_HIP_TRY(hipEventCreate(&e));
-        ctx->marks.push_back(e);
-    }
     int rc = GSW_OK;
     // Batches alternate between the library stream and the second one; on its stream a
     // batch's task copy follows the kernel two batches back, so it overlaps the other
     // stream's kernel, and its kernel takes the CUs as that kernel's last waves finish.
     const hipStream_t streams[2] = {ctx->stream, ctx->stream2};
-    hipEvent_t fork = ctx->marks[2 * nb], join = ctx->marks[2 * nb + 1];
     auto launcher = [&]() {
-        if (hipEventRecord(ctx->ev0, ctx->stream) != hipSuccess) rc = fail(GSW_ERR_HIP, "event record failed");
-        // the second stream starts after the saturation counter's reset
-        if (rc == GSW_OK && (hipEventRecord(fork, ctx->stream) != hipSuccess ||
-                             hipStreamWaitEvent(ctx->stream2, fork, 0) != hipSuccess))
-            rc = fail(GSW_ERR_HIP, "event record failed");
-        for (size_t b = 0; b < nb && rc == GSW_OK; ++b) {
+        for (size_t b = seg_begin; b < seg_end && rc == GSW_OK; ++b) {
             {
                 std::unique_lock<std::mutex> lk(m);
                 cv.wait(lk, [&] { return state[b] >= 1 || abort; });
@@ -1667,7 +1689,7 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
                 }
             }
             const Batch &B = batches[b];
-            PairGraph &g = *slot_of.at({B.kind, B.bi % B.nslot});
+            PairGraph &g = *slot_of.at({B.kind, B.slot});
             // copy once the slot's previous kernel (on either stream) is done with d_buf, then
             // run; planners reuse h_buf once `copied` has passed
             const hipStream_t st = streams[b & 1];
@@ -1676,9 +1698,9 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
                 e = hipMemcpyAsync(g.d_buf, g.h_buf, PAIR_HDR + sizeof(PairTask) * (size_t)ntasks[b],
                                    hipMemcpyHostToDevice, st);
             if (e == hipSuccess) e = hipEventRecord(g.copied, st);
-            if (e == hipSuccess) e = hipEventRecord(ctx->marks[2 * b], st);
+            if (e == hipSuccess) e = hipEventRecord(ctx->marks[2 + 2 * b], st);
             if (e == hipSuccess) e = hipGraphLaunch(g.exec, st);
-            if (e == hipSuccess) e = hipEventRecord(ctx->marks[2 * b + 1], st);
+            if (e == hipSuccess) e = hipEventRecord(ctx->marks[3 + 2 * b], st);
             if (e == hipSuccess) e = hipEventRecord(g.consumed, st);
             if (e != hipSuccess) {
                 rc = fail(GSW_ERR_HIP, "pairs batch launch: %s", hipGetErrorString(e));
@@ -1700,18 +1722,52 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
             cv.notify_all();
         }
     };
-    // the launcher on the calling thread, W planners beside it
-    run_threads(W + 1, [&](int tid) {
-        if (tid == 0)
-            launcher();
-        else
-            worker();
-    });
-    if (rc != GSW_OK) {   // nothing may still run when the caller sees the error
+    // a segment: its graphs and events, then the launcher on the calling thread and up to T
+    // planners beside it
+    auto run_segment = [&]() -> int {
+        state.resize(seg_end, 0);
+        ntasks.resize(seg_end, 0);
+        batch_cells.resize(seg_end, 0.0);
+        GSW_TRY(configure(seg_begin, seg_end));
+        if (seg_end == seg_begin) return GSW_OK;
+        next.store(seg_begin);
+        const int W = (int)std::min<size_t>((size_t)T, seg_end - seg_begin);
+        run_threads(W + 1, [&](int tid) {
+            if (tid == 0)
+                launcher();
+            else
+                worker();
+        });
+        return rc;
+    };
+    auto stop = [&](int code) {   // nothing may still run when the caller sees the error
         (void)hipStreamSynchronize(ctx->stream2);
         (void)hipStreamSynchronize(ctx->stream);
-        return rc;
+        return code;
+    };
+
+    GSW_TRY(classify(0, head));
+    const clk::time_point t_classified = clk::now();
+    add_batches();
+    seg_end = batches.size();
+    GSW_TRY(configure(0, 0));   // fork and join events
+    const hipEvent_t fork = ctx->marks[0], join = ctx->marks[1];
+    // device time from here; the second stream starts after the saturation counter's reset
+    GSW_HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+    GSW_HIP_TRY(hipEventRecord(fork, ctx->stream));
+    GSW_HIP_TRY(hipStreamWaitEvent(ctx->stream2, fork, 0));
+    if (int e = run_segment(); e != GSW_OK) return stop(e);
+    t_head = t_rest = clk::now();
+    if (head < n) {   // the rest, while the device runs the head
+        if (int e = classify(head, n); e != GSW_OK) return stop(e);
+        t_rest = clk::now();
+        seg_begin = seg_end;
+        add_batches();
+        seg_end = batches.size();
+        if (int e = run_segment(); e != GSW_OK) return stop(e);
     }
+    const size_t nb = batches.size();
+    if (nb == 0) t_first = t_rest;
     const clk::time_point t_launched = clk::now();
     GSW_HIP_TRY(hipEventRecord(join, ctx->stream2));
     GSW_HIP_TRY(hipStreamWaitEvent(ctx->stream, join, 0));
@@ -1722,11 +1778,15 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
     res->kernel_ms += ms;
     if (trace) {
         auto d = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
-        std::fprintf(stderr, "[gsw] pairs n=%lld batches=%zu: classify %.2f ms, members +%.2f ms, first launch "
-                             "+%.2f ms, all launched +%.2f ms, device done +%.2f ms (device %.2f ms)\n",
-                     (long long)n, nb, d(t_in, t_classified), d(t_classified, t_members), d(t_members, t_first),
-                     d(t_first, t_launched), d(t_launched, clk::now()), ms);
+        std::fprintf(stderr, "[gsw] pairs n=%lld head=%lld batches=%zu: head classified %.2f ms, first launch +%.2f ms, "
+                             "head launched +%.2f ms, rest classified +%.2f ms, all launched +%.2f ms, device done "
+                             "+%.2f ms (device %.2f ms)\n",
+                     (long long)n, (long long)head, nb, d(t_in, t_classified), d(t_classified, t_first),
+                     d(t_first, t_head), d(t_head, t_rest), d(t_rest, t_launched), d(t_launched, clk::now()), ms);
     }
+    int dominant = -1;   // kind with the most pairs
+    for (size_t kind = 0; kind < nv; ++kind)
+        if (kind_pairs[kind] > 0 && (dominant < 0 || kind_pairs[kind] > kind_pairs[dominant])) dominant = (int)kind;
     res->variant = dominant >= 0 ? dominant / 2 : -1;
     // the dominant tile's full batches (all of its batches if none is full): device time
     // from the first one's start event to the last one's end event, per batch (consecutive
@@ -1742,8 +1802,8 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
             if (batches[b].kind != dominant) continue;
             if (any_full && batches[b].b1 - batches[b].b0 != PAIR_BATCH) continue;
             float t0 = 0.f, t1 = 0.f;
-            GSW_HIP_TRY(hipEventElapsedTime(&t0, ctx->ev0, ctx->marks[2 * b]));
-            GSW_HIP_TRY(hipEventElapsedTime(&t1, ctx->ev0, ctx->marks[2 * b + 1]));
+            GSW_HIP_TRY(hipEventElapsedTime(&t0, ctx->ev0, ctx->marks[2 + 2 * b]));
+            GSW_HIP_TRY(hipEventElapsedTime(&t1, ctx->ev0, ctx->marks[3 + 2 * b]));
             t_lo = std::min(t_lo, (double)t0);
             t_hi = std::max(t_hi, (double)t1);
             sum_cells += batch_cells[b];

@@ -66,6 +66,40 @@ def test_many_batches_two_tiles_slot_reuse():
 
 
 @pytest.mark.parametrize("mode", ["local", "global", "semiglobal"])
+def test_head_and_rest_segments_mixed(mode):
+    """600,000 pairs (a head segment of 344,064 pairs planned before the rest is classified)
+    mixing empty sequences, int16 and int32 pairs of several tiles, and queries longer than
+    any tile (comparator): every pair equals the comparator, a sample equals the oracle, and
+    a shuffled order (other pairs in the head) gives the same score per pair."""
+    rng = np.random.default_rng(23)
+    qlen = rng.integers(1, 600, 2000)
+    qlen[:20] = 0
+    qlen[20:30] = 2100
+    rlen = rng.integers(1, 500, 500)
+    rlen[:5] = 0
+    queries = [rng.integers(0, 4, int(L)).astype(np.uint8) for L in qlen]
+    refs = [rng.integers(0, 4, int(L)).astype(np.uint8) for L in rlen]
+    n = 600_000
+    qi = rng.integers(0, 2000, n).astype(np.int32)
+    rj = rng.integers(0, 500, n).astype(np.int32)
+    # match 100: pairs whose shorter side passes 327 need int32
+    scheme = Scheme(mode=mode, match=100, mismatch=-3, gap_open=-5, gap_extend=-2)
+    al = Aligner(scheme, backend="hip").index(synth.pack(refs)).set_queries(synth.pack(queries))
+    got = al.score_pairs(qi, rj)
+    st = al._b.last_stats
+    assert st.plain_pairs > 0 and st.launches >= 8
+    assert st.cells == int((qlen[qi].astype(np.int64) * rlen[rj].astype(np.int64)).sum())
+    eager = Aligner(scheme, backend="hip-plain").index(synth.pack(refs)).set_queries(
+        synth.pack(queries)).score_pairs(qi, rj)
+    assert np.array_equal(got, eager)
+    pick = rng.choice(n, 2000, replace=False)
+    want = native.score_encoded_pairs(al._queries, al._refs, qi[pick], rj[pick], scheme)
+    assert np.array_equal(got[pick].astype(np.int64), want)
+    perm = rng.permutation(n)
+    assert np.array_equal(al.score_pairs(qi[perm], rj[perm]), got[perm])
+
+
+@pytest.mark.parametrize("mode", ["local", "global", "semiglobal"])
 def test_protein_pairs_table_kernel(mode):
     """Protein queries past 160 columns have no profile tile that fits LDS: the table
     kernel scores them (no comparator), bit-exact."""
@@ -103,10 +137,11 @@ def test_dna_pairs_both_kernels(mode, tab, monkeypatch):
     assert np.array_equal(got.astype(np.int64), want)
 
 
-@pytest.mark.parametrize("n", [1000, 200_000])
+@pytest.mark.parametrize("n", [1000, 200_000, 600_000])
 def test_pair_index_out_of_range_raises_index_error(n):
-    """The library checks pair indices while it plans (single- and multi-threaded) and the
-    host maps its error to IndexError, as indexing the sequence sets would."""
+    """The library checks pair indices while it plans (single- and multi-threaded, and in
+    the rest of a call whose head already runs) and the host maps its error to IndexError,
+    as indexing the sequence sets would."""
     rng = np.random.default_rng(5)
     refs = [rng.integers(0, 4, 50).astype(np.uint8) for _ in range(20)]
     qs = [rng.integers(0, 4, 30).astype(np.uint8) for _ in range(10)]
@@ -114,10 +149,11 @@ def test_pair_index_out_of_range_raises_index_error(n):
     qi = rng.integers(0, 10, n).astype(np.int32)
     rj = rng.integers(0, 20, n).astype(np.int32)
     assert al.score_pairs(qi, rj).shape == (n,)
-    for bad_q, bad_r in ((10, 0), (-1, 0), (0, 20), (0, -3)):
-        q2, r2 = qi.copy(), rj.copy()
-        q2[n // 2], r2[n // 2] = bad_q, bad_r
-        with pytest.raises(IndexError):
-            al.score_pairs(q2, r2)
+    for at in sorted({n // 8, n - 5}):
+        for bad_q, bad_r in ((10, 0), (-1, 0), (0, 20), (0, -3)):
+            q2, r2 = qi.copy(), rj.copy()
+            q2[at], r2[at] = bad_q, bad_r
+            with pytest.raises(IndexError):
+                al.score_pairs(q2, r2)
     # the device state is usable after the errors
     assert np.array_equal(al.score_pairs(qi, rj), al.score_pairs(qi, rj))
