@@ -157,7 +157,7 @@ def _call_traffic(workload: str):
 
 
 def _call_roofline(history, cells: int, kernel_s: float, steps: int) -> dict:
-    """Roofline of a protein top-k call as a whole.  Its 14-odd tile launches run on two
+    """Roofline of a protein top-k call as a whole.  Its tile launches (19) run on two
     streams and overlap (each launch's workgroups take CUs as the other stream's finish), so
     no single launch owns the GPU for its interval: the dominant unit is the call's set of
     batched launches, timed from the first launch's start to the last one's end (HIP events

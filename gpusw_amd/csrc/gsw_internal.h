@@ -38,8 +38,8 @@ struct DeviceCtx {
     int num_cus = 0;
     int max_smem = 0;
     hipStream_t stream = nullptr;
-    // second stream (default priority): pair task copies that overlap kernels, and every
-    // other cross launch so that consecutive launches overlap at their ends
+    // second stream (default priority): every other cross launch and every other pair batch
+    // (its task copy and kernel), so that consecutive launches overlap at their ends
     hipStream_t stream2 = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // grow-only scratch buffers owned by the library
