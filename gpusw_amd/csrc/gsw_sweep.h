@@ -211,7 +211,7 @@ __device__ __forceinline__ SweepOut sweep_group(const uint8_t *__restrict__ rseq
     auto code_at = [&](int r) -> int { return min((int)rseq[r], k.ns - 1); };
     auto load_row = [&](int c, V *dst) {
         if constexpr (TAB) {
-            // c < 32 and trow < 2^24: a 24-bit multiply (full rate)
+            // a code is below 256 and a row fits LDS: a 24-bit multiply (full rate) is exact
             const char *pc = prof_lane + __umul24((unsigned)c, (unsigned)trow);
 #pragma unroll
             for (int kk = 0; kk < R; ++kk) dst[kk] = O::from_bits(*reinterpret_cast<const int *>(pc + toff[kk]));
