@@ -54,6 +54,112 @@ __device__ inline void push_run(uint32_t *ops, int32_t &nops, uint32_t op, int32
     else ops[nops++] = ((uint32_t)len << 2) | op;
 }
 
+// What a fill leaves for the walk: the local best (first strict maximum in row-major
+// order) and its cell, the corner, the best of the last column (first such row) and of
+// the last row (first such column).
+struct TbEnd { int32_t best, bi, bj, corner, ec, eci, er, erj; };
+
+// End cell, score and walk of pair k from its fill (direction bytes in dir, trackers e);
+// writes a.out[k] and the pair's CIGAR runs.
+__device__ void finish_pair(const TbArgs &a, int64_t k, int lq, int lr, const uint8_t *q, const uint8_t *r,
+                            const uint32_t *dir, const TbEnd &e) {
+    const int go = a.go, ge = a.ge, ns = a.ns;
+    const bool local = a.mode == GSW_MODE_LOCAL, semi = a.mode == GSW_MODE_SEMIGLOBAL;
+    const bool ladder_r = a.mode == GSW_MODE_GLOBAL || (semi && !a.free_r);
+    const bool ladder_q = a.mode == GSW_MODE_GLOBAL || (semi && !a.free_q);
+    const int w = (lq + 3) / 4;
+    uint32_t *ops = a.ops + a.ops_off[k];
+    gsw_aln res{};
+    int32_t nops = 0;
+    // ---- end cell and score (kernel.py:141-157 for the score)
+    int ei = 0, ej0 = 0;
+    bool empty = false;
+    if (lq == 0 || lr == 0) {
+        empty = true;   // no aligned columns; the score is the edge formula
+        if (local || (lq == 0 && lr == 0)) res.score = 0;
+        else if (lq == 0) res.score = (semi && a.free_q) ? 0 : go + (lr - 1) * ge;
+        else res.score = (semi && a.free_r) ? 0 : go + (lq - 1) * ge;
+        if (!local) {   // a whole-sequence gap unless it is a free end
+            if (lq == 0 && lr > 0 && !(semi && a.free_q)) { push_run(ops, nops, OP_D, lr, false); res.del = lr; res.gap_opens = 1; }
+            if (lr == 0 && lq > 0 && !(semi && a.free_r)) { push_run(ops, nops, OP_I, lq, false); res.ins = lq; res.gap_opens = 1; }
+            res.q_end = (semi && a.free_r && lr == 0) ? 0 : lq;
+            res.r_end = (semi && a.free_q && lq == 0) ? 0 : lr;
+        }
+    } else if (local) {
+        res.score = e.best;
+        ei = e.bi; ej0 = e.bj;
+        empty = e.best == 0;
+    } else if (!semi) {
+        res.score = e.corner;
+        ei = lr; ej0 = lq;
+    } else {
+        int sc = e.corner;
+        ei = lr; ej0 = lq;
+        if (a.free_q && e.ec > sc) { sc = e.ec; ei = e.eci; ej0 = lq; }
+        if (a.free_r && e.er > sc) { sc = e.er; ei = lr; ej0 = e.erj; }
+        if (a.free_q && a.free_r && 0 > sc) { sc = 0; empty = true; }
+        res.score = sc;
+    }
+    // ---- walk back from (ei, ej0)
+    if (!empty) {
+        res.q_end = ej0;
+        res.r_end = ei;
+        int i = ei, j = ej0, state = 0;   // 0 = H, 1 = E, 2 = F
+        bool cont = false;                // the next gap step continues the current gap
+        for (;;) {
+            if (state == 0) {
+                if (i == 0 || j == 0) {
+                    if (i == 0 && j > 0) {
+                        if (ladder_r) { push_run(ops, nops, OP_I, j, false); res.ins += j; res.gap_opens++; j = 0; }
+                    } else if (j == 0 && i > 0) {
+                        if (ladder_q) { push_run(ops, nops, OP_D, i, false); res.del += i; res.gap_opens++; i = 0; }
+                    }
+                    break;
+                }
+                const uint32_t bits = (dir[(int64_t)(i - 1) * w + (j - 1) / 4] >> (8 * ((j - 1) & 3))) & 0xFFu;
+                const uint32_t src = bits & 3u;
+                if (src == 0) break;   // local start
+                if (src == 1) {
+                    const bool eq = min((int)q[j - 1], ns - 1) == min((int)r[i - 1], ns - 1) &&
+                                    (int)q[j - 1] < ns - 1;
+                    push_run(ops, nops, eq ? OP_EQ : OP_X, 1, true);
+                    if (eq) res.matches++; else res.mismatches++;
+                    --i; --j;
+                } else {
+                    state = src == 2 ? 1 : 2;
+                    res.gap_opens++;
+                    cont = false;
+                }
+            } else if (state == 1) {   // E: query residue j against a gap
+                const uint32_t bits = (dir[(int64_t)(i - 1) * w + (j - 1) / 4] >> (8 * ((j - 1) & 3))) & 0xFFu;
+                push_run(ops, nops, OP_I, 1, cont);
+                res.ins++;
+                cont = (bits & 4u) != 0;
+                if (!cont) state = 0;
+                --j;
+            } else {                   // F: reference residue i against a gap
+                const uint32_t bits = (dir[(int64_t)(i - 1) * w + (j - 1) / 4] >> (8 * ((j - 1) & 3))) & 0xFFu;
+                push_run(ops, nops, OP_D, 1, cont);
+                res.del++;
+                cont = (bits & 8u) != 0;
+                if (!cont) state = 0;
+                --i;
+            }
+        }
+        res.q_begin = j;
+        res.r_begin = i;
+        // runs were pushed end to start
+        for (int x = 0, y = nops - 1; x < y; ++x, --y) {
+            const uint32_t t = ops[x];
+            ops[x] = ops[y];
+            ops[y] = t;
+        }
+    }
+    res.n_ops = nops;
+    res.ops_offset = a.ops_off[k];
+    a.out[k] = res;
+}
+
 __global__ void __launch_bounds__(256) traceback_kernel(TbArgs a) {
     __shared__ int32_t s_sub[64 * 64];
     for (int i = threadIdx.x; i < a.ns * a.ns; i += blockDim.x) s_sub[i] = a.sub[i];
@@ -72,9 +178,6 @@ __global__ void __launch_bounds__(256) traceback_kernel(TbArgs a) {
         const uint8_t *q = a.qcodes + qs, *r = a.rcodes + rs;
         uint32_t *dir = a.dir + a.dir_off[k] / 4;
         const int w = (lq + 3) / 4;   // words per direction row
-        uint32_t *ops = a.ops + a.ops_off[k];
-        gsw_aln res{};
-        int32_t nops = 0;
         // ---- fill
         int best = 0, bi = 0, bj = 0;                                   // local
         int corner = NEG32, ec = NEG32, eci = 0, er = NEG32, erj = 0;   // semiglobal / global
@@ -121,94 +224,131 @@ __global__ void __launch_bounds__(256) traceback_kernel(TbArgs a) {
                 eleft = ej;
             }
         }
-        // ---- end cell and score (kernel.py:141-157 for the score)
-        int ei = 0, ej0 = 0;
-        bool empty = false;
-        if (lq == 0 || lr == 0) {
-            empty = true;   // no aligned columns; the score is the edge formula
-            if (local || (lq == 0 && lr == 0)) res.score = 0;
-            else if (lq == 0) res.score = (semi && a.free_q) ? 0 : go + (lr - 1) * ge;
-            else res.score = (semi && a.free_r) ? 0 : go + (lq - 1) * ge;
-            if (!local) {   // a whole-sequence gap unless it is a free end
-                if (lq == 0 && lr > 0 && !(semi && a.free_q)) { push_run(ops, nops, OP_D, lr, false); res.del = lr; res.gap_opens = 1; }
-                if (lr == 0 && lq > 0 && !(semi && a.free_r)) { push_run(ops, nops, OP_I, lq, false); res.ins = lq; res.gap_opens = 1; }
-                res.q_end = (semi && a.free_r && lr == 0) ? 0 : lq;
-                res.r_end = (semi && a.free_q && lq == 0) ? 0 : lr;
-            }
-        } else if (local) {
-            res.score = best;
-            ei = bi; ej0 = bj;
-            empty = best == 0;
-        } else if (!semi) {
-            res.score = corner;
-            ei = lr; ej0 = lq;
-        } else {
-            int sc = corner;
-            ei = lr; ej0 = lq;
-            if (a.free_q && ec > sc) { sc = ec; ei = eci; ej0 = lq; }
-            if (a.free_r && er > sc) { sc = er; ei = lr; ej0 = erj; }
-            if (a.free_q && a.free_r && 0 > sc) { sc = 0; empty = true; }
-            res.score = sc;
+        finish_pair(a, k, lq, lr, q, r, dir, TbEnd{best, bi, bj, corner, ec, eci, er, erj});
+    }
+}
+
+// Wave-parallel fill for queries of at most TB_G * R columns: TB_G lanes per pair, lane t
+// owns query columns [tR, tR + R) and processes row s - t + 1 at step s (a diagonal
+// pipeline), taking H and E of the column left of its block, for that row, from lane t - 1
+// (one step earlier) and H of the row above from the step before.  Each cell gets the
+// values and direction byte of the one-thread fill above, and each lane keeps the trackers
+// of its columns in the same scan order; the group reduces them with ties to the first
+// cell in row-major order (the one-thread fill's order), so the walk (traceback_walk_kernel)
+// starts from the same end cell.
+constexpr int TB_G = 16;
+
+template <int R>
+__global__ void __launch_bounds__(256) traceback_fill_kernel(TbArgs a, TbEnd *ends) {
+    static_assert(R % 4 == 0, "direction words hold four cells");
+    __shared__ int32_t s_sub[64 * 64];
+    for (int i = threadIdx.x; i < a.ns * a.ns; i += blockDim.x) s_sub[i] = a.sub[i];
+    __syncthreads();
+    const int t = (int)(threadIdx.x % TB_G);
+    const int64_t k = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / TB_G;
+    const int go = a.go, ge = a.ge, ns = a.ns;
+    const bool local = a.mode == GSW_MODE_LOCAL, semi = a.mode == GSW_MODE_SEMIGLOBAL;
+    const bool ladder_r = a.mode == GSW_MODE_GLOBAL || (semi && !a.free_r);
+    const bool ladder_q = a.mode == GSW_MODE_GLOBAL || (semi && !a.free_q);
+    int lq = 0, lr = 0;
+    const uint8_t *q = a.qcodes, *r = a.rcodes;
+    uint32_t *dir = a.dir;
+    if (k < a.n) {   // groups past the last pair idle with lq = lr = 0
+        const int qs = a.qoff[a.qi[k]], rs = a.roff[a.rj[k]];
+        lq = a.qoff[a.qi[k] + 1] - qs;
+        lr = a.roff[a.rj[k] + 1] - rs;
+        q += qs;
+        r += rs;
+        dir += a.dir_off[k] / 4;
+    }
+    const int w = (lq + 3) / 4;
+    const int j0 = t * R;   // columns j0 + 1 .. j0 + R (1-based)
+    int qc[R], hup[R], fu[R];
+#pragma unroll
+    for (int kk = 0; kk < R; ++kk) {
+        const int j = j0 + kk + 1;
+        qc[kk] = j <= lq ? min((int)q[j - 1], ns - 1) : 0;
+        hup[kk] = (j <= lq && ladder_r) ? go + (j - 1) * ge : 0;
+        fu[kk] = NEG32;
+    }
+    TbEnd e{0, 0, 0, NEG32, NEG32, 0, NEG32, 0};
+    int hin = 0, ein = NEG32, din = 0;   // H(i, j0), E(i, j0), H(i-1, j0) from the left
+    int nsteps = lr > 0 ? lr + TB_G - 1 : 0;
+    for (int off = 32; off >= 1; off >>= 1) nsteps = max(nsteps, __shfl_xor(nsteps, off, 64));
+    for (int s = 0; s < nsteps; ++s) {
+        const int i = s - t + 1;
+        if (t == 0) {   // column 0
+            hin = ladder_q ? go + (i - 1) * ge : 0;
+            din = i <= 1 ? 0 : (ladder_q ? go + (i - 2) * ge : 0);
+            ein = NEG32;
+        } else if (i == 1) {   // row 0 of column j0 (lane t - 1 was idle on it)
+            din = ladder_r ? go + (j0 - 1) * ge : 0;
         }
-        // ---- walk back from (ei, ej0)
-        if (!empty) {
-            res.q_end = ej0;
-            res.r_end = ei;
-            int i = ei, j = ej0, state = 0;   // 0 = H, 1 = E, 2 = F
-            bool cont = false;                // the next gap step continues the current gap
-            for (;;) {
-                if (state == 0) {
-                    if (i == 0 || j == 0) {
-                        if (i == 0 && j > 0) {
-                            if (ladder_r) { push_run(ops, nops, OP_I, j, false); res.ins += j; res.gap_opens++; j = 0; }
-                        } else if (j == 0 && i > 0) {
-                            if (ladder_q) { push_run(ops, nops, OP_D, i, false); res.del += i; res.gap_opens++; i = 0; }
-                        }
-                        break;
-                    }
-                    const uint32_t bits = (dir[(int64_t)(i - 1) * w + (j - 1) / 4] >> (8 * ((j - 1) & 3))) & 0xFFu;
-                    const uint32_t src = bits & 3u;
-                    if (src == 0) break;   // local start
-                    if (src == 1) {
-                        const bool eq = min((int)q[j - 1], ns - 1) == min((int)r[i - 1], ns - 1) &&
-                                        (int)q[j - 1] < ns - 1;
-                        push_run(ops, nops, eq ? OP_EQ : OP_X, 1, true);
-                        if (eq) res.matches++; else res.mismatches++;
-                        --i; --j;
+        int hout = 0, eout = NEG32;
+        if (i >= 1 && i <= lr && j0 < lq) {
+            const int32_t *srow = s_sub + min((int)r[i - 1], ns - 1) * ns;
+            int hleft = hin, eleft = ein, diag = din;
+            uint32_t word = 0;
+#pragma unroll
+            for (int kk = 0; kk < R; ++kk) {
+                const int j = j0 + kk + 1;
+                if (j <= lq) {
+                    const int fo = hup[kk] + go, fx = fu[kk] + ge;
+                    const int eo = hleft + go, ex = eleft + ge;
+                    const int fj = max(fo, fx), ej = max(eo, ex);
+                    const int d = diag + srow[qc[kk]];
+                    int h = max(max(d, ej), fj);
+                    uint32_t src;
+                    if (local && h <= 0) { h = 0; src = 0; }
+                    else src = h == d ? 1u : h == ej ? 2u : 3u;
+                    word |= (src | (ex > eo ? 4u : 0u) | (fx > fo ? 8u : 0u)) << (8 * (kk & 3));
+                    if (local) {
+                        if (h > e.best) { e.best = h; e.bi = i; e.bj = j; }
                     } else {
-                        state = src == 2 ? 1 : 2;
-                        res.gap_opens++;
-                        cont = false;
+                        if (i == lr && j == lq) e.corner = h;
+                        if (j == lq && h > e.ec) { e.ec = h; e.eci = i; }
+                        if (i == lr && h > e.er) { e.er = h; e.erj = j; }
                     }
-                } else if (state == 1) {   // E: query residue j against a gap
-                    const uint32_t bits = (dir[(int64_t)(i - 1) * w + (j - 1) / 4] >> (8 * ((j - 1) & 3))) & 0xFFu;
-                    push_run(ops, nops, OP_I, 1, cont);
-                    res.ins++;
-                    cont = (bits & 4u) != 0;
-                    if (!cont) state = 0;
-                    --j;
-                } else {                   // F: reference residue i against a gap
-                    const uint32_t bits = (dir[(int64_t)(i - 1) * w + (j - 1) / 4] >> (8 * ((j - 1) & 3))) & 0xFFu;
-                    push_run(ops, nops, OP_D, 1, cont);
-                    res.del++;
-                    cont = (bits & 8u) != 0;
-                    if (!cont) state = 0;
-                    --i;
+                    diag = hup[kk];
+                    hup[kk] = h;
+                    fu[kk] = fj;
+                    hleft = h;
+                    eleft = ej;
+                }
+                if ((kk & 3) == 3) {
+                    const int widx = (j0 + kk) / 4;
+                    if (widx < w) dir[(int64_t)(i - 1) * w + widx] = word;
+                    word = 0;
                 }
             }
-            res.q_begin = j;
-            res.r_begin = i;
-            // runs were pushed end to start
-            for (int x = 0, y = nops - 1; x < y; ++x, --y) {
-                const uint32_t t = ops[x];
-                ops[x] = ops[y];
-                ops[y] = t;
-            }
+            hout = hleft;
+            eout = eleft;
         }
-        res.n_ops = nops;
-        res.ops_offset = a.ops_off[k];
-        a.out[k] = res;
+        din = hin;   // H(i, j0) is the next row's diagonal
+        hin = __shfl_up(hout, 1, TB_G);
+        ein = __shfl_up(eout, 1, TB_G);
     }
+    // group reduction; ties go to the first cell in row-major order
+#pragma unroll
+    for (int off = TB_G / 2; off >= 1; off >>= 1) {
+        const int ob = __shfl_xor(e.best, off, TB_G), oi = __shfl_xor(e.bi, off, TB_G), oj = __shfl_xor(e.bj, off, TB_G);
+        if (ob > e.best || (ob == e.best && (oi < e.bi || (oi == e.bi && oj < e.bj)))) { e.best = ob; e.bi = oi; e.bj = oj; }
+        e.corner = max(e.corner, __shfl_xor(e.corner, off, TB_G));
+        const int oec = __shfl_xor(e.ec, off, TB_G), oeci = __shfl_xor(e.eci, off, TB_G);
+        if (oec > e.ec) { e.ec = oec; e.eci = oeci; }   // one lane holds column lq
+        const int oer = __shfl_xor(e.er, off, TB_G), oerj = __shfl_xor(e.erj, off, TB_G);
+        if (oer > e.er || (oer == e.er && oerj < e.erj)) { e.er = oer; e.erj = oerj; }
+    }
+    if (t == 0 && k < a.n) ends[k] = e;
+}
+
+// The walk of each pair after traceback_fill_kernel, one thread per pair.
+__global__ void __launch_bounds__(256) traceback_walk_kernel(TbArgs a, const TbEnd *ends) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a.n) return;
+    const int qs = a.qoff[a.qi[k]], lq = a.qoff[a.qi[k] + 1] - qs;
+    const int rs = a.roff[a.rj[k]], lr = a.roff[a.rj[k] + 1] - rs;
+    finish_pair(a, k, lq, lr, a.qcodes + qs, a.rcodes + rs, a.dir + a.dir_off[k] / 4, ends[k]);
 }
 
 }  // namespace gsw
@@ -254,6 +394,12 @@ extern "C" int gsw_traceback_pairs(const gsw_scheme *s, gsw_seqset queries, gsw_
     // launches of pairs whose direction regions total <= 1 GiB (a single larger pair alone)
     const int64_t dir_cap = (int64_t)1 << 30;
     const int threads = 256;
+    // queries up to TB_G * 64 columns: the wave-parallel fill with the smallest block of
+    // columns per lane that covers the longest query, then the walk; else one thread per pair
+    static constexpr int TB_R[] = {4, 8, 12, 16, 20, 24, 32, 48, 64};
+    int tb_r = 0;
+    for (int rr : TB_R)
+        if (!tb_r && TB_G * rr >= maxq) tb_r = rr;
     float total_ms = 0.f;
     int launches = 0;
     for (int64_t k0 = 0; k0 < n;) {
@@ -263,12 +409,12 @@ extern "C" int gsw_traceback_pairs(const gsw_scheme *s, gsw_seqset queries, gsw_
         const int64_t dir_bytes = dir_off[k1] - dir_off[k0];
         int64_t nthr = std::min<int64_t>(m, (int64_t)ctx->num_cus * 1024);
         nthr = (nthr + threads - 1) / threads * threads;
-        const size_t ws_bytes = 2 * sizeof(int32_t) * (size_t)(maxq + 1) * nthr;
+        const size_t ws_bytes = tb_r ? sizeof(TbEnd) * (size_t)m : 2 * sizeof(int32_t) * (size_t)(maxq + 1) * nthr;
         const size_t meta_bytes = 2 * sizeof(int32_t) * m + 2 * sizeof(int64_t) * m;
         const size_t out_bytes = sizeof(gsw_aln) * m + sizeof(uint32_t) * (ops_off[k1] - ops_off[k0]);
         GSW_TRY(ensure_buffer(&ctx->plain_ws, &ctx->plain_ws_bytes, ws_bytes + (size_t)dir_bytes + meta_bytes + out_bytes + 1024));
         char *base = static_cast<char *>(ctx->plain_ws);
-        int32_t *d_ws = reinterpret_cast<int32_t *>(base);
+        int32_t *d_ws = reinterpret_cast<int32_t *>(base);   // H and F rows, or the fill's trackers
         uint32_t *d_dir = reinterpret_cast<uint32_t *>(base + ws_bytes);
         char *mp = base + ws_bytes + ((dir_bytes + 255) / 256) * 256;
         int32_t *d_qi = reinterpret_cast<int32_t *>(mp);
@@ -293,7 +439,24 @@ extern "C" int gsw_traceback_pairs(const gsw_scheme *s, gsw_seqset queries, gsw_
         a.free_q = sd.free_q; a.free_r = sd.free_r;
         a.dir = d_dir; a.ws = d_ws; a.stride = nthr; a.maxq = maxq; a.out = d_out; a.ops = d_ops;
         GSW_HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
-        hipLaunchKernelGGL(traceback_kernel, dim3((unsigned)(nthr / threads)), dim3(threads), 0, ctx->stream, a);
+        if (tb_r) {
+            TbEnd *d_ends = reinterpret_cast<TbEnd *>(base);
+            const unsigned fill_blocks = (unsigned)((m * TB_G + threads - 1) / threads);
+            switch (tb_r) {
+#define GSW_TB_FILL(RR)                                                                                          \
+    case RR:                                                                                                     \
+        hipLaunchKernelGGL(traceback_fill_kernel<RR>, dim3(fill_blocks), dim3(threads), 0, ctx->stream, a, d_ends); \
+        break;
+                GSW_TB_FILL(4) GSW_TB_FILL(8) GSW_TB_FILL(12) GSW_TB_FILL(16) GSW_TB_FILL(20)
+                GSW_TB_FILL(24) GSW_TB_FILL(32) GSW_TB_FILL(48) GSW_TB_FILL(64)
+#undef GSW_TB_FILL
+            }
+            GSW_HIP_TRY(hipGetLastError());
+            hipLaunchKernelGGL(traceback_walk_kernel, dim3((unsigned)((m + threads - 1) / threads)), dim3(threads), 0,
+                               ctx->stream, a, d_ends);
+        } else {
+            hipLaunchKernelGGL(traceback_kernel, dim3((unsigned)(nthr / threads)), dim3(threads), 0, ctx->stream, a);
+        }
         GSW_HIP_TRY(hipGetLastError());
         GSW_HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
         GSW_HIP_TRY(hipMemcpyAsync(h_out + k0, d_out, sizeof(gsw_aln) * m, hipMemcpyDeviceToHost, ctx->stream));

@@ -83,6 +83,26 @@ def test_long_protein_pairs():
     assert got[4].identity == 1.0 and got[4].length == 600   # base[k][100:700] inside base[k][50:]
 
 
+@pytest.mark.parametrize("qlen", [1, 63, 64, 65, 257, 1024, 1025])
+@pytest.mark.parametrize("mode", ["local", "global", "semiglobal"])
+def test_fill_paths_at_their_limits(qlen, mode):
+    """Queries up to 16 x 64 columns take the wave-parallel fill (16 lanes per pair, the
+    smallest block of columns per lane that covers the longest query); longer ones the
+    one-thread fill.  Both equal the oracle at the lengths where the block size changes."""
+    rng = np.random.default_rng(qlen * 7 + len(mode))
+    scheme = Scheme.preset("dna").replace(mode=mode)
+    base = ["".join(rng.choice(list("ACGT"), qlen)) for _ in range(3)]
+    queries = base + [b[: max(1, qlen // 3)] for b in base]
+    refs = [b[max(0, qlen // 2 - 40): qlen // 2 + 40] + "".join(rng.choice(list("ACGT"), 30)) for b in base] \
+        + rand_seqs(rng, 3, 20, 120, "ACGT")
+    al = Aligner(scheme, backend="hip").index(refs).set_queries(queries)
+    qi = np.repeat(np.arange(len(queries)), 2).astype(np.int32)
+    rj = np.tile([0, 4], len(queries)).astype(np.int32)
+    rj[::3] = np.arange(len(rj[::3])) % len(refs)
+    got = al.align(qi, rj)
+    _compare(got, _oracle(al, qi, rj))
+
+
 def test_top_k_alignments_and_cli(tmp_path):
     rng = np.random.default_rng(33)
     refs = [(f"r{i}", s) for i, s in enumerate(rand_seqs(rng, 60, 80, 200, "ACGT"))]
