@@ -201,7 +201,7 @@ class Aligner:
         with self._b.device_scope(self.device):
             recs, ops, stats = run_traceback(self.scheme, self._d_qbuf, self._d_qoff, self._d_rbuf,
                                              self._d_roff, qi, rj)
-        self._record(int(stats.cells), stats.kernel_ms / 1e3, "traceback (one thread per pair, int32)")
+        self._record(int(stats.cells), stats.kernel_ms / 1e3, "traceback (int32 fill, walk per pair)")
         return [Alignment(self._queries.ids[int(q)], self._refs.ids[int(r)], a.score, a.q_begin, a.q_end,
                           a.r_begin, a.r_end, _cigar(ops, a.ops_offset, a.n_ops), a.matches, a.mismatches,
                           a.ins, a.del_, a.gap_opens)
