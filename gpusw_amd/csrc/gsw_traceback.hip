@@ -236,11 +236,10 @@ __global__ void __launch_bounds__(256) traceback_kernel(TbArgs a) {
 // of its columns in the same scan order; the group reduces them with ties to the first
 // cell in row-major order (the one-thread fill's order), so the walk (traceback_walk_kernel)
 // starts from the same end cell.
-constexpr int TB_G = 16;
-
-template <int R>
+template <int TB_G, int R>
 __global__ void __launch_bounds__(256) traceback_fill_kernel(TbArgs a, TbEnd *ends) {
     static_assert(R % 4 == 0, "direction words hold four cells");
+    static_assert(TB_G == 16 || TB_G == 32, "lane groups of 16 or 32");
     __shared__ int32_t s_sub[64 * 64];
     for (int i = threadIdx.x; i < a.ns * a.ns; i += blockDim.x) s_sub[i] = a.sub[i];
     __syncthreads();
@@ -394,12 +393,16 @@ extern "C" int gsw_traceback_pairs(const gsw_scheme *s, gsw_seqset queries, gsw_
     // launches of pairs whose direction regions total <= 1 GiB (a single larger pair alone)
     const int64_t dir_cap = (int64_t)1 << 30;
     const int threads = 256;
-    // queries up to TB_G * 64 columns: the wave-parallel fill with the smallest block of
-    // columns per lane that covers the longest query, then the walk; else one thread per pair
-    static constexpr int TB_R[] = {4, 8, 12, 16, 20, 24, 32, 48, 64};
-    int tb_r = 0;
-    for (int rr : TB_R)
-        if (!tb_r && TB_G * rr >= maxq) tb_r = rr;
+    // queries up to 2,048 columns: the wave-parallel fill (16 lanes per pair up to 1,024
+    // columns, else 32) with the smallest block of columns per lane that covers the longest
+    // query, then the walk; longer queries: one thread per pair
+    static constexpr int TB_R16[] = {4, 8, 12, 16, 20, 24, 32, 48, 64};
+    static constexpr int TB_R32[] = {40, 48, 64};
+    int tb_g = 0, tb_r = 0;
+    for (int rr : TB_R16)
+        if (!tb_r && 16 * rr >= maxq) { tb_g = 16; tb_r = rr; }
+    for (int rr : TB_R32)
+        if (!tb_r && 32 * rr >= maxq) { tb_g = 32; tb_r = rr; }
     float total_ms = 0.f;
     int launches = 0;
     for (int64_t k0 = 0; k0 < n;) {
@@ -441,14 +444,16 @@ extern "C" int gsw_traceback_pairs(const gsw_scheme *s, gsw_seqset queries, gsw_
         GSW_HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
         if (tb_r) {
             TbEnd *d_ends = reinterpret_cast<TbEnd *>(base);
-            const unsigned fill_blocks = (unsigned)((m * TB_G + threads - 1) / threads);
-            switch (tb_r) {
-#define GSW_TB_FILL(RR)                                                                                          \
-    case RR:                                                                                                     \
-        hipLaunchKernelGGL(traceback_fill_kernel<RR>, dim3(fill_blocks), dim3(threads), 0, ctx->stream, a, d_ends); \
+            const unsigned fill_blocks = (unsigned)((m * tb_g + threads - 1) / threads);
+            switch (tb_g * 1000 + tb_r) {
+#define GSW_TB_FILL(GG, RR)                                                                                          \
+    case GG * 1000 + RR:                                                                                             \
+        hipLaunchKernelGGL((traceback_fill_kernel<GG, RR>), dim3(fill_blocks), dim3(threads), 0, ctx->stream, a,     \
+                           d_ends);                                                                                  \
         break;
-                GSW_TB_FILL(4) GSW_TB_FILL(8) GSW_TB_FILL(12) GSW_TB_FILL(16) GSW_TB_FILL(20)
-                GSW_TB_FILL(24) GSW_TB_FILL(32) GSW_TB_FILL(48) GSW_TB_FILL(64)
+                GSW_TB_FILL(16, 4) GSW_TB_FILL(16, 8) GSW_TB_FILL(16, 12) GSW_TB_FILL(16, 16) GSW_TB_FILL(16, 20)
+                GSW_TB_FILL(16, 24) GSW_TB_FILL(16, 32) GSW_TB_FILL(16, 48) GSW_TB_FILL(16, 64)
+                GSW_TB_FILL(32, 40) GSW_TB_FILL(32, 48) GSW_TB_FILL(32, 64)
 #undef GSW_TB_FILL
             }
             GSW_HIP_TRY(hipGetLastError());

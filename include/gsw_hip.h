@@ -161,7 +161,8 @@ int gsw_list_stripe_variants(int32_t *gr, int32_t cap, int32_t *n);
 /* ---- traceback ------------------------------------------------------------
  * No counterpart in the reference, which scores only and re-aligns its top hits on the
  * CPU (docs/DESIGN.md:94-96, examples/legacy_16s_gpu_align.py:280-292).  For each pair
- * (h_qi[k], h_rj[k]) the alignment behind its score, in int32, one thread per pair:
+ * (h_qi[k], h_rj[k]) the alignment behind its score, in int32 (16 or 32 lanes fill a
+ * pair's matrix for queries up to 2,048 columns, one thread beyond; one thread walks it):
  * coordinates are 0-based half-open; free end gaps (semiglobal) are left out of the CIGAR
  * and show in the coordinates; h_ops receives the CIGAR runs of pair k at
  * h_ops[out[k].ops_offset ..+ n_ops), each (length << 2) | op with op 0 '=' (identical

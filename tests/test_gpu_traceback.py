@@ -83,12 +83,13 @@ def test_long_protein_pairs():
     assert got[4].identity == 1.0 and got[4].length == 600   # base[k][100:700] inside base[k][50:]
 
 
-@pytest.mark.parametrize("qlen", [1, 63, 64, 65, 257, 1024, 1025])
+@pytest.mark.parametrize("qlen", [1, 63, 64, 65, 257, 1024, 1025, 2048, 2049])
 @pytest.mark.parametrize("mode", ["local", "global", "semiglobal"])
 def test_fill_paths_at_their_limits(qlen, mode):
-    """Queries up to 16 x 64 columns take the wave-parallel fill (16 lanes per pair, the
-    smallest block of columns per lane that covers the longest query); longer ones the
-    one-thread fill.  Both equal the oracle at the lengths where the block size changes."""
+    """Queries up to 2,048 columns take the wave-parallel fill (16 lanes per pair up to
+    1,024 columns, else 32, with the smallest block of columns per lane that covers the
+    longest query); longer ones the one-thread fill.  All equal the oracle at the lengths
+    where the lane count or block size changes."""
     rng = np.random.default_rng(qlen * 7 + len(mode))
     scheme = Scheme.preset("dna").replace(mode=mode)
     base = ["".join(rng.choice(list("ACGT"), qlen)) for _ in range(3)]
