@@ -45,14 +45,29 @@ struct TbArgs {
 
 constexpr int NEG32 = -(1 << 30);   // the oracle's sentinel
 
-// Appends a run (the walk goes end to start).  Residue columns always merge with the run
-// before; a gap merges only while it continues (merge), so two separate gaps that happen
-// to touch stay two runs and the CIGAR's affine cost is the score.
-__device__ inline void push_run(uint32_t *ops, int32_t &nops, uint32_t op, int32_t len, bool merge) {
-    if (len <= 0) return;
-    if (merge && nops > 0 && (ops[nops - 1] & 3u) == op) ops[nops - 1] += (uint32_t)len << 2;
-    else ops[nops++] = ((uint32_t)len << 2) | op;
-}
+// Appends runs (the walk goes end to start), holding the last one in a register so that
+// extending it costs no memory access.  Residue columns always merge with the run before;
+// a gap merges only while it continues (merge), so two separate gaps that happen to touch
+// stay two runs and the CIGAR's affine cost is the score.
+struct RunSink {
+    uint32_t *ops;
+    int32_t nops = 0;   // runs stored
+    uint32_t cur = 0;   // the last run, (length << 2) | op; 0: none yet
+    __device__ void push(uint32_t op, int32_t len, bool merge) {
+        if (len <= 0) return;
+        if (merge && cur && (cur & 3u) == op) {
+            cur += (uint32_t)len << 2;
+            return;
+        }
+        if (cur) ops[nops++] = cur;
+        cur = ((uint32_t)len << 2) | op;
+    }
+    __device__ int32_t finish() {
+        if (cur) ops[nops++] = cur;
+        cur = 0;
+        return nops;
+    }
+};
 
 // What a fill leaves for the walk: the local best (first strict maximum in row-major
 // order) and its cell, the corner, the best of the last column (first such row) and of
@@ -68,9 +83,8 @@ __device__ void finish_pair(const TbArgs &a, int64_t k, int lq, int lr, const ui
     const bool ladder_r = a.mode == GSW_MODE_GLOBAL || (semi && !a.free_r);
     const bool ladder_q = a.mode == GSW_MODE_GLOBAL || (semi && !a.free_q);
     const int w = (lq + 3) / 4;
-    uint32_t *ops = a.ops + a.ops_off[k];
+    RunSink sink{a.ops + a.ops_off[k]};
     gsw_aln res{};
-    int32_t nops = 0;
     // ---- end cell and score (kernel.py:141-157 for the score)
     int ei = 0, ej0 = 0;
     bool empty = false;
@@ -80,8 +94,8 @@ __device__ void finish_pair(const TbArgs &a, int64_t k, int lq, int lr, const ui
         else if (lq == 0) res.score = (semi && a.free_q) ? 0 : go + (lr - 1) * ge;
         else res.score = (semi && a.free_r) ? 0 : go + (lq - 1) * ge;
         if (!local) {   // a whole-sequence gap unless it is a free end
-            if (lq == 0 && lr > 0 && !(semi && a.free_q)) { push_run(ops, nops, OP_D, lr, false); res.del = lr; res.gap_opens = 1; }
-            if (lr == 0 && lq > 0 && !(semi && a.free_r)) { push_run(ops, nops, OP_I, lq, false); res.ins = lq; res.gap_opens = 1; }
+            if (lq == 0 && lr > 0 && !(semi && a.free_q)) { sink.push(OP_D, lr, false); res.del = lr; res.gap_opens = 1; }
+            if (lr == 0 && lq > 0 && !(semi && a.free_r)) { sink.push(OP_I, lq, false); res.ins = lq; res.gap_opens = 1; }
             res.q_end = (semi && a.free_r && lr == 0) ? 0 : lq;
             res.r_end = (semi && a.free_q && lq == 0) ? 0 : lr;
         }
@@ -106,40 +120,56 @@ __device__ void finish_pair(const TbArgs &a, int64_t k, int lq, int lr, const ui
         res.r_end = ei;
         int i = ei, j = ej0, state = 0;   // 0 = H, 1 = E, 2 = F
         bool cont = false;                // the next gap step continues the current gap
+        // Diagonal steps are the common ones and each is a dependent read of a direction
+        // byte (an HBM miss, rows lie lq bytes apart): the words of the next LA cells on
+        // the diagonal are loaded together, la[d] for cell (i - d, j - d), d < la_n.
+        constexpr int LA = 8;
+        uint32_t la[LA];
+        int la_n = 0;
         for (;;) {
             if (state == 0) {
                 if (i == 0 || j == 0) {
                     if (i == 0 && j > 0) {
-                        if (ladder_r) { push_run(ops, nops, OP_I, j, false); res.ins += j; res.gap_opens++; j = 0; }
+                        if (ladder_r) { sink.push(OP_I, j, false); res.ins += j; res.gap_opens++; j = 0; }
                     } else if (j == 0 && i > 0) {
-                        if (ladder_q) { push_run(ops, nops, OP_D, i, false); res.del += i; res.gap_opens++; i = 0; }
+                        if (ladder_q) { sink.push(OP_D, i, false); res.del += i; res.gap_opens++; i = 0; }
                     }
                     break;
                 }
-                const uint32_t bits = (dir[(int64_t)(i - 1) * w + (j - 1) / 4] >> (8 * ((j - 1) & 3))) & 0xFFu;
+                if (la_n == 0) {
+                    la_n = min(LA, min(i, j));
+#pragma unroll
+                    for (int d = 0; d < LA; ++d)
+                        if (d < la_n) la[d] = dir[(int64_t)(i - 1 - d) * w + (j - 1 - d) / 4];
+                }
+                const uint32_t bits = (la[0] >> (8 * ((j - 1) & 3))) & 0xFFu;
                 const uint32_t src = bits & 3u;
                 if (src == 0) break;   // local start
                 if (src == 1) {
                     const bool eq = min((int)q[j - 1], ns - 1) == min((int)r[i - 1], ns - 1) &&
                                     (int)q[j - 1] < ns - 1;
-                    push_run(ops, nops, eq ? OP_EQ : OP_X, 1, true);
+                    sink.push(eq ? OP_EQ : OP_X, 1, true);
                     if (eq) res.matches++; else res.mismatches++;
                     --i; --j;
+#pragma unroll
+                    for (int d = 0; d + 1 < LA; ++d) la[d] = la[d + 1];
+                    --la_n;
                 } else {
                     state = src == 2 ? 1 : 2;
                     res.gap_opens++;
                     cont = false;
+                    la_n = 0;   // the path leaves the diagonal
                 }
             } else if (state == 1) {   // E: query residue j against a gap
                 const uint32_t bits = (dir[(int64_t)(i - 1) * w + (j - 1) / 4] >> (8 * ((j - 1) & 3))) & 0xFFu;
-                push_run(ops, nops, OP_I, 1, cont);
+                sink.push(OP_I, 1, cont);
                 res.ins++;
                 cont = (bits & 4u) != 0;
                 if (!cont) state = 0;
                 --j;
             } else {                   // F: reference residue i against a gap
                 const uint32_t bits = (dir[(int64_t)(i - 1) * w + (j - 1) / 4] >> (8 * ((j - 1) & 3))) & 0xFFu;
-                push_run(ops, nops, OP_D, 1, cont);
+                sink.push(OP_D, 1, cont);
                 res.del++;
                 cont = (bits & 8u) != 0;
                 if (!cont) state = 0;
@@ -148,12 +178,14 @@ __device__ void finish_pair(const TbArgs &a, int64_t k, int lq, int lr, const ui
         }
         res.q_begin = j;
         res.r_begin = i;
-        // runs were pushed end to start
-        for (int x = 0, y = nops - 1; x < y; ++x, --y) {
-            const uint32_t t = ops[x];
-            ops[x] = ops[y];
-            ops[y] = t;
-        }
+    }
+    // runs were pushed end to start
+    const int32_t nops = sink.finish();
+    uint32_t *ops = sink.ops;
+    for (int x = 0, y = nops - 1; x < y; ++x, --y) {
+        const uint32_t t = ops[x];
+        ops[x] = ops[y];
+        ops[y] = t;
     }
     res.n_ops = nops;
     res.ops_offset = a.ops_off[k];
