@@ -3,8 +3,9 @@
 // the CPU (reference docs/DESIGN.md:94-96, examples/legacy_16s_gpu_align.py:280-292);
 // here the hits are aligned on the GPU.
 //
-// One thread handles one pair: the DP fill is the comparator's int32 recurrence
-// (src/gpusw/kernel.py:82-157), with each cell's provenance packed into a direction byte
+// The DP fill (16 or 32 lanes per pair for queries up to 2,048 columns, one thread per
+// pair beyond) is the comparator's int32 recurrence (src/gpusw/kernel.py:82-157), with
+// each cell's provenance packed into a direction byte
 //   bits 0-1  H source: 0 = zero (local start), 1 = diagonal, 2 = E, 3 = F
 //   bit  2    E(i,j) extends E(i,j-1) (else opens from H(i,j-1))
 //   bit  3    F(i,j) extends F(i-1,j) (else opens from H(i-1,j))
