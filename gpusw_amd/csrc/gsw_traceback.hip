@@ -269,7 +269,7 @@ __global__ void __launch_bounds__(256) traceback_kernel(TbArgs a) {
 // of its columns in the same scan order; the group reduces them with ties to the first
 // cell in row-major order (the one-thread fill's order), so the walk (traceback_walk_kernel)
 // starts from the same end cell.
-template <int TB_G, int R>
+template <int TB_G, int R, bool LOCAL>
 __global__ void __launch_bounds__(256) traceback_fill_kernel(TbArgs a, TbEnd *ends) {
     static_assert(R % 4 == 0, "direction words hold four cells");
     static_assert(TB_G == 16 || TB_G == 32, "lane groups of 16 or 32");
@@ -279,7 +279,8 @@ __global__ void __launch_bounds__(256) traceback_fill_kernel(TbArgs a, TbEnd *en
     const int t = (int)(threadIdx.x % TB_G);
     const int64_t k = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / TB_G;
     const int go = a.go, ge = a.ge, ns = a.ns;
-    const bool local = a.mode == GSW_MODE_LOCAL, semi = a.mode == GSW_MODE_SEMIGLOBAL;
+    constexpr bool local = LOCAL;   // a compile-time mode keeps the column loop branch-free
+    const bool semi = a.mode == GSW_MODE_SEMIGLOBAL;
     const bool ladder_r = a.mode == GSW_MODE_GLOBAL || (semi && !a.free_r);
     const bool ladder_q = a.mode == GSW_MODE_GLOBAL || (semi && !a.free_q);
     int lq = 0, lr = 0;
@@ -307,8 +308,18 @@ __global__ void __launch_bounds__(256) traceback_fill_kernel(TbArgs a, TbEnd *en
     int hin = 0, ein = NEG32, din = 0;   // H(i, j0), E(i, j0), H(i-1, j0) from the left
     int nsteps = lr > 0 ? lr + TB_G - 1 : 0;
     for (int off = 32; off >= 1; off >>= 1) nsteps = max(nsteps, __shfl_xor(nsteps, off, 64));
+    // reference codes of rows i .. i+3, loaded four steps ahead: loads and the direction
+    // stores share one wait counter, and a code loaded in its own step would wait for the
+    // previous step's stores to reach memory (indices clamped into the reference)
+    auto code_at = [&](int idx) -> int { return lr > 0 ? (int)r[min(max(idx, 0), lr - 1)] : 0; };
+    int rc0 = code_at(-t), rc1 = code_at(1 - t), rc2 = code_at(2 - t), rc3 = code_at(3 - t);
     for (int s = 0; s < nsteps; ++s) {
         const int i = s - t + 1;
+        const int code = rc0;
+        rc0 = rc1;
+        rc1 = rc2;
+        rc2 = rc3;
+        rc3 = code_at(i + 3);
         if (t == 0) {   // column 0
             hin = ladder_q ? go + (i - 1) * ge : 0;
             din = i <= 1 ? 0 : (ladder_q ? go + (i - 2) * ge : 0);
@@ -318,35 +329,39 @@ __global__ void __launch_bounds__(256) traceback_fill_kernel(TbArgs a, TbEnd *en
         }
         int hout = 0, eout = NEG32;
         if (i >= 1 && i <= lr && j0 < lq) {
-            const int32_t *srow = s_sub + min((int)r[i - 1], ns - 1) * ns;
+            const int32_t *srow = s_sub + min(code, ns - 1) * ns;
             int hleft = hin, eleft = ein, diag = din;
             uint32_t word = 0;
+            // Every column is computed, those past lq too (their values and direction bits
+            // are read by nothing: the walk stays within lq and the lanes to the right are
+            // idle), so the loop has no branches and its table reads issue together.
 #pragma unroll
             for (int kk = 0; kk < R; ++kk) {
                 const int j = j0 + kk + 1;
-                if (j <= lq) {
-                    const int fo = hup[kk] + go, fx = fu[kk] + ge;
-                    const int eo = hleft + go, ex = eleft + ge;
-                    const int fj = max(fo, fx), ej = max(eo, ex);
-                    const int d = diag + srow[qc[kk]];
-                    int h = max(max(d, ej), fj);
-                    uint32_t src;
-                    if (local && h <= 0) { h = 0; src = 0; }
-                    else src = h == d ? 1u : h == ej ? 2u : 3u;
-                    word |= (src | (ex > eo ? 4u : 0u) | (fx > fo ? 8u : 0u)) << (8 * (kk & 3));
-                    if (local) {
-                        if (h > e.best) { e.best = h; e.bi = i; e.bj = j; }
-                    } else {
-                        if (i == lr && j == lq) e.corner = h;
-                        if (j == lq && h > e.ec) { e.ec = h; e.eci = i; }
-                        if (i == lr && h > e.er) { e.er = h; e.erj = j; }
-                    }
-                    diag = hup[kk];
-                    hup[kk] = h;
-                    fu[kk] = fj;
-                    hleft = h;
-                    eleft = ej;
+                const bool in = j <= lq;
+                const int fo = hup[kk] + go, fx = fu[kk] + ge;
+                const int eo = hleft + go, ex = eleft + ge;
+                const int fj = max(fo, fx), ej = max(eo, ex);
+                const int d = diag + srow[qc[kk]];
+                int h = max(max(d, ej), fj);
+                uint32_t src = h == d ? 1u : h == ej ? 2u : 3u;
+                if (local && h <= 0) {
+                    h = 0;
+                    src = 0;
                 }
+                word |= (src | (ex > eo ? 4u : 0u) | (fx > fo ? 8u : 0u)) << (8 * (kk & 3));
+                if (local) {
+                    if (in && h > e.best) { e.best = h; e.bi = i; e.bj = j; }
+                } else {
+                    if (i == lr && j == lq) e.corner = h;
+                    if (j == lq && h > e.ec) { e.ec = h; e.eci = i; }
+                    if (in && i == lr && h > e.er) { e.er = h; e.erj = j; }
+                }
+                diag = hup[kk];
+                hup[kk] = h;
+                fu[kk] = fj;
+                hleft = h;
+                eleft = ej;
                 if ((kk & 3) == 3) {
                     const int widx = (j0 + kk) / 4;
                     if (widx < w) dir[(int64_t)(i - 1) * w + widx] = word;
@@ -478,11 +493,16 @@ extern "C" int gsw_traceback_pairs(const gsw_scheme *s, gsw_seqset queries, gsw_
         if (tb_r) {
             TbEnd *d_ends = reinterpret_cast<TbEnd *>(base);
             const unsigned fill_blocks = (unsigned)((m * tb_g + threads - 1) / threads);
+            const bool loc = sd.mode == GSW_MODE_LOCAL;
             switch (tb_g * 1000 + tb_r) {
 #define GSW_TB_FILL(GG, RR)                                                                                          \
     case GG * 1000 + RR:                                                                                             \
-        hipLaunchKernelGGL((traceback_fill_kernel<GG, RR>), dim3(fill_blocks), dim3(threads), 0, ctx->stream, a,     \
-                           d_ends);                                                                                  \
+        if (loc)                                                                                                     \
+            hipLaunchKernelGGL((traceback_fill_kernel<GG, RR, true>), dim3(fill_blocks), dim3(threads), 0,           \
+                               ctx->stream, a, d_ends);                                                              \
+        else                                                                                                         \
+            hipLaunchKernelGGL((traceback_fill_kernel<GG, RR, false>), dim3(fill_blocks), dim3(threads), 0,          \
+                               ctx->stream, a, d_ends);                                                              \
         break;
                 GSW_TB_FILL(16, 4) GSW_TB_FILL(16, 8) GSW_TB_FILL(16, 12) GSW_TB_FILL(16, 16) GSW_TB_FILL(16, 20)
                 GSW_TB_FILL(16, 24) GSW_TB_FILL(16, 32) GSW_TB_FILL(16, 48) GSW_TB_FILL(16, 64)
