@@ -308,18 +308,8 @@ __global__ void __launch_bounds__(256) traceback_fill_kernel(TbArgs a, TbEnd *en
     int hin = 0, ein = NEG32, din = 0;   // H(i, j0), E(i, j0), H(i-1, j0) from the left
     int nsteps = lr > 0 ? lr + TB_G - 1 : 0;
     for (int off = 32; off >= 1; off >>= 1) nsteps = max(nsteps, __shfl_xor(nsteps, off, 64));
-    // reference codes of rows i .. i+3, loaded four steps ahead: loads and the direction
-    // stores share one wait counter, and a code loaded in its own step would wait for the
-    // previous step's stores to reach memory (indices clamped into the reference)
-    auto code_at = [&](int idx) -> int { return lr > 0 ? (int)r[min(max(idx, 0), lr - 1)] : 0; };
-    int rc0 = code_at(-t), rc1 = code_at(1 - t), rc2 = code_at(2 - t), rc3 = code_at(3 - t);
     for (int s = 0; s < nsteps; ++s) {
         const int i = s - t + 1;
-        const int code = rc0;
-        rc0 = rc1;
-        rc1 = rc2;
-        rc2 = rc3;
-        rc3 = code_at(i + 3);
         if (t == 0) {   // column 0
             hin = ladder_q ? go + (i - 1) * ge : 0;
             din = i <= 1 ? 0 : (ladder_q ? go + (i - 2) * ge : 0);
@@ -329,7 +319,7 @@ __global__ void __launch_bounds__(256) traceback_fill_kernel(TbArgs a, TbEnd *en
         }
         int hout = 0, eout = NEG32;
         if (i >= 1 && i <= lr && j0 < lq) {
-            const int32_t *srow = s_sub + min(code, ns - 1) * ns;
+            const int32_t *srow = s_sub + min((int)r[i - 1], ns - 1) * ns;
             int hleft = hin, eleft = ein, diag = din;
             uint32_t word = 0;
             // Every column is computed, those past lq too (their values and direction bits
