@@ -69,6 +69,13 @@ class GswAln(ctypes.Structure):
                 ("n_ops", _i32), ("ops_offset", _i64)]
 
 
+#: numpy view of a gsw_aln array (same field offsets as the C struct)
+ALN_DTYPE = np.dtype({"names": [f for f, _ in GswAln._fields_],
+                      "formats": [np.int64 if t is _i64 else np.int32 for _, t in GswAln._fields_],
+                      "offsets": [getattr(GswAln, f).offset for f, _ in GswAln._fields_],
+                      "itemsize": ctypes.sizeof(GswAln)})
+
+
 #: symbol -> argtypes (every function returns int status unless listed in _RESTYPE)
 _SIGNATURES = {
     "gsw_last_error": [],

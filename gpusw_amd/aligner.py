@@ -202,10 +202,13 @@ class Aligner:
             recs, ops, stats = run_traceback(self.scheme, self._d_qbuf, self._d_qoff, self._d_rbuf,
                                              self._d_roff, qi, rj)
         self._record(int(stats.cells), stats.kernel_ms / 1e3, "traceback (int32 fill, walk per pair)")
-        return [Alignment(self._queries.ids[int(q)], self._refs.ids[int(r)], a.score, a.q_begin, a.q_end,
-                          a.r_begin, a.r_end, _cigar(ops, a.ops_offset, a.n_ops), a.matches, a.mismatches,
-                          a.ins, a.del_, a.gap_opens)
-                for q, r, a in zip(qi, rj, recs)]
+        qids, rids = self._queries.ids, self._refs.ids
+        col = {f: recs[f].tolist() for f in recs.dtype.names}
+        return [Alignment(qids[q], rids[r], sc, qb, qe, rb, re_, _cigar(ops, off, nops), m, x, ins, dl, gop)
+                for q, r, sc, qb, qe, rb, re_, off, nops, m, x, ins, dl, gop in zip(
+                    qi.tolist(), rj.tolist(), col["score"], col["q_begin"], col["q_end"], col["r_begin"],
+                    col["r_end"], col["ops_offset"], col["n_ops"], col["matches"], col["mismatches"],
+                    col["ins"], col["del_"], col["gap_opens"])]
 
     def top_k_alignments(self, queries=None, k: int = 5, *, query_batch: int = 512) -> list:
         """``top_k`` followed by the traceback of every hit: ``[(query_id, [Alignment,

@@ -176,7 +176,8 @@ class HipPlainBackend(HipBackend):
 
 def run_traceback(scheme, qbuf, qoff, rbuf, roff, qi_host, rj_host):
     """GPU traceback of pairs (``gsw_traceback_pairs``): ``(records, ops, stats)`` with
-    records a ``GswAln`` array and ops the concatenated CIGAR runs."""
+    records a structured array of ``gsw_aln`` (``_hip.ALN_DTYPE``) and ops the
+    concatenated CIGAR runs."""
     qset = _hip.seqset_for(qbuf, qoff)
     rset = _hip.seqset_for(rbuf, roff)
     qi = np.ascontiguousarray(qi_host, dtype=np.int32)
@@ -192,7 +193,7 @@ def run_traceback(scheme, qbuf, qoff, rbuf, roff, qi_host, rj_host):
     stats = _hip.GswStats()
     _hip.call("gsw_traceback_pairs", ctypes.byref(s), qset.handle, rset.handle, _hip.ptr(qi), _hip.ptr(rj),
               n, ctypes.cast(recs, ctypes.c_void_p), _hip.ptr(ops), cap, ctypes.byref(stats))
-    return recs[:n], ops, stats
+    return np.frombuffer(recs, dtype=_hip.ALN_DTYPE, count=n), ops, stats
 
 
 def _check_pair_indices(qi, rj, nq, nr) -> None:
