@@ -32,9 +32,21 @@ class DeviceScores:
 
 
 def _policy(scheme) -> int:
-    # 'int32' forced: 32-bit kernels.  Otherwise the library picks int16 per pair
-    # where the score range allows it (exact either way).
-    return _hip.DTYPE_INT32 if scheme.dtype == "int32" else _hip.DTYPE_AUTO
+    """The library's element-type policy for a scheme's ``dtype``.
+
+    ``"auto"`` applies the reference's int16 envelope (``src/gpusw/scheme.py:116-141``)
+    to every pair with the pair's own lengths instead of the call's longest ones: a pair
+    runs on the packed int16 tiles only where no DP value can saturate, otherwise on the
+    int32 tiles.  Every pair the call-wide envelope sends to int32 that could saturate is
+    therefore on the int32 tiles, as in ``backends/cuda.py:68-85``.  ``"int16"`` reaches
+    the library only after ``resolve_dtype`` accepted it; local results that reach 32767
+    would be rescored on the int32 tiles.  ``"int32"``: 32-bit tiles throughout.
+    """
+    if scheme.dtype == "int32":
+        return _hip.DTYPE_INT32
+    if scheme.dtype == "int16":
+        return _hip.DTYPE_INT16
+    return _hip.DTYPE_AUTO
 
 
 def _describe(kind: str, scheme, stats: _hip.GswStats, dtype: str) -> str:
@@ -108,18 +120,26 @@ class HipBackend(Backend):
         rset = _hip.seqset_for(rbuf, roff)
         qi = np.ascontiguousarray(qi_host, dtype=np.int32)
         rj = np.ascontiguousarray(rj_host, dtype=np.int32)
-        # indices are checked by the library while it plans (IndexError), not in extra passes
-        buf = self._pair_out(n)
         s, table = scheme_struct_cached(scheme)
-        stats = _hip.GswStats()
-        _hip.call("gsw_score_pairs", ctypes.byref(s), _policy(scheme), qset.handle, rset.handle,
-                  _hip.ptr(qi), _hip.ptr(rj), int(n), buf.ptr, ctypes.byref(stats))
-        self._note(stats)
-        self.last_cells = int(stats.cells)   # counted by the library while planning
         host = np.empty(n, dtype=np.int32)
-        if n:
-            _hip.call("gsw_memcpy_d2h", _hip.ptr(host), buf.ptr, 4 * n)
-        return host, stats.kernel_ms / 1e3, _describe("pairs", scheme, stats, dtype)
+        # at most max_threads_per_launch pairs per library call (as the reference's Metal
+        # backend chunks its pairs, src/gpusw/backends/metal.py:126); indices are checked by
+        # the library while it plans (IndexError), not in extra passes
+        cap = max(1, int(self.max_threads_per_launch))
+        kernel_ms, cells, stats = 0.0, 0, _hip.GswStats()
+        for k0 in range(0, max(n, 1), cap):
+            m = min(cap, n - k0)
+            buf = self._pair_out(m)
+            stats = _hip.GswStats()
+            _hip.call("gsw_score_pairs", ctypes.byref(s), _policy(scheme), qset.handle, rset.handle,
+                      _hip.ptr(qi[k0:]), _hip.ptr(rj[k0:]), int(m), buf.ptr, ctypes.byref(stats))
+            self._note(stats)
+            kernel_ms += stats.kernel_ms
+            cells += int(stats.cells)    # counted by the library while planning
+            if m:
+                _hip.call("gsw_memcpy_d2h", _hip.ptr(host[k0:]), buf.ptr, 4 * m)
+        self.last_cells = cells
+        return host, kernel_ms / 1e3, _describe("pairs", scheme, stats, dtype)
 
     def as_host(self, scores_native):
         return scores_native.to_host().astype(np.int32, copy=False)

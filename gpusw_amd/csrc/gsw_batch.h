@@ -24,9 +24,10 @@ struct PairTask {
     int64_t oa, ob;          // output indices (ob < 0: none)
 };
 
+// An int16 result that reached 32767 (local mode under GSW_DTYPE_INT16): rescored in int32.
 struct SatEntry {
-    int32_t qs, ql;          // query start and length
-    int32_t r, pad;          // reference id
+    int32_t q;               // query index (-1: the output row names it, cross products)
+    int32_t r;               // reference index
     int64_t o;               // output index
 };
 

@@ -63,10 +63,18 @@ struct SchemeDev {
     std::vector<int32_t> h_sub;
 };
 int scheme_prepare(const gsw_scheme *s, DeviceCtx *ctx, SchemeDev *out);
+// The scheme with query and reference exchanged: transposed table, free-end flags swapped.
+// score(q, r) under s equals score(r, q) under the transposed scheme, in every mode.
+int scheme_transpose(DeviceCtx *ctx, const SchemeDev &s, SchemeDev *out);
 
 // Static bound check used to choose int16 per pair (sound for every DP cell).
+bool int16_entries(const SchemeDev &s);   // table and gap scores fit int16
 bool fits_int16(const SchemeDev &s, int lq, int lr);
 bool fits_int32(const SchemeDev &s, int lq, int lr);
+// Element type of a pair under a dtype policy: GSW_DTYPE_AUTO applies the envelope per pair
+// (int16 only where no value can saturate); GSW_DTYPE_INT16 runs every local pair in int16
+// and relies on the kernels' saturation list; GSW_DTYPE_INT32 never uses int16.
+bool use_int16(const SchemeDev &s, int dtype_policy, int lq, int lr);
 
 // Closed-form score when one side is empty (src/gpusw/kernel.py:63-80).
 int32_t empty_edge_score(const SchemeDev &s, int lq, int lr);
@@ -122,11 +130,14 @@ struct BatchResult { double kernel_ms = 0; int64_t rescored = 0; int64_t plain_p
                      int launches = 0; int variant = -1; int64_t cells = -1; /* -1: not counted */
                      double top_ms = 0; int64_t top_cells = 0; int top_variant = -1, top_striped = 0; };
 
+// Cross product of queries h_qlist[a] x every reference into d_out[outrow(a) * nr + b],
+// outrow(a) = h_outrow ? h_outrow[a] : a.
 int batch_cross(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqSet &qs,
                 const int32_t *h_qlist, int32_t nq, const SeqSet &rs, int32_t *d_out,
-                BatchResult *res);
+                BatchResult *res, const int32_t *h_outrow = nullptr);
+// Explicit pairs (h_qi[k], h_rj[k]) into d_out[h_outidx ? h_outidx[k] : k].
 int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqSet &qs,
                 const SeqSet &rs, const int32_t *h_qi, const int32_t *h_rj, int64_t n,
-                int32_t *d_out, BatchResult *res);
+                int32_t *d_out, BatchResult *res, const int64_t *h_outidx = nullptr);
 
 }  // namespace gsw

@@ -133,6 +133,14 @@ int scheme_prepare(const gsw_scheme *s, DeviceCtx *ctx, SchemeDev *out) {
     return GSW_OK;
 }
 
+int scheme_transpose(DeviceCtx *ctx, const SchemeDev &s, SchemeDev *out) {
+    std::vector<int32_t> t((size_t)s.ns * s.ns);
+    for (int a = 0; a < s.ns; ++a)
+        for (int b = 0; b < s.ns; ++b) t[(size_t)b * s.ns + a] = s.h_sub[(size_t)a * s.ns + b];
+    const gsw_scheme g{s.mode, s.go, s.ge, s.free_r, s.free_q, s.ns, t.data()};
+    return scheme_prepare(&g, ctx, out);
+}
+
 // Bounds of any DP value for a (lq, lr) pair: every H lies in [lower, upper]
 // (upper: at most min(lq,lr) positive substitutions; lower: a diagonal run plus
 // one gap run, each step costing at least min(go, ge, min_sub, 0)).
@@ -146,13 +154,29 @@ static void dp_bounds(const SchemeDev &s, int64_t lq, int64_t lr, int64_t *lower
     }
 }
 
+bool int16_entries(const SchemeDev &s) {
+    return s.sub_min >= -32767 && s.sub_max <= 32767 && s.go >= -32767 && s.ge >= -32767;
+}
+
+// The envelope of src/gpusw/scheme.py:116-141 for one pair, with the margins of the packed
+// kernels.  Local: H, E and F are alignment scores of prefixes (>= 0 in the clamped form),
+// at most min(lq, lr) * max_sub, so below 32767 no value saturates and no result can be
+// mistaken for a saturated one.  Global/semiglobal: every value lies in [lo, hi].
 bool fits_int16(const SchemeDev &s, int lq, int lr) {
-    if (s.sub_min < -32767 || s.sub_max > 32767 || s.go < -32767 || s.ge < -32767) return false;
-    if (s.mode == GSW_MODE_LOCAL) return true;   // saturation detected at run time
+    if (!int16_entries(s)) return false;
     int64_t lo, hi;
     dp_bounds(s, lq, lr, &lo, &hi);
+    if (s.mode == GSW_MODE_LOCAL) return hi + std::max(s.sub_max, 0) + 16 <= 32767;
     int64_t margin = -(int64_t)s.go - (int64_t)s.ge - (int64_t)std::min(s.sub_min, 0) + 16;
     return lo - margin >= -32768 && hi + std::max(s.sub_max, 0) + 16 <= 32767;
+}
+
+bool use_int16(const SchemeDev &s, int dtype_policy, int lq, int lr) {
+    if (dtype_policy == GSW_DTYPE_INT32) return false;
+    // a caller-checked int16 local pair runs packed whatever its bound: saturated results
+    // (32767) are listed by the kernel and rescored in int32
+    if (dtype_policy == GSW_DTYPE_INT16 && s.mode == GSW_MODE_LOCAL) return int16_entries(s);
+    return fits_int16(s, lq, lr);
 }
 
 bool fits_int32(const SchemeDev &s, int lq, int lr) {

@@ -11,7 +11,9 @@
 // in flight on the two slots' streams, so the second takes the CUs as the first one's last
 // workgroups finish.  When a request's last item completes, a device top-k of its score rows
 // (score descending, ties to the lower reference index) is copied back and its memory is
-// released.  Saturated int16 results are rescored in int32 before a launch is retired.
+// released.  Queries run in int16 only where the score envelope proves that no value
+// saturates (src/gpusw/scheme.py:116-141, applied per query), otherwise on the int32 tiles, so
+// launches need no saturation list.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -99,6 +101,7 @@ struct Request {
     int status = GSW_OK;
     std::string error;
     bool done = false;
+    bool claimed = false;                     // a gsw_server_wait is waiting for it
     double t_submit = 0, t_admit = 0, t_done = 0;
     int64_t cells = 0;
     int launches = 0;
@@ -111,8 +114,7 @@ struct Slot {
     hipEvent_t ev = nullptr;
     void *items = nullptr;
     size_t items_cap = 0;
-    int32_t *counters = nullptr;   // [0] work counter, [1] saturation count
-    SatEntry *sat = nullptr;
+    int32_t *counters = nullptr;   // [0] work counter
     int db = -1;
     std::vector<std::pair<Request *, int>> reqs;
     std::vector<CrossItem> host_items;
@@ -120,7 +122,6 @@ struct Slot {
     size_t bnd_bytes = 0;
 };
 
-constexpr int SAT_CAP = 1 << 16;
 constexpr int NSLOTS = 2;
 
 }  // namespace
@@ -353,7 +354,7 @@ int Server::launch_locked(int si, Request *head) {
     GSW_TRY(ensure_buffer(&sl.items, &sl.items_cap, bytes));
     GSW_HIP_TRY(hipStreamWaitEvent(sl.st, codes_ready, 0));   // this launch's query codes are uploaded
     GSW_HIP_TRY(hipMemcpyAsync(sl.items, sl.host_items.data(), bytes, hipMemcpyHostToDevice, sl.st));
-    GSW_HIP_TRY(hipMemsetAsync(sl.counters, 0, 2 * sizeof(int32_t), sl.st));
+    GSW_HIP_TRY(hipMemsetAsync(sl.counters, 0, sizeof(int32_t), sl.st));
     const size_t lds = cross_lds_bytes(v, d.s.ns);
     int grid = 0;
     GSW_TRY(occupancy_grid(&ctx, striped ? v.stripe_fn : v.cross_fn, CROSS_WAVES * 64, lds, &grid));
@@ -379,10 +380,7 @@ int Server::launch_locked(int si, Request *head) {
     a.ns = d.s.ns;
     a.k = SweepConst{d.s.mode, d.s.go, d.s.ge, d.s.free_q, d.s.free_r, d.s.ns};
     a.out = reinterpret_cast<int32_t *>(score_base);
-    a.sat = sl.sat;
-    a.sat_count = sl.counters + 1;
-    a.sat_cap = SAT_CAP;
-    a.sat_check = (v.packed && v.local) ? 1 : 0;
+    a.sat_check = 0;   // int16 only where no value can saturate
     a.items = sl.items;
     a.nitems = (int32_t)sl.host_items.size();
     a.counter = sl.counters;
@@ -434,44 +432,6 @@ int Server::finalize(Request *r) {
 int Server::retire(int si) {
     Slot &sl = slots[si];
     GSW_HIP_TRY(hipEventSynchronize(sl.ev));
-    int32_t nsat = 0;
-    GSW_HIP_TRY(hipMemcpy(&nsat, sl.counters + 1, sizeof(int32_t), hipMemcpyDeviceToHost));
-    if (nsat > 0) {
-        const ServerDb &d = *dbs[sl.db];
-        std::vector<SatEntry> h(std::min(nsat, SAT_CAP));
-        GSW_HIP_TRY(hipMemcpy(h.data(), sl.sat, sizeof(SatEntry) * h.size(), hipMemcpyDeviceToHost));
-        std::vector<int32_t> qs, ql, rj;
-        std::vector<int64_t> oi;
-        int maxq = 1;
-        if (nsat > SAT_CAP) {
-            // list overflowed: rescore every pair of this launch's items in int32
-            for (const CrossItem &it : sl.host_items) {
-                for (int p = it.rbeg; p < it.rend; ++p) {
-                    const int ref = d.rs->order_desc[p];
-                    qs.push_back(it.qsa); ql.push_back(it.lqa); rj.push_back(ref); oi.push_back(it.outa + ref);
-                    if (it.outb >= 0) {
-                        qs.push_back(it.qsb); ql.push_back(it.lqb); rj.push_back(ref); oi.push_back(it.outb + ref);
-                    }
-                    maxq = std::max({maxq, it.lqa, it.lqb});
-                }
-            }
-        } else {
-            for (const SatEntry &e : h) {
-                qs.push_back(e.qs); ql.push_back(e.ql); rj.push_back(e.r); oi.push_back(e.o);
-                maxq = std::max(maxq, e.ql);
-            }
-        }
-        PlainJob job{};
-        job.qcodes = code_base; job.qoff = nullptr; job.rcodes = d.rs->d_codes; job.roff = d.rs->d_off;
-        job.dtype = GSW_DTYPE_INT32; job.threads = 256; job.maxq = maxq;
-        job.h_qstart = qs.data(); job.h_qlen = ql.data(); job.h_rj = rj.data(); job.h_outidx = oi.data();
-        job.npairs = (int64_t)qs.size();
-        job.d_out = reinterpret_cast<int32_t *>(score_base);
-        float ms = 0.f;
-        GSW_TRY(plain_run(&ctx, d.s, job, &ms));
-        std::lock_guard<std::mutex> lk(mu);
-        rescored += (int64_t)qs.size();
-    }
     std::unique_lock<std::mutex> lk(mu);
     for (auto &pr : sl.reqs) {
         Request *r = pr.first;
@@ -560,8 +520,7 @@ int gsw_server_create(size_t mem_budget_bytes, double launch_cells, gsw_server *
     for (int i = 0; i < NSLOTS; ++i) {
         GSW_HIP_TRY(hipStreamCreateWithFlags(&srv->slots[i].st, hipStreamNonBlocking));
         GSW_HIP_TRY(hipEventCreateWithFlags(&srv->slots[i].ev, hipEventDisableTiming));
-        GSW_HIP_TRY(hipMalloc(&srv->slots[i].counters, 2 * sizeof(int32_t)));
-        GSW_HIP_TRY(hipMalloc(&srv->slots[i].sat, sizeof(SatEntry) * SAT_CAP));
+        GSW_HIP_TRY(hipMalloc(&srv->slots[i].counters, sizeof(int32_t)));
     }
     Server *p = srv.get();
     p->worker = std::thread([p] { p->run(); });
@@ -626,10 +585,14 @@ int gsw_server_wait(gsw_server server, int64_t req_id, int32_t timeout_ms, int32
     auto it = srv->reqs.find(req_id);
     if (it == srv->reqs.end()) return fail(GSW_ERR_INVALID, "unknown request %lld", (long long)req_id);
     Request *r = it->second.get();
+    // one waiter at a time: the waiter that returns the result frees the request
+    if (r->claimed) return fail(GSW_ERR_INVALID, "request %lld is already being waited for", (long long)req_id);
+    r->claimed = true;
     auto ready = [&] { return r->done; };
     if (timeout_ms < 0) {
         srv->cv_done.wait(lk, ready);
     } else if (!srv->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready)) {
+        r->claimed = false;
         return fail(GSW_ERR_TIMEOUT, "request %lld not finished", (long long)req_id);
     }
     if (info) {
@@ -691,7 +654,6 @@ int gsw_server_destroy(gsw_server server) {
         }
         (void)hipEventDestroy(srv->slots[i].ev);
         (void)hipFree(srv->slots[i].counters);
-        (void)hipFree(srv->slots[i].sat);
         if (srv->slots[i].items) (void)hipFree(srv->slots[i].items);
         if (srv->slots[i].bnd) (void)hipFree(srv->slots[i].bnd);
     }

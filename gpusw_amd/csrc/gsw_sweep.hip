@@ -24,11 +24,13 @@
 
 namespace gsw {
 
-__device__ __forceinline__ void emit(const KArgs &a, int64_t o, int qs, int ql, int r, int score) {
+// Store a result; with the saturation check on (packed local launches under
+// GSW_DTYPE_INT16), a result that reached 32767 is also listed for int32 rescoring.
+__device__ __forceinline__ void emit(const KArgs &a, int64_t o, int q, int r, int score) {
     a.out[o] = score;
     if (a.sat_check && score >= 32767) {
         const int i = atomicAdd(a.sat_count, 1);
-        if (i < a.sat_cap) a.sat[i] = SatEntry{qs, ql, r, 0, o};
+        if (i < a.sat_cap) a.sat[i] = SatEntry{q, r, o};
     }
 }
 
@@ -95,8 +97,8 @@ __attribute__((amdgpu_waves_per_eu(WavesPerEU<LOCAL, G, R>::value))) cross_kerne
             const SweepOut res = sweep_group<V, G, R, LOCAL>(a.rcodes + rs, lr, nsteps, prof_lane, t,
                                                              a.k, lqa, lqb, -item.shift);
             if (t == 0 && ref >= 0) {
-                emit(a, item.outa + ref, qsa, lqa, ref, res.a);
-                if (item.outb >= 0) emit(a, item.outb + ref, qsb, lqb, ref, res.b);
+                emit(a, item.outa + ref, -1, ref, res.a);
+                if (item.outb >= 0) emit(a, item.outb + ref, -1, ref, res.b);
             }
         }
         __syncthreads();
@@ -179,8 +181,8 @@ __attribute__((amdgpu_waves_per_eu(WavesPerEU<LOCAL, G, R>::value))) stripe_kern
                         if (item.outb >= 0) vb = max(vb, a.out[item.outb + ref]);
                     }
                     if (last) {
-                        emit(a, item.outa + ref, qsa, lqa, ref, va);
-                        if (item.outb >= 0) emit(a, item.outb + ref, qsb, lqb, ref, vb);
+                        emit(a, item.outa + ref, -1, ref, va);
+                        if (item.outb >= 0) emit(a, item.outb + ref, -1, ref, vb);
                     } else {
                         a.out[item.outa + ref] = va;
                         if (item.outb >= 0) a.out[item.outb + ref] = vb;
@@ -260,8 +262,8 @@ __global__ void __launch_bounds__(WAVES * 64) pairs_kernel(KArgs a) {
         const SweepOut res = sweep_group<V, G, R, LOCAL, false, true>(a.rcodes + rs, lr, nsteps, lane_lds, t,
                                                                       a.k, lqa, lqb, -shift);
         if (t == 0 && valid) {
-            emit(a, task.oa, qsa, lqa, task.ref, res.a);
-            if (task.ob >= 0) emit(a, task.ob, qsb, lqb, task.ref, res.b);
+            emit(a, task.oa, qa, task.ref, res.a);
+            if (task.ob >= 0) emit(a, task.ob, qb, task.ref, res.b);
         }
     }
 }
@@ -328,8 +330,8 @@ __global__ void __launch_bounds__(WAVES * 64) pairs_tab_kernel(KArgs a) {
         const SweepOut res = sweep_group<V, G, R, LOCAL, false, true, true>(
             a.rcodes + rs, lr, nsteps, tab_lane, t, a.k, lqa, lqb, -shift, nullptr, nullptr, false, false, toff, trow);
         if (t == 0 && valid) {
-            emit(a, task.oa, qsa, lqa, task.ref, res.a);
-            if (task.ob >= 0) emit(a, task.ob, qsb, lqb, task.ref, res.b);
+            emit(a, task.oa, qa, task.ref, res.a);
+            if (task.ob >= 0) emit(a, task.ob, qb, task.ref, res.b);
         }
     }
 }
@@ -631,39 +633,33 @@ static int prepare_sat(DeviceCtx *ctx, SatBuffers *sb, int32_t cap) {
     return GSW_OK;
 }
 
-// Rescore the listed saturated int16 results in int32 with the plain kernel.  Sets
-// *overflowed (and rescored nothing) when more pairs saturated than the list holds.
-static int rescore_saturated(DeviceCtx *ctx, const SchemeDev &s, const SatBuffers &sb,
-                             const uint8_t *qcodes, const SeqSet &rs, int32_t *d_out, BatchResult *res,
-                             bool *overflowed) {
+// The saturated results listed by the launches since prepare_sat.  *overflowed: more results
+// saturated than the list holds (then none is returned and the caller rescores all of its
+// packed local work).
+static int read_saturated(DeviceCtx *ctx, const SatBuffers &sb, std::vector<SatEntry> *out, bool *overflowed) {
     int32_t count = 0;
     GSW_HIP_TRY(hipMemcpyAsync(&count, ctx->counters + 1, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
     GSW_HIP_TRY(hipStreamSynchronize(ctx->stream));
     *overflowed = count > sb.cap;
+    out->clear();
     if (count == 0 || *overflowed) return GSW_OK;
-    std::vector<SatEntry> h(count);
-    GSW_HIP_TRY(hipMemcpy(h.data(), sb.d_sat, sizeof(SatEntry) * count, hipMemcpyDeviceToHost));
-    std::vector<int32_t> qst(count), qln(count), rj(count);
-    std::vector<int64_t> oi(count);
-    int maxq = 1;
-    for (int i = 0; i < count; ++i) {
-        qst[i] = h[i].qs;
-        qln[i] = h[i].ql;
-        rj[i] = h[i].r;
-        oi[i] = h[i].o;
-        maxq = std::max(maxq, h[i].ql);
-    }
-    PlainJob job{};
-    job.qcodes = qcodes; job.qoff = nullptr; job.rcodes = rs.d_codes; job.roff = rs.d_off;
-    job.dtype = GSW_DTYPE_INT32; job.threads = 256; job.maxq = maxq;
-    job.h_qstart = qst.data(); job.h_qlen = qln.data(); job.h_rj = rj.data(); job.h_outidx = oi.data();
-    job.npairs = count;
-    job.d_out = d_out;
-    float pms = 0.f;
-    GSW_TRY(plain_run(ctx, s, job, &pms));
-    res->kernel_ms += pms;
-    res->rescored += count;
-    res->launches++;
+    out->resize((size_t)count);
+    GSW_HIP_TRY(hipMemcpy(out->data(), sb.d_sat, sizeof(SatEntry) * (size_t)count, hipMemcpyDeviceToHost));
+    return GSW_OK;
+}
+
+// Rescore explicit pairs (qi[i], rj[i]) into d_out[oi[i]] with the batched int32 tiles.
+static int rescore_pairs_int32(DeviceCtx *ctx, const SchemeDev &s, const SeqSet &qs, const SeqSet &rs,
+                               const std::vector<int32_t> &qi, const std::vector<int32_t> &rj,
+                               const std::vector<int64_t> &oi, int32_t *d_out, BatchResult *res) {
+    if (qi.empty()) return GSW_OK;
+    BatchResult r;
+    GSW_TRY(batch_pairs(ctx, s, GSW_DTYPE_INT32, qs, rs, qi.data(), rj.data(), (int64_t)qi.size(), d_out, &r,
+                        oi.data()));
+    res->kernel_ms += r.kernel_ms;
+    res->launches += r.launches;
+    res->plain_pairs += r.plain_pairs;
+    res->rescored += (int64_t)qi.size();
     return GSW_OK;
 }
 
@@ -714,7 +710,7 @@ int plan_cross_units(const SchemeDev &s, int dtype_policy, const std::vector<Cro
     for (const CrossQuery &q : queries) {
         if (!fits_int32(s, q.len, max_lr))
             return fail(GSW_ERR_OVERFLOW, "score range of query %d exceeds the int32 kernels", q.id);
-        const bool use16 = dtype_policy != GSW_DTYPE_INT32 && fits_int16(s, q.len, max_lr);
+        const bool use16 = use_int16(s, dtype_policy, q.len, max_lr);
         if (use16 && !track_last_col) {
             packed_free.push_back(q);
             continue;
@@ -894,8 +890,9 @@ int64_t expand_stripe_items(const CrossUnit *u0, const CrossUnit *u1, const SeqS
 // ---------------------------------------------------------------- cross
 int batch_cross(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqSet &qs,
                 const int32_t *h_qlist, int32_t nq, const SeqSet &rs, int32_t *d_out,
-                BatchResult *res) {
+                BatchResult *res, const int32_t *h_outrow) {
     const int32_t nr = rs.n;
+    auto orow = [&](int a) -> int64_t { return h_outrow ? (int64_t)h_outrow[a] : (int64_t)a; };
     if (nq == 0 || nr == 0) return GSW_OK;
     const bool local = s.mode == GSW_MODE_LOCAL;
     double mean_lr = rs.n_nonempty ? (double)rs.total / rs.n_nonempty : 1.0;
@@ -911,13 +908,13 @@ int batch_cross(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
         const int lq = qs.len[q];
         if (lq == 0) {
             for (int b = 0; b < nr; ++b) {
-                e_idx.push_back((int64_t)a * nr + b);
+                e_idx.push_back(orow(a) * nr + b);
                 e_val.push_back(empty_edge_score(s, 0, rs.len[b]));
             }
         } else {
             for (int p = rs.n_nonempty; p < nr; ++p) {
                 const int b = rs.order_desc[p];
-                e_idx.push_back((int64_t)a * nr + b);
+                e_idx.push_back(orow(a) * nr + b);
                 e_val.push_back(empty_edge_score(s, lq, 0));
             }
         }
@@ -928,7 +925,7 @@ int batch_cross(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
     for (int a = 0; a < nq; ++a) {
         const int q = h_qlist[a];
         if (qs.len[q] == 0 || rs.n_nonempty == 0) continue;
-        cq.push_back(CrossQuery{qs.h_off[q], qs.len[q], (int64_t)a * nr, q});
+        cq.push_back(CrossQuery{qs.h_off[q], qs.len[q], orow(a) * nr, q});
     }
     std::vector<CrossUnit> units;
     std::vector<CrossQuery> plain_cq;
@@ -1123,29 +1120,41 @@ int batch_cross(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
 
     // -- edges and saturation rescoring
     GSW_TRY(scatter_values(ctx, d_out, e_idx, e_val));
+    std::vector<SatEntry> sat;
     bool overflowed = false;
-    GSW_TRY(rescore_saturated(ctx, s, sb, qs.d_codes, rs, d_out, res, &overflowed));
+    GSW_TRY(read_saturated(ctx, sb, &sat, &overflowed));
     if (overflowed) {
-        // too many saturated pairs to list: rescore every batched int16 row in int32
+        // too many saturated results to list: every row of a packed local unit again, on the
+        // int32 tiles (the rows' empty-reference cells are written again with the same values)
         std::vector<int32_t> rows, ql;
-        int maxq = 1;
         for (const CrossUnit &u : units) {
             if (!u.v->packed) continue;
-            rows.push_back((int32_t)(u.a.out / nr)); ql.push_back(u.a.id); maxq = std::max(maxq, u.a.len);
-            if (u.has_b) { rows.push_back((int32_t)(u.b.out / nr)); ql.push_back(u.b.id); maxq = std::max(maxq, u.b.len); }
+            rows.push_back((int32_t)(u.a.out / nr)); ql.push_back(u.a.id);
+            if (u.has_b) { rows.push_back((int32_t)(u.b.out / nr)); ql.push_back(u.b.id); }
         }
-        PlainJob job{};
-        job.qcodes = qs.d_codes; job.qoff = qs.d_off; job.rcodes = rs.d_codes; job.roff = rs.d_off;
-        job.dtype = GSW_DTYPE_INT32; job.threads = 256; job.maxq = maxq;
-        job.h_qlist = ql.data(); job.h_outrow = rows.data(); job.nq = (int32_t)ql.size();
-        job.nr = nr; job.ldo = nr; job.d_out = d_out;
-        float pms = 0.f;
-        GSW_TRY(plain_run(ctx, s, job, &pms));
-        res->kernel_ms += pms;
+        BatchResult r;
+        GSW_TRY(batch_cross(ctx, s, GSW_DTYPE_INT32, qs, ql.data(), (int32_t)ql.size(), rs, d_out, &r,
+                            rows.data()));
+        res->kernel_ms += r.kernel_ms;
+        res->launches += r.launches;
+        res->plain_pairs += r.plain_pairs;
         res->rescored += (int64_t)ql.size() * nr;
-        res->launches++;
-        // empty reference columns of those rows were just overwritten: restore them
-        GSW_TRY(scatter_values(ctx, d_out, e_idx, e_val));
+    } else if (!sat.empty()) {
+        // the listed results as explicit pairs: the output row names the query
+        std::vector<std::pair<int64_t, int32_t>> row_query((size_t)nq);
+        for (int a = 0; a < nq; ++a) row_query[a] = {orow(a), h_qlist[a]};
+        std::sort(row_query.begin(), row_query.end());
+        std::vector<int32_t> qi, rj;
+        std::vector<int64_t> oi;
+        for (const SatEntry &e : sat) {
+            const int64_t row = e.o / nr;
+            auto it = std::lower_bound(row_query.begin(), row_query.end(), std::make_pair(row, INT32_MIN));
+            if (it == row_query.end() || it->first != row) return fail(GSW_ERR_HIP, "saturation list names no row");
+            qi.push_back(it->second);
+            rj.push_back(e.r);
+            oi.push_back(e.o);
+        }
+        GSW_TRY(rescore_pairs_int32(ctx, s, qs, rs, qi, rj, oi, d_out, res));
     }
     return GSW_OK;
 }
@@ -1333,8 +1342,9 @@ constexpr int MAX_PAIR_SLOTS = 8;   // pinned buffers + graphs per tile
 
 int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqSet &qs,
                 const SeqSet &rs, const int32_t *h_qi, const int32_t *h_rj, int64_t n,
-                int32_t *d_out, BatchResult *res) {
+                int32_t *d_out, BatchResult *res, const int64_t *h_outidx) {
     if (n == 0) return GSW_OK;
+    auto oidx = [h_outidx](int64_t k) -> int64_t { return h_outidx ? h_outidx[k] : k; };
     // GSW_TRACE_LAUNCHES=1: host phases of the call to stderr
     static const bool trace = [] { const char *e = std::getenv("GSW_TRACE_LAUNCHES"); return e && *e == '1'; }();
     using clk = std::chrono::steady_clock;
@@ -1345,6 +1355,7 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
     const int wmax = max_width();
     const double mean_lr = rs.n_nonempty ? (double)rs.total / rs.n_nonempty : 1.0;
     const int T = n >= 65536 ? host_threads() : 1;
+    const bool local16 = dtype_policy == GSW_DTYPE_INT16 && local && int16_entries(s);
     const size_t nv = 2 * variant_table().size();   // kinds: 2 * variant id + (table kernel)
     // The pair list runs in segments.  A call of at least four batches classifies, plans and
     // launches a head of 21/8 batches first and classifies the rest while the device runs
@@ -1421,7 +1432,7 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
             const int lq = qs.len[q], lr = rs.len[r];
             P.cells += (int64_t)lq * lr;
             if (lq == 0 || lr == 0) {
-                P.e_idx.push_back(k);
+                P.e_idx.push_back(oidx(k));
                 P.e_val.push_back(empty_edge_score(s, lq, lr));
                 continue;
             }
@@ -1442,7 +1453,8 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
                 P.bad_code = GSW_ERR_OVERFLOW;
                 return;
             }
-            const bool use16 = dtype_policy != GSW_DTYPE_INT32 && ok16;
+            // GSW_DTYPE_INT16 runs every local pair packed (saturated results are listed)
+            const bool use16 = dtype_policy == GSW_DTYPE_INT32 ? false : local16 ? true : ok16;
             const Variant *v = nullptr;
             bool tab = false;
             if (lq <= wmax) {
@@ -1459,7 +1471,7 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
             if (!v) {
                 P.p_qi.push_back(q);
                 P.p_rj.push_back(r);
-                P.p_oi.push_back(k);
+                P.p_oi.push_back(oidx(k));
                 continue;
             }
             vid[k] = (int16_t)(2 * v->id + (tab ? 1 : 0));
@@ -1632,10 +1644,10 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
             }
             if (v.packed && i + 1 < nm && S.a[i + 1].r == x.r && S.a[i + 1].shift == x.shift) {
                 const Member &y = S.a[i + 1];
-                tasks[nt++] = PairTask{x.q, y.q, x.r, x.shift, x.k, y.k};
+                tasks[nt++] = PairTask{x.q, y.q, x.r, x.shift, oidx(x.k), oidx(y.k)};
                 i += 2;
             } else {
-                tasks[nt++] = PairTask{x.q, -1, x.r, x.shift, x.k, -1};
+                tasks[nt++] = PairTask{x.q, -1, x.r, x.shift, oidx(x.k), -1};
                 i += 1;
             }
         }
@@ -1831,32 +1843,82 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
         }
     }
 
-    if (!p_qi.empty()) {
-        int maxq = 1;
-        bool use16 = dtype_policy != GSW_DTYPE_INT32 && !local;
-        for (size_t i = 0; i < p_qi.size(); ++i) {
-            maxq = std::max(maxq, qs.len[p_qi[i]]);
-            use16 = use16 && fits_int16(s, qs.len[p_qi[i]], rs.len[p_rj[i]]);
+    // saturated int16 results (GSW_DTYPE_INT16, local) are listed before any other call
+    // resets the list; they run again on the int32 tiles below: the listed ones or, when more
+    // saturated than the list holds, every pair that ran packed
+    std::vector<int32_t> rqi, rrj;
+    std::vector<int64_t> roi;
+    {
+        std::vector<SatEntry> sat;
+        bool overflowed = false;
+        GSW_TRY(read_saturated(ctx, sb, &sat, &overflowed));
+        if (overflowed) {
+            const std::vector<Variant> &vt = variant_table();
+            for (int64_t k = 0; k < n; ++k) {
+                if (vid[k] < 0) continue;
+                const Variant &v = vt[vid[k] / 2];
+                if (!v.packed || !v.local) continue;
+                rqi.push_back(h_qi[k]);
+                rrj.push_back(h_rj[k]);
+                roi.push_back(oidx(k));
+            }
+        } else {
+            for (const SatEntry &e : sat) {
+                rqi.push_back(e.q);
+                rrj.push_back(e.r);
+                roi.push_back(e.o);
+            }
         }
-        PlainJob job{};
-        job.qcodes = qs.d_codes; job.qoff = qs.d_off; job.rcodes = rs.d_codes; job.roff = rs.d_off;
-        job.dtype = use16 ? GSW_DTYPE_INT16 : GSW_DTYPE_INT32;
-        job.threads = 256; job.maxq = maxq;
-        job.h_qi = p_qi.data(); job.h_rj = p_rj.data(); job.h_outidx = p_oi.data();
-        job.npairs = (int64_t)p_qi.size(); job.d_out = d_out;
-        float pms = 0.f;
-        GSW_TRY(plain_run(ctx, s, job, &pms));
-        res->kernel_ms += pms;
-        res->plain_pairs += (int64_t)p_qi.size();
-        res->launches++;
+    }
+    // (the calls below reuse this thread's planning buffers vid and members: nothing reads
+    // them after this point)
+
+    // Pairs without a tile have a query wider than every pairs tile.  Where the reference
+    // fits one, the pair is scored with query and reference exchanged under the transposed
+    // scheme (same score in every mode); pairs longer than every tile on both sides run on
+    // the one-thread-per-pair comparator.
+    if (!p_qi.empty()) {
+        std::vector<int32_t> tq, tr, cq, cr;
+        std::vector<int64_t> to, co;
+        for (size_t i = 0; i < p_qi.size(); ++i) {
+            const bool swap = qs.len[p_qi[i]] > wmax && rs.len[p_rj[i]] <= wmax;
+            (swap ? tq : cq).push_back(p_qi[i]);
+            (swap ? tr : cr).push_back(p_rj[i]);
+            (swap ? to : co).push_back(p_oi[i]);
+        }
+        if (!tq.empty()) {
+            SchemeDev st;
+            GSW_TRY(scheme_transpose(ctx, s, &st));
+            BatchResult r;
+            GSW_TRY(batch_pairs(ctx, st, dtype_policy, rs, qs, tr.data(), tq.data(), (int64_t)tq.size(), d_out, &r,
+                                to.data()));
+            res->kernel_ms += r.kernel_ms;
+            res->launches += r.launches;
+            res->plain_pairs += r.plain_pairs;
+            res->rescored += r.rescored;
+        }
+        if (!cq.empty()) {
+            int maxq = 1;
+            bool use16 = dtype_policy != GSW_DTYPE_INT32 && !local;
+            for (size_t i = 0; i < cq.size(); ++i) {
+                maxq = std::max(maxq, qs.len[cq[i]]);
+                use16 = use16 && fits_int16(s, qs.len[cq[i]], rs.len[cr[i]]);
+            }
+            PlainJob job{};
+            job.qcodes = qs.d_codes; job.qoff = qs.d_off; job.rcodes = rs.d_codes; job.roff = rs.d_off;
+            job.dtype = use16 ? GSW_DTYPE_INT16 : GSW_DTYPE_INT32;
+            job.threads = 256; job.maxq = maxq;
+            job.h_qi = cq.data(); job.h_rj = cr.data(); job.h_outidx = co.data();
+            job.npairs = (int64_t)cq.size(); job.d_out = d_out;
+            float pms = 0.f;
+            GSW_TRY(plain_run(ctx, s, job, &pms));
+            res->kernel_ms += pms;
+            res->plain_pairs += (int64_t)cq.size();
+            res->launches++;
+        }
     }
     GSW_TRY(scatter_values(ctx, d_out, e_idx, e_val));
-
-    bool overflowed = false;
-    GSW_TRY(rescore_saturated(ctx, s, sb, qs.d_codes, rs, d_out, res, &overflowed));
-    if (overflowed)
-        return fail(GSW_ERR_LIMIT, "more than %d saturated int16 pairs; use dtype='int32'", sb.cap);
-    return GSW_OK;
+    return rescore_pairs_int32(ctx, s, qs, rs, rqi, rrj, roi, d_out, res);
 }
 
 }  // namespace gsw

@@ -460,7 +460,9 @@ extern "C" int gsw_traceback_pairs(const gsw_scheme *s, gsw_seqset queries, gsw_
         char *mp = base + ws_bytes + ((dir_bytes + 255) / 256) * 256;
         int32_t *d_qi = reinterpret_cast<int32_t *>(mp);
         int32_t *d_rj = d_qi + m;
-        int64_t *d_doff = reinterpret_cast<int64_t *>(d_rj + m + (m & 1));
+        // mp is 256-byte aligned and the two int32 lists take 8m bytes: the int64 lists and the
+        // records (8-byte aligned) start on an 8-byte boundary
+        int64_t *d_doff = reinterpret_cast<int64_t *>(d_rj + m);
         int64_t *d_ooff = d_doff + m;
         gsw_aln *d_out = reinterpret_cast<gsw_aln *>(d_ooff + m);
         uint32_t *d_ops = reinterpret_cast<uint32_t *>(d_out + m);

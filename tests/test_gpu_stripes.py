@@ -1,10 +1,13 @@
 """Queries wider than a tile run as stripes of G*R columns (stripe kernel): scores stay
 bit-exact with the oracle in every mode, in int16 and int32, when striped and unstriped
 queries are mixed and paired, when int16 saturates in a late stripe, and when served."""
+import ctypes
+
 import numpy as np
 import pytest
 
 from gpusw_amd import Aligner, Scheme, _hip
+from gpusw_amd.backends.hip import scheme_struct_cached
 from gpusw_amd.encode import funnel
 from gpusw_amd.serve import Server
 from oracle import native
@@ -66,17 +69,32 @@ def test_protein_beyond_widest_tile(mode):
 
 
 def test_saturation_in_late_stripe_rescored():
-    """Local int16 reaches 32767 only after several stripes; the pair is rescored in int32."""
+    """A local score past int16 that builds up over several stripes.  With dtype "auto" the
+    envelope sends the pair to the int32 stripes; forced to int16 through the C ABI, the
+    int16 result reaches 32767 only in a late stripe and is rescored on the int32 tiles."""
     scheme = Scheme.preset("blosum62")
     rng = np.random.default_rng(21)
     w = "W" * 3100                      # self score 11 per residue: 34100 > 32767
     queries = [w, "".join(rng.choice(list(PROT), 2000)), w[:600]]
     refs = [w, "W" * 200, "".join(rng.choice(list(PROT), 700)), w[:2990] + "A" * 20]
-    got, st = _score(queries, refs, scheme)
     want = _oracle(queries, refs, scheme)
     assert int(want.max()) > 32767
+    got, st = _score(queries, refs, scheme)
     assert np.array_equal(got.astype(np.int64), want)
-    assert st.rescored >= 2
+    assert st.rescored == 0 and st.plain_pairs == 0
+    al = Aligner(scheme, backend="hip").index(refs).set_queries(queries)
+    qset = _hip.seqset_for(al._d_qbuf, al._d_qoff)
+    rset = _hip.seqset_for(al._d_rbuf, al._d_roff)
+    s, _table = scheme_struct_cached(scheme)
+    out = _hip.DeviceBuffer(4 * want.size, np.int32, want.shape)
+    st = _hip.GswStats()
+    qlist = np.arange(len(queries), dtype=np.int32)
+    _hip.call("gsw_score_cross", ctypes.byref(s), _hip.DTYPE_INT16, qset.handle, _hip.ptr(qlist),
+              len(queries), rset.handle, out.ptr, ctypes.byref(st))
+    assert np.array_equal(out.to_host().astype(np.int64), want)
+    # both saturated pairs are longer than every explicit-pairs tile (2,048 columns) on both
+    # sides: their int32 rescoring is the comparator's
+    assert st.rescored == int((want >= 32767).sum()) == 2 and st.plain_pairs == 2
 
 
 def test_served_long_queries_equal_aligner():

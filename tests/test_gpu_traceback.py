@@ -122,3 +122,17 @@ def test_top_k_alignments_and_cli(tmp_path):
     assert main(["topk", "--queries", str(qf), "--refs", str(rf), "-k", "2", "--traceback", "-o", str(out)]) == 0
     rows = [l.split(",") for l in out.read_text().splitlines()]
     assert rows[0][-1] == "cigar" and rows[1][2] == "r0" and rows[1][9] == "60=" and float(rows[1][4]) == 1.0
+
+
+@pytest.mark.parametrize("npairs", [1, 7, 301])
+def test_odd_pair_counts(npairs):
+    """Odd launch sizes: the per-pair offset lists and records after the two int32 index
+    lists keep their 8-byte alignment."""
+    scheme, alpha = SCHEMES["dna_local"]
+    rng = np.random.default_rng(stable_seed(f"odd{npairs}"))
+    queries = rand_seqs(rng, 13, 1, 120, alpha)
+    refs = rand_seqs(rng, 11, 1, 150, alpha)
+    al = Aligner(scheme, backend="hip").index(refs).set_queries(queries)
+    qi = rng.integers(0, len(queries), npairs).astype(np.int32)
+    rj = rng.integers(0, len(refs), npairs).astype(np.int32)
+    _compare(al.align(qi, rj), _oracle(al, qi, rj))
