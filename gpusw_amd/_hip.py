@@ -100,6 +100,8 @@ _SIGNATURES = {
                         ctypes.POINTER(GswStats)],
     "gsw_score_pairs": [ctypes.POINTER(GswScheme), _i32, _vp, _vp, _vp, _vp, _i64, _vp,
                         ctypes.POINTER(GswStats)],
+    "gsw_score_pairs_noalloc": [ctypes.POINTER(GswScheme), _i32, _vp, _vp, _vp, _vp, _i64, _vp,
+                                ctypes.POINTER(GswStats)],
     "gsw_topk_rows": [_vp, _i32, _i32, _i32, _vp, _vp, ctypes.POINTER(GswStats)],
     "gsw_memcpy_d2d": [_vp, _vp, _sz],
     "gsw_server_create": [_sz, ctypes.c_double, ctypes.POINTER(_vp)],

@@ -79,6 +79,9 @@ std::vector<Variant> &variant_table();
 const Variant *pick_variant(int lq, double mean_lr, bool packed, bool local, int ns, size_t lds_limit,
                             bool for_pairs);
 int max_width();
+// Tile of an explicit pair with query length lq; *tab: run it with the table kernel.
+const Variant *pick_pairs_tile(int lq, double mean_lr, bool packed, bool local, int ns, size_t lds_limit,
+                               bool *tab);
 size_t cross_lds_bytes(const Variant &v, int ns);
 size_t pairs_wave_bytes(const Variant &v, int ns);               // one wave's private profiles
 int pairs_waves(const Variant &v, int ns, size_t lds_limit);      // waves per workgroup (0: none fit)

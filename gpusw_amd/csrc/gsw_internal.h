@@ -62,6 +62,7 @@ struct SchemeDev {
     const int32_t *d_sub;                       // device copy (cached by content)
     std::vector<int32_t> h_sub;
 };
+int scheme_validate(const gsw_scheme *s, SchemeDev *out);   // host only; d_sub left null
 int scheme_prepare(const gsw_scheme *s, DeviceCtx *ctx, SchemeDev *out);
 // The scheme with query and reference exchanged: transposed table, free-end flags swapped.
 // score(q, r) under s equals score(r, q) under the transposed scheme, in every mode.

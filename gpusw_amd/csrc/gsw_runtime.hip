@@ -99,7 +99,7 @@ struct SubCacheKey {
 static std::mutex g_sub_mu;
 static std::map<SubCacheKey, int32_t *> g_sub_cache;   // src/gpusw/_compile.py:83-93
 
-int scheme_prepare(const gsw_scheme *s, DeviceCtx *ctx, SchemeDev *out) {
+int scheme_validate(const gsw_scheme *s, SchemeDev *out) {
     if (!s) return fail(GSW_ERR_INVALID, "scheme is null");
     if (s->mode < 0 || s->mode > 2) return fail(GSW_ERR_INVALID, "mode %d out of range", s->mode);
     if (s->gap_open > 0 || s->gap_extend > 0)
@@ -119,6 +119,12 @@ int scheme_prepare(const gsw_scheme *s, DeviceCtx *ctx, SchemeDev *out) {
     out->sub_max = *std::max_element(out->h_sub.begin(), out->h_sub.end());
     if (out->sub_min < -(1 << 20) || out->sub_max > (1 << 20))
         return fail(GSW_ERR_OVERFLOW, "substitution scores beyond +/-2^20 are not supported");
+    out->d_sub = nullptr;
+    return GSW_OK;
+}
+
+int scheme_prepare(const gsw_scheme *s, DeviceCtx *ctx, SchemeDev *out) {
+    GSW_TRY(scheme_validate(s, out));
     std::lock_guard<std::mutex> lk(g_sub_mu);
     SubCacheKey key{ctx->device, out->h_sub};
     auto it = g_sub_cache.find(key);

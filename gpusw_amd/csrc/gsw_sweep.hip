@@ -559,8 +559,8 @@ static bool pairs_prefer_profiles() {
 }
 
 // Tile of an explicit pair with query length lq; *tab: run it with the table kernel.
-static const Variant *pick_pairs_tile(int lq, double mean_lr, bool packed, bool local, int ns, size_t lds_limit,
-                                      bool *tab) {
+const Variant *pick_pairs_tile(int lq, double mean_lr, bool packed, bool local, int ns, size_t lds_limit,
+                               bool *tab) {
     *tab = false;
     if (pairs_prefer_profiles()) {
         const Variant *v = pick_variant(lq, mean_lr, packed, local, ns, lds_limit, true);

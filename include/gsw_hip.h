@@ -135,6 +135,23 @@ int gsw_score_pairs(const gsw_scheme *s, int32_t dtype_policy,
                     const int32_t *h_qi, const int32_t *h_rj, int64_t n,
                     int32_t *d_out, gsw_stats *stats);
 
+/* ---- explicit pairs without device allocation -------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the same
+ * scores as gsw_score_pairs, computed by the same table kernels, with every buffer beyond
+ * the inputs and outputs in a file-scope device array of about 18 MiB per GPU (held until
+ * the process ends).  The call allocates nothing: no hipMalloc/hipFree, no streams, events or
+ * scheme cache entries; it runs on the null stream and returns when its kernels are done.
+ * Limits: at most 262,144 pairs per call (GSW_ERR_LIMIT beyond, checked before any device
+ * work) and queries of at most 2,048 columns (GSW_ERR_LIMIT otherwise).  Every call on a GPU
+ * shares that GPU's array, which keeps the last call's contents; the library serialises calls
+ * on one GPU with a lock, so calls that overlap on one GPU wait for each other instead of
+ * racing.  stats->kernel_ms is the call's wall time (no events are created). */
+int gsw_score_pairs_noalloc(const gsw_scheme *s, int32_t dtype_policy,
+                            gsw_seqset queries, gsw_seqset refs,
+                            const int32_t *h_qi, const int32_t *h_rj, int64_t n,
+                            int32_t *d_out, gsw_stats *stats);
+
 /* ---- device top-k ---------------------------------------------------------
  * Per row of a device int32 (nrows, ncols) score matrix, the k best columns,
  * best first, ties broken toward the lower column index.  Replaces
