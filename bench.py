@@ -8,11 +8,22 @@ computed into HBM plus a device top_k(5) of every row.  Inputs are resident in H
 before the timed region.  Also reported:
 
 * ``roofline``: packed-int16 VALU roofline of the cross kernel -- algorithmic int16 ops
-  (cells x 10 for local mode) over the kernel's device time measured with HIP events on
-  the library stream, against the gfx950 peak of 78.6 T int16-ops/s;
+  (cells x 9 for local mode: 4 add/sub + 5 max, the clamp max(H, 0) folded into saturating
+  E/F updates; global/semiglobal 8) over the kernel's device time measured with HIP events
+  on the library stream, against the gfx950 peak of 78.6 T int16-ops/s (the 10-op count
+  that prices max(H, 0) separately is kept as ``survey_frac``);
 * ``plain``: the one-thread-per-pair comparator on the same GPU (a query subsample);
 * ``cpu_baseline``: the oracle's C restatement on the host cores over a bounded sample
-  of the same pairs (those pairs are also checked bit-exact against the GPU scores).
+  of the same pairs (those pairs are also checked bit-exact against the GPU scores), and
+  ``cpu_baseline.reference_python``: the reference's own pure-Python algorithm
+  (``oracle.gotoh.score_codes``, a restatement of ``src/gpusw/reference.py:38-125``) on
+  every host core through ``multiprocessing``, on a smaller sample;
+* ``protein_topk``: the BLOSUM62 top-5 of 1,000 queries against 100,000 references --
+  GCUPS, the call's roofline, the int16 plain comparator's speedup, oracle-checked hits;
+* ``serving``: Poisson arrivals at one sub-saturation rate (>= 400 requests of 1-512
+  queries, 70 % DNA / 30 % protein against both resident databases) through the
+  continuous-batching server: p50/p99 latency, GCUPS, and every served top-k checked
+  against the batched top-k of the same queries, plus an oracle sample of served hits.
 
 ``--workload`` selects another configuration for extra lines: ``plain_dna`` (the 64 x 256
 int32 case, oracle-checked in full), ``protein_topk`` (BLOSUM62
@@ -36,11 +47,11 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "bit-exact GCUPS on 1 MI355X (batched vs plain); p50/p99 latency when served"
 PEAK_INT16_TOPS = 78.6          # 256 CU x 64 lanes x 2 (packed) x 2.4 GHz, int16 ops/s / 1e12
-OPS_PER_CELL = {"local": 10, "global": 8, "semiglobal": 8}
-# int16 ops the kernels issue per cell (a packed instruction is 2 int16 ops for 2 cells,
-# one per query of the pair).  Local folds max(H, 0) into saturating E/F updates and
-# issues 9, so its algorithmic fraction can reach 10/9 at full VALU issue.
-ISSUED_OPS_PER_CELL = {10: 9, 8: 8}
+# int16 ops per cell update: local 4 add/sub + 5 max (E, F and H-gap kept >= 0 by unsigned
+# saturating subtraction, so max(H, 0) needs no op of its own), global/semiglobal 4 + 4
+OPS_PER_CELL = {"local": 9, "global": 8, "semiglobal": 8}
+# the survey's count, which prices max(H, 0) as a separate op in local mode
+SURVEY_OPS_PER_CELL = {"local": 10, "global": 8, "semiglobal": 8}
 
 
 def log(msg: str) -> None:
@@ -76,6 +87,51 @@ def cpu_baseline(qenc, renc, scheme, gpu_scores, rng, target_s: float = 12.0):
         "value": round(cells / dt / 1e9, 4), "unit": "GCUPS", "cores": threads, "kind": "port",
         "sample": f"{n} random (query, ref) pairs of the same workload, {cells:.3e} cells, {dt:.1f} s",
     }, mism, n
+
+
+# pairs scored by the pure-Python leg's worker processes (set before they are forked)
+_PY_JOB = None
+
+
+def _py_score(k: int) -> int:
+    from oracle.gotoh import score_codes
+
+    qc, qo, rc, ro, sub, mode, go, ge, fq, fr, qi, rj = _PY_JOB
+    q, r = int(qi[k]), int(rj[k])
+    return score_codes(qc[qo[q]:qo[q + 1]], rc[ro[r]:ro[r + 1]], sub, mode, go, ge, fq, fr)
+
+
+def cpu_reference_python(q, r, scheme, rng, target_s: float = 10.0):
+    """The reference's own CPU path -- its pure-Python double loop (``src/gpusw/reference.py
+    :38-125``, restated as ``oracle.gotoh.score_codes``) -- on every host core: forked
+    worker processes over a seeded sample of (query, reference) pairs, sized to about
+    ``target_s`` seconds.  Runs before this process touches the GPU; returns the result line
+    and the scored pairs, which are checked against the GPU scores afterwards."""
+    import multiprocessing as mp
+
+    global _PY_JOB
+    (qc, qo), (rc, ro) = q, r
+    cores = len(os.sched_getaffinity(0))
+    qlen = np.diff(qo).astype(np.int64)
+    rlen = np.diff(ro).astype(np.int64)
+    n = 4 * cores + int(target_s * cores / 0.25)      # upper bound; the loop stops at target_s
+    qi = rng.integers(0, qlen.size, n).astype(np.int32)
+    rj = rng.integers(0, rlen.size, n).astype(np.int32)
+    _PY_JOB = (qc, qo, rc, ro, scheme.substitution_table(), scheme.mode, scheme.gap_open, scheme.gap_extend,
+               bool(scheme.free_end_gaps_query), bool(scheme.free_end_gaps_ref), qi, rj)
+    done, scores = 0, []
+    with mp.get_context("fork").Pool(cores) as pool:
+        t0 = time.perf_counter()
+        while done < n and time.perf_counter() - t0 < target_s:
+            chunk = min(n - done, 2 * cores)
+            scores += pool.map(_py_score, range(done, done + chunk), chunksize=1)
+            done += chunk
+        dt = time.perf_counter() - t0
+    cells = float((qlen[qi[:done]] * rlen[rj[:done]]).sum())
+    line = {"value": round(cells / dt / 1e9, 6), "unit": "GCUPS", "cores": cores, "kind": "reference",
+            "sample": f"{done} random (query, ref) pairs of the same workload, {cells:.3e} cells, {dt:.1f} s, "
+                      f"pure-Python Gotoh loop in {cores} worker processes"}
+    return line, (qi[:done], rj[:done], np.array(scores, dtype=np.int64))
 
 
 def _tile_name(variant: int) -> str | None:
@@ -114,9 +170,10 @@ def _measured_traffic(workload: str, kernel: str | None):
     return None, None
 
 
-def _roofline(history, ops_per_cell: int, workload: str) -> dict:
+def _roofline(history, mode: str, workload: str) -> dict:
     """Roofline of the dominant kernel: the batched launch with the most cells in each timed
     call (its cells x ops over its device time from HIP events on the stream it ran on)."""
+    ops, survey = OPS_PER_CELL[mode], SURVEY_OPS_PER_CELL[mode]
     top = [h for h in history if h.top_variant >= 0 and h.top_ms > 0]
     if not top:
         return {"bound": "valu", "achieved": None, "peak": PEAK_INT16_TOPS, "unit": "TOP/s (int16)",
@@ -127,15 +184,14 @@ def _roofline(history, ops_per_cell: int, workload: str) -> dict:
     sel = [h for h in top if (h.top_variant, h.top_striped) == (var, striped)]
     ms = float(np.mean([h.top_ms for h in sel]))
     cells = float(np.mean([h.top_cells for h in sel]))
-    achieved = cells * ops_per_cell / (ms * 1e-3) / 1e12
+    achieved = cells * ops / (ms * 1e-3) / 1e12
     kernel = _kernel_name(var, striped)
     traffic, source = _measured_traffic(workload, kernel)
     out = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_INT16_TOPS, "unit": "TOP/s (int16)",
            "frac": round(achieved / PEAK_INT16_TOPS, 4), "traffic": traffic,
            "kernel": kernel, "tile": _tile_name(var), "launch_ms": round(ms, 3), "launch_cells": int(cells),
-           "ops_per_cell": ops_per_cell, "issued_ops_per_cell": ISSUED_OPS_PER_CELL[ops_per_cell],
-           "issued_frac": round(achieved * ISSUED_OPS_PER_CELL[ops_per_cell] / ops_per_cell / PEAK_INT16_TOPS, 4),
-           "launches_timed": len(sel)}
+           "ops_per_cell": ops, "survey_ops_per_cell": survey,
+           "survey_frac": round(achieved * survey / ops / PEAK_INT16_TOPS, 4), "launches_timed": len(sel)}
     if traffic is not None:
         out["traffic_unit"] = "bytes/launch (rocprofv3 FETCH_SIZE, WRITE_SIZE, copy-calibrated)"
         out["traffic_source"] = source
@@ -157,23 +213,23 @@ def _call_traffic(workload: str):
 
 
 def _call_roofline(history, cells: int, kernel_s: float, steps: int) -> dict:
-    """Roofline of a protein top-k call as a whole.  Its tile launches (19) run on two
-    streams and overlap (each launch's workgroups take CUs as the other stream's finish), so
-    no single launch owns the GPU for its interval: the dominant unit is the call's set of
+    """Roofline of a protein top-k call as a whole.  Its tile launches run on two streams
+    and overlap (each launch's workgroups take CUs as the other stream's finish), so no
+    single launch owns the GPU for its interval: the dominant unit is the call's set of
     batched launches, timed from the first launch's start to the last one's end (HIP events
     on the library stream, joined with the second stream)."""
-    achieved = cells * OPS_PER_CELL["local"] / kernel_s / 1e12
+    ops, survey = OPS_PER_CELL["local"], SURVEY_OPS_PER_CELL["local"]
+    achieved = cells * ops / kernel_s / 1e12
     calls = max(1, len(history) // max(steps, 1))
     traffic, src = _call_traffic("protein")
     out = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_INT16_TOPS, "unit": "TOP/s (int16)",
            "frac": round(achieved / PEAK_INT16_TOPS, 4),
            "traffic": None if traffic is None else round(traffic * calls, 1),
            "kernel": "all batched launches of a top_k call (cross_kernel + stripe_kernel tiles, two streams)",
-           "ops_per_cell": OPS_PER_CELL["local"], "issued_ops_per_cell": ISSUED_OPS_PER_CELL[OPS_PER_CELL["local"]],
-           "issued_frac": round(achieved * ISSUED_OPS_PER_CELL[OPS_PER_CELL["local"]] / OPS_PER_CELL["local"]
-                                / PEAK_INT16_TOPS, 4),
+           "ops_per_cell": ops, "survey_ops_per_cell": survey,
+           "survey_frac": round(achieved * survey / ops / PEAK_INT16_TOPS, 4),
            "kernels_ms_per_step": round(kernel_s * 1e3, 2), "calls_per_step": calls,
-           "largest_launch": _roofline(history, OPS_PER_CELL["local"], "protein")}
+           "largest_launch": _roofline(history, "local", "protein")}
     if traffic is not None:
         out["traffic_unit"] = "bytes per step (sum of the per-dispatch rocprofv3 rows of every batched kernel)"
         out["traffic_source"] = src
@@ -189,6 +245,11 @@ def run_dna_cross(args):
     (qc, qo), (rc, ro) = synth.dna_16s_cross(seed=0)
     log(f"workload generated in {time.perf_counter() - t0:.1f}s")
     scheme = Scheme.preset("dna")
+    rng = np.random.default_rng(1)
+    py_line = py_pairs = None
+    if not args.no_cpu:   # before any GPU work: the worker processes are forked
+        py_line, py_pairs = cpu_reference_python((qc, qo), (rc, ro), scheme, np.random.default_rng(11))
+        log(f"cpu reference (pure Python): {py_line['value']} GCUPS on {py_line['cores']} processes")
     al = Aligner(scheme, backend="hip").index((rc, ro)).set_queries((qc, qo))
     b = al._b
     nq, nr = al._queries.n, al._refs.n
@@ -221,7 +282,7 @@ def run_dna_cross(args):
     value = cells / (wall / args.steps) / 1e9
     stats = b.last_stats
     kavg = float(np.mean(kernel_s))
-    roof = _roofline(history, OPS_PER_CELL["local"], "dna16s")
+    roof = _roofline(history, "local", "dna16s")
     roof.update({"kernels_ms_per_step": round(kavg * 1e3, 3), "launches_per_step": int(stats.launches)})
 
     # ---- correctness of the timed result (outside the timed region)
@@ -230,7 +291,6 @@ def run_dna_cross(args):
     topk_ok = bool(np.array_equal(order, idx) and np.array_equal(np.take_along_axis(host, order, 1), sc))
 
     # ---- plain comparator on a query subsample
-    rng = np.random.default_rng(1)
     nplain = args.plain_queries
     psel = np.sort(rng.choice(nq, nplain, replace=False)).astype(np.int32)
     pb = gpusw_amd.backends.resolve_backend("hip-plain")
@@ -259,6 +319,9 @@ def run_dna_cross(args):
     if not args.no_cpu:
         cpu, mism, nsample = cpu_baseline(al._queries, al._refs, scheme, host, rng)
         log(f"cpu baseline: {cpu['value']} GCUPS on {cpu['cores']} threads; {mism} mismatches in {nsample}")
+        pqi, prj, pwant = py_pairs
+        py_line["mismatches_vs_gpu"] = int((host[pqi, prj].astype(np.int64) != pwant).sum())
+        cpu["reference_python"] = py_line
 
     line = {
         "metric": METRIC,
@@ -288,6 +351,12 @@ def run_dna_cross(args):
         "traceback": traceback,
         "cpu_baseline": cpu,
     }
+    if not args.no_protein or not args.no_serving:
+        prot, prot_pool = protein_leg(args)
+        if not args.no_protein:
+            line["protein_topk"] = prot
+        if not args.no_serving:
+            line["serving"] = serving_leg(args, ((qc, qo), (rc, ro), idx, sc), prot_pool)
     return line
 
 
@@ -339,59 +408,176 @@ def run_plain_dna(args):
     }
 
 
-def _plain_subsample_cross(al, scheme, qsel):
+def _plain_subsample_cross(al, scheme, qsel, dtype):
     """GCUPS of the one-thread-per-pair kernel on a query subsample (same GPU)."""
     import gpusw_amd
 
     pb = gpusw_amd.backends.resolve_backend("hip-plain")
     nr = al._refs.n
-    sc, dt, _ = pb.run_cross(scheme, al._queries.max_len, "int32", 128, al._d_qbuf, al._d_qoff,
+    sc, dt, _ = pb.run_cross(scheme, al._queries.max_len, dtype, 128, al._d_qbuf, al._d_qoff,
                              al._d_rbuf, al._d_roff, qsel, len(qsel), nr)
     cells = int(al._queries.lengths[qsel].astype(np.int64).sum()) * int(al._refs.lengths.astype(np.int64).sum())
     return sc.to_host(), cells / dt / 1e9
 
 
-def run_protein_topk(args):
-    """Protein top-k: 1,000 lognormal queries vs 100,000 lognormal references, BLOSUM62
-    local -11/-1, device top_k(5) per query batch of 512 (the Aligner.top_k loop)."""
+def protein_leg(args, steps: int | None = None, warmup: int | None = None):
+    """Protein top-k: 1,000 lognormal queries vs 100,000 lognormal references, BLOSUM62 local
+    -11/-1, device top_k(5) per query batch of 512 (the Aligner.top_k loop).  Returns the
+    result dict and the workload with its top-5 (the serving leg's expected results)."""
     import gpusw_amd
     from gpusw_amd import Aligner, Scheme, synth
+    from oracle import native
 
+    steps = args.protein_steps if steps is None else steps
+    warmup = 1 if warmup is None else warmup
     (qc, qo), (rc, ro) = synth.protein_topk(seed=0)
     scheme = Scheme.preset("blosum62")
     al = Aligner(scheme, backend="hip").index((rc, ro)).set_queries((qc, qo))
     nq, nr = al._queries.n, al._refs.n
     cells = int(al._queries.lengths.astype(np.int64).sum()) * int(al._refs.lengths.astype(np.int64).sum())
     log(f"protein top-k: {nq} x {nr}, {cells:.3e} cells")
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         al.top_k(k=5)
     gpusw_amd._hip.call("gsw_synchronize")
     al._b.history = []          # the backend appends each call's stats
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         res = al.top_k(k=5)
-    wall = (time.perf_counter() - t0) / args.steps
+    wall = (time.perf_counter() - t0) / steps
     history, al._b.history = al._b.history, None
-    kernel_s = sum(h.kernel_ms for h in history) / 1e3 / args.steps
-    # checks: top-k of a query subsample against the comparator's full rows
+    kernel_s = sum(h.kernel_ms for h in history) / 1e3 / steps
+    value = cells / wall / 1e9
+    log(f"protein top-k: {value:.1f} GCUPS ({wall * 1e3:.1f} ms per call)")
+    # checks: top-k of a query subsample against the comparator's full rows (int16 stores,
+    # as the reference's envelope resolves this workload), and the oracle on sampled hits
     rng = np.random.default_rng(2)
+    dtype = scheme.resolve_dtype(int(al._queries.max_len), int(al._refs.max_len))
     qsel = np.sort(rng.choice(nq, 8, replace=False)).astype(np.int32)
-    prow, plain_gcups = _plain_subsample_cross(al, scheme, qsel)
+    prow, plain_gcups = _plain_subsample_cross(al, scheme, qsel, dtype)
     order = np.argsort(-prow, axis=1, kind="stable")[:, :5]
     ok = bool(np.array_equal(order, res.topk_idx[qsel]) and
               np.array_equal(np.take_along_axis(prow, order, 1), res.topk_scores[qsel]))
-    value = cells / wall / 1e9
-    return {
-        "metric": METRIC, "value": round(value, 2), "unit": "GCUPS", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(wall * 1e3, 2), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "int16", "data": "synthetic",
+    hq = rng.choice(nq, 200, replace=False)
+    hqi = np.repeat(hq, 5).astype(np.int32)
+    hrj = res.topk_idx[hq].reshape(-1).astype(np.int32)
+    want = native.score_encoded_pairs(al._queries, al._refs, hqi, hrj, scheme)
+    hit_mism = int((res.topk_scores[hq].reshape(-1).astype(np.int64) != want).sum())
+    out = {
+        "value": round(value, 2), "unit": "GCUPS", "ms_per_step": round(wall * 1e3, 2), "steps": steps,
+        "warmup": warmup, "dtype": "int16",
         "config": {"workload": "protein top-k: 1000 queries (lognormal median 350, 30-2000) x 100000 "
                                "references (lognormal median 300, 30-3000), BLOSUM62 local -11/-1, top_k(5)",
                    "cells": cells},
-        "roofline": _call_roofline(history, cells, kernel_s, args.steps),
-        "plain": {"gcups": round(plain_gcups, 2), "queries": int(len(qsel)), "speedup": round(value / plain_gcups, 2)},
-        "checks": {"topk_equals_comparator_rows": ok},
+        "roofline": _call_roofline(history, cells, kernel_s, steps),
+        "plain": {"gcups": round(plain_gcups, 2), "queries": int(len(qsel)), "store_dtype": dtype,
+                  "speedup": round(value / plain_gcups, 2)},
+        "checks": {"topk_equals_comparator_rows": ok, "oracle_hits_checked": int(hqi.size),
+                   "oracle_hit_mismatches": hit_mism},
     }
+    return out, ((qc, qo), (rc, ro), res.topk_idx, res.topk_scores)
+
+
+def _serve(srv, dbs, pools, rate: float, n: int, seed: int):
+    """Submit n Poisson-timed requests (1-512 queries, log-uniform; 70 % DNA, 30 % protein)
+    and wait for all of them.  Returns per request (kind, query indices, ServedResult) and
+    the submission window."""
+    from gpusw_amd import synth
+
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / rate, n)
+    sizes = np.floor(np.exp(rng.uniform(np.log(1), np.log(513), n))).astype(int)
+    kinds = (rng.random(n) >= 0.7).astype(int)            # 0 DNA, 1 protein
+    seqs = [synth.unpack(*pools[0][0]), synth.unpack(*pools[1][0])]
+    reqs = []
+    for i in range(n):
+        pick = rng.integers(0, len(seqs[kinds[i]]), sizes[i])
+        reqs.append((kinds[i], pick, synth.pack([seqs[kinds[i]][j] for j in pick])))
+    t0 = time.perf_counter()
+    ids = []
+    next_t = t0
+    for i, (kind, _pick, q) in enumerate(reqs):
+        next_t += gaps[i]
+        delay = next_t - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        ids.append(srv.submit(dbs[kind], q, k=5))
+    offered_s = time.perf_counter() - t0
+    res = [srv.result(r) for r in ids]
+    wall = time.perf_counter() - t0
+    return [(k, p, r) for (k, p, _q), r in zip(reqs, res)], offered_s, wall
+
+
+def _check_served(served, pools, oracle_sample: int, seed: int):
+    """Every served top-k row against the batched top-k of the same query (Aligner.top_k
+    of the whole pool), and the oracle on a sample of served hits."""
+    from gpusw_amd import Scheme
+    from oracle import native
+
+    bad_rows = 0
+    hits = [[], []]
+    for kind, pick, r in served:
+        want_idx, want_sc = pools[kind][2][pick], pools[kind][3][pick]
+        got_idx, got_sc = r.result.topk_idx, r.result.topk_scores
+        bad_rows += int((~((got_idx == want_idx).all(1) & (got_sc == want_sc).all(1))).sum())
+        for a in range(len(pick)):
+            hits[kind].append((int(pick[a]), int(got_idx[a, 0]), int(got_sc[a, 0])))
+    rng = np.random.default_rng(seed)
+    mism, checked = 0, 0
+    for kind, scheme in ((0, Scheme.preset("dna")), (1, Scheme.preset("blosum62"))):
+        if not hits[kind]:
+            continue
+        h = np.array(hits[kind], dtype=np.int64)
+        sel = h[rng.choice(len(h), min(oracle_sample, len(h)), replace=False)]
+        (qc, qo), (rc, ro) = pools[kind][0], pools[kind][1]
+        want = native.score_pairs(qc, qo, rc, ro, sel[:, 0].astype(np.int32), sel[:, 1].astype(np.int32),
+                                  scheme.substitution_table(), scheme.mode, scheme.gap_open, scheme.gap_extend,
+                                  scheme.free_end_gaps_query, scheme.free_end_gaps_ref)
+        mism += int((want != sel[:, 2]).sum())
+        checked += len(sel)
+    return bad_rows, mism, checked
+
+
+def _serving_line(served, offered_s, wall, rate, n):
+    lat = np.array([r.latency_ms for _k, _p, r in served])
+    cells = sum(r.cells for _k, _p, r in served)
+    return {"offered_rate_rps": rate, "requests": n, "achieved_rate_rps": round(n / wall, 3),
+            "offered_window_s": round(offered_s, 3), "gcups": round(cells / wall / 1e9, 2),
+            "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 2),
+                           "p99": round(float(np.percentile(lat, 99)), 2),
+                           "mean": round(float(lat.mean()), 2), "max": round(float(lat.max()), 2)}}
+
+
+def serving_leg(args, dna_pool, prot_pool):
+    """The serving mix at one sub-saturation rate, every served result checked."""
+    from gpusw_amd import Scheme
+    from gpusw_amd.serve import Server
+
+    pools = (dna_pool, prot_pool)
+    with Server(memory_budget=args.budget << 30, launch_cells=args.launch_cells) as srv:
+        dbs = (srv.add_database(dna_pool[1], Scheme.preset("dna")),
+               srv.add_database(prot_pool[1], Scheme.preset("blosum62")))
+        served, offered_s, wall = _serve(srv, dbs, pools, args.serve_rate, args.serve_requests, seed=7007)
+    line = _serving_line(served, offered_s, wall, args.serve_rate, args.serve_requests)
+    bad, mism, checked = _check_served(served, pools, 400, seed=8)
+    line["config"] = {"workload": "serving mix: Poisson arrivals, 1-512 queries per request (log-uniform), 70% DNA "
+                                  "vs the 16S-shape DB (20,000 refs), 30% protein vs the 100,000-ref DB, top_k(5)",
+                      "memory_budget_gib": args.budget, "launch_cells": args.launch_cells}
+    line["checks"] = {"served_rows_checked": int(sum(len(p) for _k, p, _r in served)),
+                      "rows_differing_from_batched_topk": bad, "oracle_hits_checked": checked,
+                      "oracle_hit_mismatches": mism}
+    log(f"serving at {args.serve_rate} req/s: {line['gcups']} GCUPS, p50 {line['latency_ms']['p50']} ms, "
+        f"p99 {line['latency_ms']['p99']} ms, achieved {line['achieved_rate_rps']} req/s, "
+        f"{bad} rows differ, {mism} oracle mismatches")
+    return line
+
+
+def run_protein_topk(args):
+    prot, _pool = protein_leg(args, steps=args.steps, warmup=args.warmup)
+    line = {"metric": METRIC, "value": prot["value"], "unit": "GCUPS", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": prot["ms_per_step"], "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "int16", "data": "synthetic"}
+    line.update({k: prot[k] for k in ("config", "roofline", "plain", "checks")})
+    return line
 
 
 def run_read_pairs(args):
@@ -438,7 +624,7 @@ def run_read_pairs(args):
             "scaling": "weak", "vs_baseline": None, "dtype": "int16", "data": "synthetic",
             "config": {"workload": f"read pairs: 1,000,000 (150-nt read, 200-400-nt window) pairs, dna "
                                    f"{mode}, int16, HIP-graph batches of 131,072", "cells": cells},
-            "roofline": dict(_roofline(history, OPS_PER_CELL[mode], "readpairs"),
+            "roofline": dict(_roofline(history, mode, "readpairs"),
                              all_batches_ms=round(st.kernel_ms, 2), host_wall_ms=round(st.wall_ms, 2),
                              launches=int(st.launches)),
             "plain": {"gcups": round(pcells / pdt / 1e9, 2), "pairs": int(sub.size),
@@ -450,63 +636,38 @@ def run_read_pairs(args):
 
 
 def run_serving(args):
-    """Serving mix: Poisson arrivals of requests holding 1-512 queries (log-uniform count),
-    70% DNA against the 16S-shape database and 30% protein against the protein database,
-    top_k(5); continuous batching under a device-memory budget.  Reports per offered rate
-    the achieved throughput and p50/p99 latency."""
-    from gpusw_amd import Scheme, synth
+    """The serving sweep: one line per offered rate (every served result checked)."""
+    from gpusw_amd import Aligner, Scheme, synth
     from gpusw_amd.serve import Server
 
     (dqc, dqo), (drc, dro) = synth.dna_16s_cross(seed=0)
-    (pqc, pqo), (prc, pro) = synth.protein_topk(seed=0)
-    dq = synth.unpack(dqc, dqo)
-    pq = synth.unpack(pqc, pqo)
+    dna = Aligner(Scheme.preset("dna"), backend="hip").index((drc, dro)).top_k((dqc, dqo), k=5)
+    _prot, prot_pool = protein_leg(args, steps=1, warmup=0)
+    pools = (((dqc, dqo), (drc, dro), dna.topk_idx, dna.topk_scores), prot_pool)
     rates = [float(x) for x in args.rates.split(",")]
     lines = []
     with Server(memory_budget=args.budget << 30, launch_cells=args.launch_cells) as srv:
-        d_dna = srv.add_database((drc, dro), Scheme.preset("dna"))
-        d_prot = srv.add_database((prc, pro), Scheme.preset("blosum62"))
+        dbs = (srv.add_database(pools[0][1], Scheme.preset("dna")),
+               srv.add_database(pools[1][1], Scheme.preset("blosum62")))
         for rate in rates:
-            rng = np.random.default_rng(int(rate * 1000) + 7)
-            n = args.requests
-            gaps = rng.exponential(1.0 / rate, n)
-            sizes = np.floor(np.exp(rng.uniform(np.log(1), np.log(513), n))).astype(int)
-            kinds = rng.random(n) < 0.7
-            reqs = []
-            for i in range(n):
-                src = dq if kinds[i] else pq
-                pick = rng.integers(0, len(src), sizes[i])
-                seqs = [src[j] for j in pick]
-                reqs.append((d_dna if kinds[i] else d_prot, synth.pack(seqs)))
-            t0 = time.perf_counter()
-            ids = []
-            next_t = t0
-            for i, (db, q) in enumerate(reqs):
-                next_t += gaps[i]
-                delay = next_t - time.perf_counter()
-                if delay > 0:
-                    time.sleep(delay)
-                ids.append(srv.submit(db, q, k=5))
-            offered_s = time.perf_counter() - t0
-            res = [srv.result(r) for r in ids]
-            wall = time.perf_counter() - t0
-            lat = np.array([r.latency_ms for r in res])
-            cells = sum(r.cells for r in res)
+            served, offered_s, wall = _serve(srv, dbs, pools, rate, args.requests, seed=int(rate * 1000) + 7)
+            s = _serving_line(served, offered_s, wall, rate, args.requests)
+            bad, mism, checked = _check_served(served, pools, 100, seed=int(rate))
             lines.append({
-                "metric": METRIC, "value": round(cells / wall / 1e9, 2), "unit": "GCUPS", "n_gpus": 1,
-                "steps": 1, "warmup": 0, "ms_per_step": round(wall * 1e3, 2), "higher_is_better": True,
-                "scaling": "weak", "vs_baseline": None, "dtype": "int16", "data": "synthetic",
+                "metric": METRIC, "value": s["gcups"], "unit": "GCUPS", "n_gpus": 1, "steps": 1, "warmup": 0,
+                "ms_per_step": round(wall * 1e3, 2), "higher_is_better": True, "scaling": "weak",
+                "vs_baseline": None, "dtype": "int16", "data": "synthetic",
                 "config": {"workload": "serving mix: Poisson arrivals, 1-512 queries per request (log-uniform), "
                                        "70% DNA vs 16S-shape DB (20,000 refs), 30% protein vs 100,000-ref DB, top_k(5)",
-                           "offered_rate_rps": rate, "requests": n,
-                           "achieved_rate_rps": round(n / wall, 3), "offered_window_s": round(offered_s, 3),
+                           "offered_rate_rps": rate, "requests": args.requests,
+                           "achieved_rate_rps": s["achieved_rate_rps"], "offered_window_s": s["offered_window_s"],
                            "memory_budget_gib": args.budget, "launch_cells": args.launch_cells},
-                "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 2),
-                               "p99": round(float(np.percentile(lat, 99)), 2),
-                               "mean": round(float(lat.mean()), 2), "max": round(float(lat.max()), 2)},
+                "latency_ms": s["latency_ms"],
+                "checks": {"rows_differing_from_batched_topk": bad, "oracle_hits_checked": checked,
+                           "oracle_hit_mismatches": mism},
             })
-            log(f"serving rate {rate}: {lines[-1]['value']} GCUPS, p50 {lines[-1]['latency_ms']['p50']} ms, "
-                f"p99 {lines[-1]['latency_ms']['p99']} ms, achieved {n / wall:.2f} req/s")
+            log(f"serving rate {rate}: {s['gcups']} GCUPS, p50 {s['latency_ms']['p50']} ms, "
+                f"p99 {s['latency_ms']['p99']} ms, achieved {s['achieved_rate_rps']} req/s, {bad} rows differ")
     return lines
 
 
@@ -536,7 +697,13 @@ def main() -> int:
     ap.add_argument("--rates", default="2,5,50,100,200,500,1000,2000",
                     help="serving: offered request rates (req/s); 50-2,000 is the configured sweep, "
                          "2 and 5 show the latency below saturation")
-    ap.add_argument("--requests", type=int, default=40, help="serving: requests per rate")
+    ap.add_argument("--requests", type=int, default=40, help="serving sweep: requests per rate")
+    ap.add_argument("--serve-rate", type=float, default=5.0,
+                    help="default run's serving leg: offered rate (req/s), below the ~8 req/s one GPU completes")
+    ap.add_argument("--serve-requests", type=int, default=400, help="default run's serving leg: requests")
+    ap.add_argument("--protein-steps", type=int, default=3, help="default run's protein leg: timed calls")
+    ap.add_argument("--no-protein", action="store_true", help="default run: skip the protein top-k leg")
+    ap.add_argument("--no-serving", action="store_true", help="default run: skip the serving leg")
     ap.add_argument("--budget", type=int, default=64, help="serving: device memory budget (GiB)")
     ap.add_argument("--launch-cells", type=float, default=2e11, help="serving: cells per launch")
     ap.add_argument("--plain-queries", type=int, default=32)

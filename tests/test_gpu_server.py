@@ -103,3 +103,53 @@ def test_budget_admission_and_rejection(dbs):
         w_idx, w_sc = _expected(dna, dna_refs, q, 4)
         assert np.array_equal(res.result.topk_idx, w_idx)
         assert np.array_equal(res.result.topk_scores, w_sc)
+
+
+def test_serving_mix_at_full_database_sizes():
+    """The serving mix at its workload: the 16S-shape DNA database (20,000 references) and
+    the protein database (100,000 references) resident in one server, 24 Poisson-timed
+    requests of 1-64 queries (70 % DNA, 30 % protein).  Every served top-k equals
+    Aligner.top_k of the same queries alone (src/gpusw/aligner.py:169-195), and the oracle
+    agrees with a sample of the served hits."""
+    import time
+
+    from gpusw_amd import synth
+    from oracle import native
+
+    (dqc, dqo), (drc, dro) = synth.dna_16s_cross(seed=0)
+    (pqc, pqo), (prc, pro) = synth.protein_topk(seed=0)
+    pools = [(synth.unpack(dqc, dqo), Scheme.preset("dna"), (drc, dro)),
+             (synth.unpack(pqc, pqo), Scheme.preset("blosum62"), (prc, pro))]
+    rng = np.random.default_rng(404)
+    n = 24
+    kinds = (rng.random(n) >= 0.7).astype(int)
+    sizes = np.floor(np.exp(rng.uniform(0, np.log(65), n))).astype(int)
+    gaps = rng.exponential(1 / 20.0, n)
+    reqs = [(int(k), rng.integers(0, len(pools[k][0]), int(m))) for k, m in zip(kinds, sizes)]
+    with Server(memory_budget=16 << 30) as srv:
+        dbs = [srv.add_database(p[2], p[1]) for p in pools]
+        ids = []
+        for (k, pick), g in zip(reqs, gaps):
+            time.sleep(g)
+            ids.append(srv.submit(dbs[k], synth.pack([pools[k][0][j] for j in pick]), k=5))
+        served = [srv.result(i) for i in ids]
+        st = srv.stats()
+    assert st["completed"] == n and st["rejected"] == 0
+    alone = [Aligner(p[1], backend="hip").index(p[2]) for p in pools]
+    hits = [[], []]
+    for (k, pick), r in zip(reqs, served):
+        want = alone[k].top_k(synth.pack([pools[k][0][j] for j in pick]), k=5)
+        assert np.array_equal(r.result.topk_idx, want.topk_idx)
+        assert np.array_equal(r.result.topk_scores, want.topk_scores)
+        hits[k] += [(int(q), int(i), int(s)) for q, i, s in
+                    zip(pick, r.result.topk_idx[:, 0], r.result.topk_scores[:, 0])]
+    for k, (qc, qo) in enumerate([(dqc, dqo), (pqc, pqo)]):
+        if not hits[k]:
+            continue
+        h = np.array(hits[k][:60], dtype=np.int64)
+        rc, ro = pools[k][2]
+        sch = pools[k][1]
+        want = native.score_pairs(qc, qo, rc, ro, h[:, 0].astype(np.int32), h[:, 1].astype(np.int32),
+                                  sch.substitution_table(), sch.mode, sch.gap_open, sch.gap_extend,
+                                  sch.free_end_gaps_query, sch.free_end_gaps_ref)
+        assert np.array_equal(want, h[:, 2])
