@@ -54,6 +54,7 @@ struct KArgs {
     int64_t bnd_stride;          //   each an H plane then an E plane of bnd_stride words (>= item rows)
     const int64_t *bpos;         //   row offsets of the sorted references (n + 1)
     int32_t tab_copies;          // pairs table kernel: interleaved copies of the table
+    int32_t prow;                // pairs kernel: last profile row (rows 0..prow are built)
 };
 
 
@@ -79,12 +80,13 @@ std::vector<Variant> &variant_table();
 const Variant *pick_variant(int lq, double mean_lr, bool packed, bool local, int ns, size_t lds_limit,
                             bool for_pairs);
 int max_width();
-// Tile of an explicit pair with query length lq; *tab: run it with the table kernel.
-const Variant *pick_pairs_tile(int lq, double mean_lr, bool packed, bool local, int ns, size_t lds_limit,
-                               bool *tab);
+// Tile of an explicit pair with query length lq; *tab: run it with the table kernel, else
+// with the profile kernel (profiles of prof_rows rows; prof_rows <= 0: table kernel only).
+const Variant *pick_pairs_tile(int lq, double mean_lr, bool packed, bool local, int ns, int prof_rows,
+                               size_t lds_limit, bool *tab);
 size_t cross_lds_bytes(const Variant &v, int ns);
 size_t pairs_wave_bytes(const Variant &v, int ns);               // one wave's private profiles
-int pairs_waves(const Variant &v, int ns, size_t lds_limit);      // waves per workgroup (0: none fit)
+int pairs_waves(const Variant &v, int rows, size_t lds_limit);    // waves per workgroup (0: none fit)
 int occupancy_grid(DeviceCtx *ctx, const void *fn, int threads, size_t lds, int *grid);
 
 // A query scored against a whole reference set: code start/length in the query code

@@ -89,6 +89,7 @@ struct SeqSet {
     int32_t n = 0;
     int32_t max_len = 0;
     int64_t total = 0;
+    int32_t max_code = 255;            // largest code of the set (read once at creation)
     std::vector<int32_t> order_desc;   // indices sorted by length, longest first
     std::vector<int32_t> rank;         // rank[i] = position of sequence i in order_desc
     int32_t *d_order_desc = nullptr;   // device copy of order_desc (non-empty only)

@@ -13,6 +13,11 @@ namespace gsw {
 
 static thread_local std::string g_error;
 
+__device__ __forceinline__ int wave_max(int v) {
+    for (int off = 32; off >= 1; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
 void set_error(const char *fmt, ...) {
     char buf[1024];
     va_list ap;
@@ -199,6 +204,15 @@ int32_t empty_edge_score(const SchemeDev &s, int lq, int lr) {
     return (semi && s.free_r) ? 0 : (int32_t)(s.go + (int64_t)(lq - 1) * s.ge);
 }
 
+// Largest byte of codes[0, n) into *out (which starts at zero).
+__global__ void max_code_kernel(const uint8_t *codes, int64_t n, int32_t *out) {
+    int m = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        m = max(m, (int)codes[i]);
+    m = wave_max(m);
+    if ((threadIdx.x & 63) == 0 && m > 0) atomicMax(out, m);
+}
+
 }  // namespace gsw
 
 using namespace gsw;
@@ -319,6 +333,21 @@ int gsw_seqset_create(const uint8_t *d_codes, const int32_t *d_off, const int32_
     for (int i = 0; i < n; ++i) set->rank[set->order_desc[i]] = i;
     set->n_nonempty = 0;
     while (set->n_nonempty < n && set->len[set->order_desc[set->n_nonempty]] > 0) set->n_nonempty++;
+    // the largest code: explicit-pairs profiles need rows only up to it (codes at or past the
+    // table side read its last row)
+    if (set->total > 0) {
+        GSW_HIP_TRY(hipMemsetAsync(ctx->counters + 63, 0, sizeof(int32_t), ctx->stream));
+        const int64_t first = h_off[0];
+        hipLaunchKernelGGL(max_code_kernel, dim3((unsigned)std::min<int64_t>(1024, (set->total + 255) / 256)),
+                           dim3(256), 0, ctx->stream, d_codes + first, set->total, ctx->counters + 63);
+        GSW_HIP_TRY(hipGetLastError());
+        int32_t mc = 0;
+        GSW_HIP_TRY(hipMemcpyAsync(&mc, ctx->counters + 63, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+        GSW_HIP_TRY(hipStreamSynchronize(ctx->stream));
+        set->max_code = mc;
+    } else {
+        set->max_code = 0;
+    }
     set->bpos.assign((size_t)n + 1, 0);
     for (int p = 0; p < n; ++p) set->bpos[p + 1] = set->bpos[p] + set->len[set->order_desc[p]];
     if (n > 0) {

@@ -353,4 +353,155 @@ __device__ __forceinline__ SweepOut sweep_group(const uint8_t *__restrict__ rseq
     return out;
 }
 
+// The sweep of sweep_group for explicit pairs with private profiles, pipelined for two
+// waves per SIMD: the step loop is unrolled by two with two profile buffers, so that the
+// profile row of row i+1 is read from LDS during row i's columns and the reference code of
+// row i+2 is read from global memory during row i; neither load is waited for until the
+// step that uses it (no register copy forces an early wait).  (Unrolling by four to read
+// codes three steps ahead measured 10-20 % slower.)  prow is the largest profile row (the
+// last row present: codes past it read it); rows are ROWB bytes apart.
+template <typename V, int G, int R, bool LOCAL>
+__device__ __forceinline__ SweepOut sweep_pairs_pipe(const uint8_t *__restrict__ rseq, int lr, int nsteps,
+                                                     const char *prof_lane, int t, const SweepConst &k,
+                                                     int lqa, int lqb, int cbase, int prow) {
+    using O = VOps<V>;
+    using L = ProfLayout<G, R>;
+    constexpr int KC = L::KC;
+    const bool semi = k.mode == 2;
+    const bool track_endcol = semi && k.free_q;
+
+    V hup[R], fn[R];
+    const V GOV = O::splat(k.go), GEV = O::splat(k.ge);
+    const V GOP = O::splat(-k.go), GEP = O::splat(-k.ge);
+    const V ZERO = O::splat(0);
+#pragma unroll
+    for (int kk = 0; kk < R; ++kk) {
+        if (LOCAL) {
+            hup[kk] = ZERO;
+            fn[kk] = ZERO;
+        } else {
+            const int h0 = row0_value(k, cbase + t * R + kk + 1);
+            hup[kk] = O::splat(h0);
+            fn[kk] = O::splat(max(h0 + k.go, O::NEG + k.ge));   // F(1, col)
+        }
+    }
+    V hout = LOCAL ? ZERO : O::splat(row0_value(k, cbase + (t + 1) * R));
+    V hin = LOCAL ? ZERO : O::splat(t == 0 ? col0_value(k, 1) : row0_value(k, cbase + t * R));
+    V din = LOCAL ? ZERO : O::splat(row0_value(k, cbase));
+    V ein = LOCAL ? ZERO : O::splat(O::NEG);
+    V eout = ein;
+    V best = ZERO;
+    V endcol = O::splat(O::NEG);
+    const bool ladder = !LOCAL && (k.mode == 1 || (k.mode == 2 && !k.free_q));
+    V c0v = ladder ? O::splat(col0_value(k, 2)) : ZERO;
+    const int rlast = max(lr - 1, 0);   // idle groups (lr = 0) read their row 0 harmlessly
+    auto code = [&](int row) -> int { return min((int)rseq[min(max(row, 0), rlast)], prow); };
+    auto load_row = [&](int c, V *dst) {
+        const char *pc = prof_lane + c * L::ROWB;
+#pragma unroll
+        for (int kc = 0; kc < KC; ++kc) {
+            const int4 x = *reinterpret_cast<const int4 *>(pc + kc * L::CH);
+            dst[4 * kc + 0] = O::from_bits(x.x);
+            dst[4 * kc + 1] = O::from_bits(x.y);
+            dst[4 * kc + 2] = O::from_bits(x.z);
+            dst[4 * kc + 3] = O::from_bits(x.w);
+        }
+    };
+    V pa[KC * 4], pb[KC * 4];
+    load_row(code(-t), pa);              // row 0 - t: step 0
+    int cx = code(1 - t);                // row 1 - t: its profile is read during step 0
+    int cy = 0;
+
+    // One step: row = s - t with profile `use`; reads the profile of row + 1 (code cin)
+    // into `nxt` and the code of row + 2 into cout.
+    auto step = [&](int s, V *use, V *nxt, int cin, int &cout) {
+        const int row = s - t;
+        cout = code(row + 2);
+        load_row(cin, nxt);
+        if ((unsigned)row < (unsigned)lr) {
+            V e = ein;
+            V hgo = LOCAL ? O::subu(hin, GOP) : O::add(hin, GOV);
+            V h = hin;
+            V xd = O::add(din, use[0]);
+#pragma unroll
+            for (int kk = 0; kk < R; ++kk) {
+                const V xn = (kk + 1 < R) ? O::add(hup[kk], use[kk + 1]) : xd;
+                if (LOCAL) {
+                    e = O::mx(O::subu(e, GEP), hgo);
+                    h = O::mx(O::mx(xd, fn[kk]), e);
+                    best = O::mx(best, h);
+                    hup[kk] = h;
+                    hgo = O::subu(h, GOP);
+                    fn[kk] = O::mx(O::subu(fn[kk], GEP), hgo);
+                } else {
+                    e = O::mx(O::add(e, GEV), hgo);
+                    h = O::mx(O::mx(xd, fn[kk]), e);
+                    hup[kk] = h;
+                    hgo = O::add(h, GOV);
+                    fn[kk] = O::mx(O::add(fn[kk], GEV), hgo);
+                }
+                xd = xn;
+            }
+            hout = h;
+            eout = e;
+            if (!LOCAL && track_endcol) endcol = O::mx(endcol, h);
+        }
+        din = hin;
+        hin = O::from_bits(wave_shr1(O::bits(hout)));
+        ein = O::from_bits(wave_shr1(O::bits(eout)));
+        if (t == 0) {
+            hin = c0v;
+            ein = LOCAL ? ZERO : O::splat(O::NEG);
+        }
+        if (ladder) c0v = O::add(c0v, GEV);
+    };
+    for (int s = 0; s < nsteps; s += 2) {   // a step past nsteps has no row in range: harmless
+        step(s, pa, pb, cx, cy);
+        step(s + 1, pb, pa, cy, cx);
+    }
+
+    SweepOut out;
+    if (LOCAL) {
+        int ba = O::lo(best), bb = O::hi(best);
+        ba = group_max_i32<G>(ba);
+        bb = O::PACKED ? group_max_i32<G>(bb) : ba;
+        out.a = ba;
+        out.b = bb;
+        return out;
+    }
+    int corner_a = RES_NONE, corner_b = RES_NONE, endrow_a = RES_NONE, endrow_b = RES_NONE;
+    if (lr > 0) {
+#pragma unroll
+        for (int kk = 0; kk < R; ++kk) {
+            const int col = cbase + t * R + kk;
+            const int va = O::lo(hup[kk]);
+            const int vb = O::hi(hup[kk]);
+            if (col == lqa - 1) corner_a = va;
+            if (col == lqb - 1) corner_b = vb;
+            if (col < lqa) endrow_a = max(endrow_a, va);
+            if (col < lqb) endrow_b = max(endrow_b, vb);
+        }
+    }
+    int ca = corner_a, cb = corner_b;
+    if (semi) {
+        if (track_endcol) {
+            if (cbase + t * R + R - 1 == lqa - 1) ca = max(ca, O::lo(endcol));
+            if (cbase + t * R + R - 1 == lqb - 1) cb = max(cb, O::hi(endcol));
+        }
+        if (k.free_r) {
+            ca = max(ca, endrow_a);
+            cb = max(cb, endrow_b);
+        }
+    }
+    ca = group_max_i32<G>(ca);
+    cb = O::PACKED ? group_max_i32<G>(cb) : ca;
+    if (semi && k.free_q && k.free_r) {
+        ca = max(ca, 0);
+        cb = max(cb, 0);
+    }
+    out.a = ca;
+    out.b = cb;
+    return out;
+}
+
 }  // namespace gsw

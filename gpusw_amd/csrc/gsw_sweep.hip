@@ -197,9 +197,11 @@ __attribute__((amdgpu_waves_per_eu(WavesPerEU<LOCAL, G, R>::value))) stripe_kern
 // ---------------------------------------------------------------- pairs kernel
 // Waves take blocks of 64/G tasks from a global counter.  Every lane writes the
 // profile entries of its own columns into the wave's LDS region and reads only those,
-// so no barrier is needed.  Tasks of one block share their front-padding shift (the host pads blocks).
-// Private profiles limit it to one workgroup per CU for DNA, so registers are not capped
-// for occupancy; the sweep is software-pipelined instead.
+// so no barrier is needed.  Tasks of one block share their front-padding shift (the host
+// pads blocks).  The profile holds rows 0..prow only (prow: the largest reference code of
+// the set, or the table's last row), so for DNA references without N a wave's profiles of
+// up to 20 columns per lane take 20 KiB and eight waves fit a CU; the sweep is pipelined
+// for that occupancy (sweep_pairs_pipe).
 template <typename V, int G, int R, bool LOCAL, int WAVES>
 __global__ void __launch_bounds__(WAVES * 64) pairs_kernel(KArgs a) {
     using O = VOps<V>;
@@ -209,10 +211,15 @@ __global__ void __launch_bounds__(WAVES * 64) pairs_kernel(KArgs a) {
     extern __shared__ __align__(16) char lds[];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int t = lane % G, g = lane / G;
+    const int nrows = a.prow + 1;
     // blocks of NSLOT groups interleave their private profiles (see ProfLayout)
-    const int block_bytes = a.ns * L::ROWB;
+    const int block_bytes = nrows * L::ROWB;
     char *lane_lds = lds + (size_t)wave * (GPW / L::NSLOT) * block_bytes + (size_t)(g / L::NSLOT) * block_bytes +
                      L::lane_offset(g, t);
+    // tables of up to 64 entries (ns <= 8, DNA) are held across the lanes of a register,
+    // lane l holding entry l, and read with ds_bpermute: no memory latency in the builds
+    const bool lane_table = a.ns * a.ns <= 64;
+    const int subv = lane_table && lane < a.ns * a.ns ? a.sub[lane] : 0;
     const PairTask *tasks = static_cast<const PairTask *>(a.items);
     const int nitems = a.nitems_ptr ? __builtin_amdgcn_readfirstlane(*a.nitems_ptr) : a.nitems;
     const int nblocks = (nitems + GPW - 1) / GPW;
@@ -232,35 +239,34 @@ __global__ void __launch_bounds__(WAVES * 64) pairs_kernel(KArgs a) {
         const int qsb = valid ? a.qoff[qb] : 0, lqb = valid ? a.qoff[qb + 1] - qsb : 0;
         const int rs = valid ? a.roff[task.ref] : 0;
         const int lr = valid ? a.roff[task.ref + 1] - rs : 0;
-        // this lane's profile columns: per chunk of 4 columns, the 8 query codes are read
-        // once and the ns profile rows are written (loops kept rolled: low register use)
-#pragma unroll 1
-        for (int kc = 0; kc < KC; ++kc) {
-            int ca[4], cb[4];   // code, or -1 = back padding (NEG), -2 = front padding (0)
+        // this lane's profile columns: all query codes are read first (one round trip), then
+        // each row's entries are looked up and written 4 columns at a time
+        int ca[KC * 4], cb[KC * 4];   // code, or -1 = back padding (NEG), -2 = front padding (0)
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int kk = kc * 4 + e;
-                const int col = t * R + kk - shift;
-                ca[e] = (kk >= R || col >= lqa) ? -1 : col < 0 ? -2 : min((int)a.qcodes[qsa + col], a.ns - 1);
-                cb[e] = (kk >= R || col >= lqb) ? -1 : col < 0 ? -2 : min((int)a.qcodes[qsb + col], a.ns - 1);
-            }
+        for (int kk = 0; kk < KC * 4; ++kk) {
+            const int col = t * R + kk - shift;
+            ca[kk] = (kk >= R || col >= lqa) ? -1 : col < 0 ? -2 : min((int)a.qcodes[qsa + col], a.ns - 1);
+            cb[kk] = (kk >= R || col >= lqb) ? -1 : col < 0 ? -2 : min((int)a.qcodes[qsb + col], a.ns - 1);
+        }
+        auto entry = [&](int c, int code) -> int {
+            const int v = lane_table ? __builtin_amdgcn_ds_bpermute((c * a.ns + max(code, 0)) * 4, subv)
+                                     : a.sub[c * a.ns + max(code, 0)];
+            return code == -1 ? O::NEG : code == -2 ? 0 : v;
+        };
 #pragma unroll 1
-            for (int c = 0; c < a.ns; ++c) {
-                const int32_t *srow = a.sub + c * a.ns;
+        for (int c = 0; c < nrows; ++c) {
+#pragma unroll
+            for (int kc = 0; kc < KC; ++kc) {
                 uint32_t w[4];
 #pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    const int va = ca[e] == -1 ? O::NEG : ca[e] == -2 ? 0 : srow[ca[e]];
-                    const int vb = cb[e] == -1 ? O::NEG : cb[e] == -2 ? 0 : srow[cb[e]];
-                    w[e] = O::pack(va, vb);
-                }
+                for (int e = 0; e < 4; ++e) w[e] = O::pack(entry(c, ca[kc * 4 + e]), entry(c, cb[kc * 4 + e]));
                 *reinterpret_cast<int4 *>(lane_lds + c * L::ROWB + kc * L::CH) =
                     int4{(int)w[0], (int)w[1], (int)w[2], (int)w[3]};
             }
         }
         const int nsteps = __builtin_amdgcn_readfirstlane(wave_max_i32(lr)) + G - 1;
-        const SweepOut res = sweep_group<V, G, R, LOCAL, false, true>(a.rcodes + rs, lr, nsteps, lane_lds, t,
-                                                                      a.k, lqa, lqb, -shift);
+        const SweepOut res = sweep_pairs_pipe<V, G, R, LOCAL>(a.rcodes + rs, lr, nsteps, lane_lds, t, a.k, lqa, lqb,
+                                                              -shift, a.prow);
         if (t == 0 && valid) {
             emit(a, task.oa, qa, task.ref, res.a);
             if (task.ob >= 0) emit(a, task.ob, qb, task.ref, res.b);
@@ -381,7 +387,7 @@ static void launch_stripe(const KArgs &a, int grid, size_t lds, hipStream_t st) 
 std::vector<Variant> &variant_table() {
     static std::vector<Variant> table = [] {
         std::vector<Variant> v = {
-            GSW_VARIANT4(8, 8),   GSW_VARIANT4(8, 12),  GSW_VARIANT4(8, 16),  GSW_SVARIANT4(8, 20),
+            GSW_VARIANT4(8, 8),   GSW_VARIANT4(8, 12),  GSW_VARIANT4(8, 16),  GSW_VARIANT4(8, 19),  GSW_SVARIANT4(8, 20),
             GSW_SVARIANT4(8, 24), GSW_SVARIANT4(8, 32), GSW_VARIANT4(8, 34),  GSW_VARIANT4(8, 36),
             GSW_VARIANT4(8, 40),  GSW_VARIANT4(8, 44),  GSW_VARIANT4(8, 48),  GSW_VARIANT4(16, 24),
             GSW_VARIANT4(16, 28), GSW_VARIANT4(16, 32), GSW_VARIANT4(16, 36), GSW_VARIANT4(16, 40),
@@ -536,9 +542,9 @@ size_t cross_lds_bytes(const Variant &v, int ns) {
     const size_t ch = (size_t)std::max(v.G, 16) * 16;
     return (size_t)ns * ((v.R + 3) / 4) * ch;
 }
-size_t pairs_wave_bytes(const Variant &v, int ns) { return (size_t)ns * ((v.R + 3) / 4) * 64 * 16; }
-int pairs_waves(const Variant &v, int ns, size_t lds_limit) {
-    return (int)std::min<size_t>(PAIRS_WAVES, lds_limit / pairs_wave_bytes(v, ns));
+size_t pairs_wave_bytes(const Variant &v, int rows) { return (size_t)rows * ((v.R + 3) / 4) * 64 * 16; }
+int pairs_waves(const Variant &v, int rows, size_t lds_limit) {
+    return (int)std::min<size_t>(PAIRS_WAVES, lds_limit / pairs_wave_bytes(v, rows));
 }
 int pairs_tab_copies(const Variant &v, int ns) {
     const size_t nq = (size_t)ns + 2;
@@ -550,37 +556,50 @@ size_t pairs_tab_lds_bytes(const Variant &v, int ns) {
     return 4 * (size_t)ns * (v.packed ? nq * nq : nq) * pairs_tab_copies(v, ns);
 }
 
-// Explicit pairs run the table kernel (measured faster than per-lane profiles on the
-// 1,000,000 read-pair workload: 3 waves per SIMD against 1-2).  GSW_PAIRS_TAB=0 prefers
-// per-lane profiles where they fit LDS.
-static bool pairs_prefer_profiles() {
-    const char *e = std::getenv("GSW_PAIRS_TAB");
-    return e && e[0] == '0';
+// Kernel of explicit pairs: by default the substitution-table kernel (three waves per SIMD,
+// one address add per column), else per-lane profiles where the table does not fit.  The
+// profile kernel issues 11 % fewer VALU instructions on the 1,000,000 read pairs (178 against
+// 201 per step of 20 columns) but runs two waves per SIMD, whose memory waits cost as much:
+// both take 1.00-1.02 ms per batch of 131,072 pairs.  GSW_PAIRS_TAB=0 runs pairs on profiles
+// wherever they fit, GSW_PAIRS_TAB=1 on the table only.  Returns 1 (table), 0 (profiles) or
+// -1 (default).
+static int pairs_kernel_pref() {
+    const char *e = std::getenv("GSW_PAIRS_TAB");   // read per call: a caller may change it
+    return e && e[0] == '1' ? 1 : e && e[0] == '0' ? 0 : -1;
 }
 
-// Tile of an explicit pair with query length lq; *tab: run it with the table kernel.
-const Variant *pick_pairs_tile(int lq, double mean_lr, bool packed, bool local, int ns, size_t lds_limit,
-                               bool *tab) {
-    *tab = false;
-    if (pairs_prefer_profiles()) {
-        const Variant *v = pick_variant(lq, mean_lr, packed, local, ns, lds_limit, true);
-        if (v) return v;
-    }
+const Variant *pick_pairs_tile(int lq, double mean_lr, bool packed, bool local, int ns, int prof_rows,
+                               size_t lds_limit, bool *tab) {
+    const int pref = prof_rows <= 0 ? 1 : pairs_kernel_pref();
     const Variant *best = nullptr;
+    bool best_tab = false;
     double best_cost = 1e30;
     int g_force_g, g_force_r;
     forced_tile(&g_force_g, &g_force_r);
     for (const Variant &v : variant_table()) {
-        if (!v.pairs_tab_fn || v.packed != packed || v.local != local || v.G * v.R < lq) continue;
-        if (pairs_tab_lds_bytes(v, ns) > lds_limit) continue;
+        if (v.packed != packed || v.local != local || v.G * v.R < lq) continue;
         if (g_force_g && (v.G != g_force_g || v.R != g_force_r)) continue;
+        const int pw = prof_rows > 0 ? pairs_waves(v, prof_rows, lds_limit) : 0;
+        const bool tab_ok = v.pairs_tab_fn && pairs_tab_lds_bytes(v, ns) <= lds_limit;
+        bool use_tab;
+        if (pref == 1) {
+            if (!tab_ok) continue;
+            use_tab = true;
+        } else if (pref == 0) {
+            if (pw == 0 && !tab_ok) continue;
+            use_tab = pw == 0;
+        } else {
+            if (pw == 0 && !tab_ok) continue;
+            use_tab = tab_ok;
+        }
         const double c = tile_cost(v.G, v.R, lq, mean_lr);
         if (c < best_cost) {
             best_cost = c;
             best = &v;
+            best_tab = use_tab;
         }
     }
-    *tab = best != nullptr;
+    *tab = best_tab;
     return best;
 }
 
@@ -1185,14 +1204,14 @@ struct PairGraph {
 static std::mutex g_graph_mu;
 static std::map<std::tuple<int, int, int>, std::unique_ptr<PairGraph>> g_pair_graphs;
 
-static int pair_graph(DeviceCtx *ctx, const Variant &v, bool tab, int slot, int ns, PairGraph **out) {
+static int pair_graph(DeviceCtx *ctx, const Variant &v, bool tab, int slot, int ns, int rows, PairGraph **out) {
     std::lock_guard<std::mutex> lk(g_graph_mu);
     auto key = std::make_tuple(ctx->device, 2 * v.id + (tab ? 1 : 0), slot);
     auto &pg = g_pair_graphs[key];
-    const int waves = tab ? PAIRS_TAB_WAVES : pairs_waves(v, ns, (size_t)ctx->max_smem);
+    const int waves = tab ? PAIRS_TAB_WAVES : pairs_waves(v, rows, (size_t)ctx->max_smem);
     if (waves == 0) return fail(GSW_ERR_LIMIT, "pairs tile (%d, %d) does not fit in LDS", v.G, v.R);
-    const size_t lds = tab ? pairs_tab_lds_bytes(v, ns) : (size_t)waves * pairs_wave_bytes(v, ns);
-    if (pg && pg->lds == lds) {
+    const size_t lds = tab ? pairs_tab_lds_bytes(v, ns) : (size_t)waves * pairs_wave_bytes(v, rows);
+    if (pg && pg->lds == lds && pg->waves == waves) {
         *out = pg.get();
         return GSW_OK;
     }
@@ -1356,6 +1375,8 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
     const double mean_lr = rs.n_nonempty ? (double)rs.total / rs.n_nonempty : 1.0;
     const int T = n >= 65536 ? host_threads() : 1;
     const bool local16 = dtype_policy == GSW_DTYPE_INT16 && local && int16_entries(s);
+    // profile rows of the profile kernel: codes past the set's largest one never occur
+    const int prow = std::min(s.ns - 1, rs.max_code);
     const size_t nv = 2 * variant_table().size();   // kinds: 2 * variant id + (table kernel)
     // The pair list runs in segments.  A call of at least four batches classifies, plans and
     // launches a head of 21/8 batches first and classifies the rest while the device runs
@@ -1461,7 +1482,8 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
                 const size_t ci = (size_t)(use16 ? 1 : 0) * (wmax + 1) + lq;
                 if (!vdone[ci]) {
                     bool tb = false;
-                    vcache[ci] = pick_pairs_tile(lq, mean_lr, use16, local, s.ns, (size_t)ctx->max_smem, &tb);
+                    vcache[ci] = pick_pairs_tile(lq, mean_lr, use16, local, s.ns, prow + 1, (size_t)ctx->max_smem,
+                                                 &tb);
                     vtab[ci] = tb;
                     vdone[ci] = 1;
                 }
@@ -1560,7 +1582,7 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
             const Batch &B = batches[bb];
             if (slot_of.count({B.kind, B.slot})) continue;
             PairGraph *pg = nullptr;
-            GSW_TRY(pair_graph(ctx, *B.v, B.tab, B.slot, s.ns, &pg));
+            GSW_TRY(pair_graph(ctx, *B.v, B.tab, B.slot, s.ns, prow + 1, &pg));
             slot_of[{B.kind, B.slot}] = pg;
             KArgs a = base_args(s, qs, rs, d_out);
             a.sat = sb.d_sat;
@@ -1573,6 +1595,7 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
             void *params[] = {&a};
             hipKernelNodeParams kp{};
             a.tab_copies = B.tab ? pairs_tab_copies(*B.v, s.ns) : 1;
+            a.prow = prow;
             kp.func = const_cast<void *>(B.tab ? B.v->pairs_tab_fn : B.v->pairs_fn);
             kp.gridDim = dim3(pg->grid);
             kp.blockDim = dim3(pg->waves * 64);

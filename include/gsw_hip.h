@@ -114,8 +114,8 @@ int gsw_score_pairs_plain(const gsw_scheme *s, int32_t dtype, int32_t threads,
 /* ---- indexed sequence sets ------------------------------------------------
  * A set of n sequences already resident on the device (codes + n+1 offsets, as
  * produced by the reference's encode.funnel, src/gpusw/encode.py:164-220).  The
- * set borrows the device buffers (caller keeps them alive) and copies the host
- * offsets to build its length index.  Replaces the Aligner's per-set device state
+ * set borrows the device buffers (caller keeps them alive, codes unchanged: the set
+ * records their largest value) and copies the host offsets to build its length index.  Replaces the Aligner's per-set device state
  * (src/gpusw/aligner.py:69-87). */
 typedef struct gsw_seqset_s *gsw_seqset;
 int gsw_seqset_create(const uint8_t *d_codes, const int32_t *d_off, const int32_t *h_off,

@@ -48,8 +48,8 @@ def test_every_forced_tile_exact(mode):
 @pytest.mark.parametrize("mode", ["local", "global", "semiglobal"])
 @pytest.mark.parametrize("tab", ["1", "0"])
 def test_every_forced_tile_exact_pairs(mode, tab, monkeypatch):
-    """Explicit pairs on every compiled tile, with the table kernel (default) and with
-    per-lane profiles (GSW_PAIRS_TAB=0, where the profiles fit LDS)."""
+    """Explicit pairs on every compiled tile, with the table kernel (GSW_PAIRS_TAB=1) and
+    with per-lane profiles (GSW_PAIRS_TAB=0, where the profiles fit LDS)."""
     monkeypatch.setenv("GSW_PAIRS_TAB", tab)
     rng = np.random.default_rng(29)
     scheme = Scheme(mode=mode)

@@ -123,19 +123,25 @@ def test_protein_pairs_table_kernel(mode):
 
 
 @pytest.mark.parametrize("mode", ["local", "global", "semiglobal"])
-@pytest.mark.parametrize("tab", ["0", "1"])
-def test_dna_pairs_both_kernels(mode, tab, monkeypatch):
-    """DNA pairs through the table kernel (default) and through per-lane profiles."""
-    monkeypatch.setenv("GSW_PAIRS_TAB", tab)
+@pytest.mark.parametrize("tab", ["0", "1", None])
+@pytest.mark.parametrize("ref_codes", [4, 5])
+def test_dna_pairs_both_kernels(mode, tab, ref_codes, monkeypatch):
+    """DNA pairs through the table kernel (GSW_PAIRS_TAB=1), through per-lane profiles
+    (GSW_PAIRS_TAB=0) and the default choice, with references of ACGT only (profiles of four
+    rows) and with N (all five rows); the queries hold N either way."""
+    if tab is None:
+        monkeypatch.delenv("GSW_PAIRS_TAB", raising=False)
+    else:
+        monkeypatch.setenv("GSW_PAIRS_TAB", tab)
     rng = np.random.default_rng(42)
     scheme = Scheme.preset("dna").replace(mode=mode)
-    queries = [rng.integers(0, 4, int(L)).astype(np.uint8) for L in rng.integers(1, 400, 300)]
-    refs = [rng.integers(0, 4, int(L)).astype(np.uint8) for L in rng.integers(1, 500, 80)]
+    queries = [rng.integers(0, 5, int(L)).astype(np.uint8) for L in rng.integers(1, 400, 300)]
+    refs = [rng.integers(0, ref_codes, int(L)).astype(np.uint8) for L in rng.integers(1, 500, 80)]
     qi = rng.integers(0, 300, 5000).astype(np.int32)
     rj = rng.integers(0, 80, 5000).astype(np.int32)
     al = Aligner(scheme, backend="hip").index(synth.pack(refs)).set_queries(synth.pack(queries))
     got = al.score_pairs(qi, rj)
-    assert al._b.last_stats.top_striped == (3 if tab == "1" else 2)
+    assert al._b.last_stats.top_striped == (2 if tab == "0" else 3)   # default: the table kernel
     want = native.score_encoded_pairs(al._queries, al._refs, qi, rj, scheme)
     assert np.array_equal(got.astype(np.int64), want)
 
