@@ -468,7 +468,7 @@ def protein_leg(args, steps: int | None = None, warmup: int | None = None):
         "warmup": warmup, "dtype": "int16",
         "config": {"workload": "protein top-k: 1000 queries (lognormal median 350, 30-2000) x 100000 "
                                "references (lognormal median 300, 30-3000), BLOSUM62 local -11/-1, top_k(5)",
-                   "cells": cells, "query_batch": qb},
+                   "cells": cells, "query_batch": qb or "Aligner default (2 GiB score block)"},
         "roofline": _call_roofline(history, cells, kernel_s, steps),
         "plain": {"gcups": round(plain_gcups, 2), "queries": int(len(qsel)), "store_dtype": dtype,
                   "speedup": round(value / plain_gcups, 2)},
@@ -703,8 +703,9 @@ def main() -> int:
                     help="default run's serving leg: offered rate (req/s), below the ~8 req/s one GPU completes")
     ap.add_argument("--serve-requests", type=int, default=400, help="default run's serving leg: requests")
     ap.add_argument("--protein-steps", type=int, default=3, help="default run's protein leg: timed calls")
-    ap.add_argument("--protein-query-batch", type=int, default=512,
-                    help="protein top-k: queries per device batch (Aligner.top_k query_batch; the reference's 512)")
+    ap.add_argument("--protein-query-batch", type=int, default=None,
+                    help="protein top-k: queries per device batch (Aligner.top_k query_batch; default: the "
+                         "Aligner's, a 2 GiB score block, here all 1,000; the reference uses 512)")
     ap.add_argument("--no-protein", action="store_true", help="default run: skip the protein top-k leg")
     ap.add_argument("--no-serving", action="store_true", help="default run: skip the serving leg")
     ap.add_argument("--budget", type=int, default=64, help="serving: device memory budget (GiB)")

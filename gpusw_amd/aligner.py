@@ -160,13 +160,22 @@ class Aligner:
                                [self._refs.ids[j] for j in rj], self.scheme)
         return host
 
-    def top_k(self, queries=None, k: int = 5, *, query_batch: int = 512) -> AlignResult:
+    #: score-block size that sizes top_k's default query batch (device int32 scores)
+    TOPK_BLOCK_BYTES = 2 << 30
+
+    def top_k(self, queries=None, k: int = 5, *, query_batch: int | None = None) -> AlignResult:
         """Per-query best ``k`` references (score descending, ties to the lower index).
 
-        Only the top-k of each batch leaves the device.
+        Only the top-k of each batch leaves the device.  ``query_batch`` bounds the queries
+        scored per device batch (their score block is ``query_batch x n_refs`` int32); the
+        default (None) takes as many as keep that block within 2 GiB, a small share of an
+        MI355X's 288 GB -- the reference's fixed 512 (``src/gpusw/aligner.py:169``) splits a
+        1,000-query call in two.  Results do not depend on it.
         """
         self._need(queries, "top_k")
         nq, nr = self._queries.n, self._refs.n
+        if query_batch is None:
+            query_batch = max(1, self.TOPK_BLOCK_BYTES // (4 * max(nr, 1)))
         kk = min(k, nr)
         idx_out = np.empty((nq, kk), dtype=np.int32)
         sc_out = np.empty((nq, kk), dtype=np.int32)
@@ -210,7 +219,7 @@ class Aligner:
                     col["r_end"], col["ops_offset"], col["n_ops"], col["matches"], col["mismatches"],
                     col["ins"], col["del_"], col["gap_opens"])]
 
-    def top_k_alignments(self, queries=None, k: int = 5, *, query_batch: int = 512) -> list:
+    def top_k_alignments(self, queries=None, k: int = 5, *, query_batch: int | None = None) -> list:
         """``top_k`` followed by the traceback of every hit: ``[(query_id, [Alignment,
         ...]), ...]`` best first."""
         res = self.top_k(queries, k, query_batch=query_batch)

@@ -2,6 +2,7 @@
 // scoring calls and the serving loop).
 #pragma once
 
+#include "gsw_host.h"
 #include "gsw_internal.h"
 #include "gsw_sweep.h"
 
@@ -16,12 +17,6 @@ struct CrossItem {
     int32_t rbeg, rend;      // range in the sorted reference order
     int32_t shift;           // front padding columns (semiglobal, free query ends)
     int32_t nstr;            // stripes of G*R columns (stripe kernel; 1 otherwise)
-};
-
-// Task of the pairs kernel: queries qa and qb (qb < 0: single) against reference ref.
-struct PairTask {
-    int32_t qa, qb, ref, shift;
-    int64_t oa, ob;          // output indices (ob < 0: none)
 };
 
 // An int16 result that reached 32767 (local mode under GSW_DTYPE_INT16): rescored in int32.

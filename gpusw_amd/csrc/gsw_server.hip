@@ -29,55 +29,7 @@ namespace gsw {
 
 namespace {
 
-constexpr size_t ALIGN = 256;
-inline size_t round_up(size_t v) { return (v + ALIGN - 1) & ~(ALIGN - 1); }
 
-// First-fit allocator over [0, cap) offsets (host bookkeeping for a device arena).
-class Arena {
-  public:
-    void init(size_t cap) {
-        cap_ = cap;
-        used_ = 0;
-        free_.clear();
-        if (cap) free_[0] = cap;
-    }
-    bool alloc(size_t bytes, size_t *off) {
-        bytes = round_up(std::max<size_t>(bytes, 1));
-        for (auto it = free_.begin(); it != free_.end(); ++it) {
-            if (it->second < bytes) continue;
-            *off = it->first;
-            const size_t rest = it->second - bytes, start = it->first + bytes;
-            free_.erase(it);
-            if (rest) free_[start] = rest;
-            used_ += bytes;
-            return true;
-        }
-        return false;
-    }
-    void release(size_t off, size_t bytes) {
-        bytes = round_up(std::max<size_t>(bytes, 1));
-        used_ -= bytes;
-        auto it = free_.emplace(off, bytes).first;
-        auto nx = std::next(it);
-        if (nx != free_.end() && it->first + it->second == nx->first) {
-            it->second += nx->second;
-            free_.erase(nx);
-        }
-        if (it != free_.begin()) {
-            auto pv = std::prev(it);
-            if (pv->first + pv->second == it->first) {
-                pv->second += it->second;
-                free_.erase(it);
-            }
-        }
-    }
-    size_t cap() const { return cap_; }
-    size_t used() const { return used_; }
-
-  private:
-    std::map<size_t, size_t> free_;
-    size_t cap_ = 0, used_ = 0;
-};
 
 struct ServerDb {
     SchemeDev s;
@@ -256,7 +208,10 @@ int Server::admit_locked() {
         const SeqSet &rs = *dbs[r->db]->rs;
         r->codes_bytes = std::max<size_t>(r->codes.size(), 1);
         r->scores_bytes = 4 * ((size_t)r->nq * rs.n + 2 * (size_t)r->nq * std::max(r->kk, 1));
-        if (round_up(r->codes_bytes) > code_arena.cap() || round_up(r->scores_bytes) > score_arena.cap()) {
+        size_t ca = 0, sa = 0;
+        const Admission adm = admit_head(code_arena, score_arena, r->codes_bytes, r->scores_bytes, &ca, &sa);
+        if (adm == Admission::WAIT) break;
+        if (adm == Admission::REJECT) {
             r->status = GSW_ERR_BUSY;
             r->error = "request needs more device memory than the server budget";
             r->done = true;
@@ -265,12 +220,6 @@ int Server::admit_locked() {
             queue.pop_front();
             cv_done.notify_all();
             continue;
-        }
-        size_t ca = 0, sa = 0;
-        if (!code_arena.alloc(r->codes_bytes, &ca)) break;
-        if (!score_arena.alloc(r->scores_bytes, &sa)) {
-            code_arena.release(ca, r->codes_bytes);
-            break;
         }
         queue.pop_front();
         r->codes_at = ca;

@@ -1259,88 +1259,6 @@ static int host_threads() {
     return t;
 }
 
-// Host threads for planning, started on first use and kept: starting and joining 7
-// threads costs 0.2-0.4 ms, several times per call.  run(T, fn) runs fn(1..T-1) on pool
-// threads and fn(0) on the caller, and returns when all have returned.  One run at a time;
-// a caller that finds the pool busy starts threads of its own instead.  A pairs call runs
-// several jobs back to back and a sleeping thread takes tens of microseconds to wake, so
-// the workers (for their next job) and the caller (for the workers to finish) spin up to
-// 200 microseconds before they sleep: 70-90 us per short job become about 15.
-class HostPool {
-  public:
-    template <typename F> void run(int T, F &&fn) {
-        std::unique_lock<std::mutex> call(call_mu_, std::try_to_lock);
-        if (!call.owns_lock()) {
-            std::vector<std::thread> th;
-            for (int i = 1; i < T; ++i) th.emplace_back([&fn, i] { fn(i); });
-            fn(0);
-            for (auto &x : th) x.join();
-            return;
-        }
-        {
-            std::lock_guard<std::mutex> lk(mu_);
-            while ((int)threads_.size() < T - 1) {
-                const int w = (int)threads_.size() + 1;
-                threads_.emplace_back([this, w] { loop(w); });
-            }
-            job_ = [&fn](int i) { fn(i); };
-            njobs_ = T;
-            remaining_.store(T - 1, std::memory_order_relaxed);
-            gen_.fetch_add(1, std::memory_order_release);
-        }
-        cv_.notify_all();
-        fn(0);
-        spin([&] { return remaining_.load(std::memory_order_acquire) == 0; });
-        std::unique_lock<std::mutex> lk(mu_);
-        done_.wait(lk, [&] { return remaining_.load(std::memory_order_acquire) == 0; });
-        job_ = nullptr;
-    }
-    ~HostPool() {
-        {
-            std::lock_guard<std::mutex> lk(mu_);
-            quit_.store(true);
-        }
-        cv_.notify_all();
-        for (auto &t : threads_) t.join();
-    }
-
-  private:
-    // pred() or 200 microseconds, whichever comes first
-    template <typename P> static void spin(P &&pred) {
-        using clk = std::chrono::steady_clock;
-        const clk::time_point end = clk::now() + std::chrono::microseconds(200);
-        for (int i = 0; !pred(); ++i)
-            if ((i & 255) == 255 && clk::now() > end) return;
-    }
-    void loop(int w) {
-        uint64_t seen = 0;
-        for (;;) {
-            spin([&] { return gen_.load(std::memory_order_acquire) != seen || quit_.load(std::memory_order_relaxed); });
-            std::function<void(int)> f;
-            {
-                std::unique_lock<std::mutex> lk(mu_);
-                cv_.wait(lk, [&] { return quit_.load() || gen_.load() != seen; });
-                if (quit_.load()) return;
-                seen = gen_.load();
-                if (w >= njobs_) continue;
-                f = job_;
-            }
-            f(w);
-            // under the lock, so the caller cannot miss the notification
-            std::lock_guard<std::mutex> lk(mu_);
-            if (remaining_.fetch_sub(1, std::memory_order_acq_rel) == 1) done_.notify_all();
-        }
-    }
-    std::mutex call_mu_, mu_;
-    std::condition_variable cv_, done_;
-    std::vector<std::thread> threads_;
-    std::function<void(int)> job_;
-    int njobs_ = 0;
-    std::atomic<int> remaining_{0};
-    std::atomic<uint64_t> gen_{0};
-    std::atomic<bool> quit_{false};
-};
-
 // The library's one pool (a function-local static in the template below would give each
 // caller's lambda type a pool of its own).
 static HostPool &host_pool() {
@@ -1607,74 +1525,17 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
         return GSW_OK;
     };
 
-    // Plan one batch into its slot's pinned buffer: counting sort by reference length rank
-    // (longest first), then (stable) by last-column register when it is tracked; pair
-    // adjacent members of one reference; pad blocks of 64/G tasks at shift changes.  Each
-    // member's query, reference and shift are gathered once, reading the pair list in index
-    // order, into the sorted records, which the later passes read in sequence.
-    struct Member { int64_t k; int32_t q, r, shift; };
-    struct Scratch {
-        std::vector<Member> a, b;
-        std::vector<int32_t> cnt;
-        std::vector<int32_t> shift_of_len;   // front padding per query length (0..wmax)
-        int shift_r = -1;                    // the R it holds (0: last column not tracked)
-    };
+    // Plan one batch into its slot's pinned buffer (plan_pair_batch: by reference length
+    // rank, then front padding; adjacent members of one reference packed into one task).
+    using Scratch = PairPlanScratch;
     std::vector<double> batch_cells;
+    const PairPlanView view{h_qi, h_rj, nullptr, h_outidx, qs.len.data(), rs.len.data(), rs.rank.data(),
+                            rs.n_nonempty, wmax};
     auto plan = [&](const Batch &B, PairGraph &g, Scratch &S, double *cells) -> int32_t {
-        const Variant &v = *B.v;
-        const int gpw = 64 / v.G;
-        const size_t nm = (size_t)(B.b1 - B.b0);
-        if (S.shift_r != (track_last_col ? v.R : 0)) {   // a tile's members are at most wmax long
-            S.shift_r = track_last_col ? v.R : 0;
-            S.shift_of_len.assign((size_t)wmax + 1, 0);
-            if (track_last_col)
-                for (int lq = 0; lq <= wmax; ++lq) S.shift_of_len[lq] = (v.R - lq % v.R) % v.R;
-        }
-        S.cnt.assign((size_t)rs.n_nonempty + 1, 0);
-        for (int64_t i = B.b0; i < B.b1; ++i) S.cnt[rs.rank[h_rj[members[i]]] + 1]++;
-        for (int32_t r = 0; r < rs.n_nonempty; ++r) S.cnt[r + 1] += S.cnt[r];
-        S.a.resize(nm);
-        double c = 0;
-        for (int64_t i = B.b0; i < B.b1; ++i) {
-            const int64_t k = members[i];
-            const int32_t q = h_qi[k], r = h_rj[k];
-            const int lq = qs.len[q];
-            c += (double)lq * rs.len[r];
-            S.a[S.cnt[rs.rank[r]]++] = Member{k, q, r, S.shift_of_len[lq]};
-        }
-        *cells = c;
-        if (track_last_col) {
-            S.cnt.assign(v.R + 1, 0);
-            for (const Member &x : S.a) S.cnt[x.shift + 1]++;
-            int distinct = 0;
-            for (int r = 1; r <= v.R; ++r) distinct += S.cnt[r] > 0;
-            // a second (stable) pass by front padding, unless the whole batch shares one
-            if (distinct > 1) {
-                for (int r = 0; r < v.R; ++r) S.cnt[r + 1] += S.cnt[r];
-                S.b.resize(nm);
-                for (const Member &x : S.a) S.b[S.cnt[x.shift]++] = x;
-                S.a.swap(S.b);
-            }
-        }
-        PairTask *tasks = reinterpret_cast<PairTask *>(g.h_buf + PAIR_HDR);
-        int32_t nt = 0;
-        int cur_shift = -1;
-        for (size_t i = 0; i < nm;) {
-            const Member &x = S.a[i];
-            if (x.shift != cur_shift) {
-                while (nt % gpw) tasks[nt++] = PairTask{-1, -1, -1, cur_shift, -1, -1};
-                cur_shift = x.shift;
-            }
-            if (v.packed && i + 1 < nm && S.a[i + 1].r == x.r && S.a[i + 1].shift == x.shift) {
-                const Member &y = S.a[i + 1];
-                tasks[nt++] = PairTask{x.q, y.q, x.r, x.shift, oidx(x.k), oidx(y.k)};
-                i += 2;
-            } else {
-                tasks[nt++] = PairTask{x.q, -1, x.r, x.shift, oidx(x.k), -1};
-                i += 1;
-            }
-        }
-        while (nt % gpw) tasks[nt++] = PairTask{-1, -1, -1, cur_shift, -1, -1};
+        PairPlanView in = view;
+        in.members = members.data();
+        const int32_t nt = plan_pair_batch(in, B.b0, B.b1, B.v->G, B.v->R, B.v->packed, track_last_col, S,
+                                           reinterpret_cast<PairTask *>(g.h_buf + PAIR_HDR), cells);
         int32_t *hdr = reinterpret_cast<int32_t *>(g.h_buf);
         hdr[0] = nt;   // task count
         hdr[1] = 0;    // work counter (the graph's copy zeroes it on the device)
