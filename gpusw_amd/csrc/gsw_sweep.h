@@ -444,7 +444,10 @@ __device__ __forceinline__ SweepOut sweep_pairs_pipe(const uint8_t *__restrict__
             }
             hout = h;
             eout = e;
-            if (!LOCAL && track_endcol) endcol = O::mx(endcol, h);
+            // unconditional (read only when track_endcol): a branch here splits the step, and
+            // the scheduler then sinks the F updates below it as dependent add/max pairs, each
+            // needing a wait state between them (about 70 s_nop per step)
+            if (!LOCAL) endcol = O::mx(endcol, h);
         }
         din = hin;
         hin = O::from_bits(wave_shr1(O::bits(hout)));
@@ -470,6 +473,9 @@ __device__ __forceinline__ SweepOut sweep_pairs_pipe(const uint8_t *__restrict__
         return out;
     }
     int corner_a = RES_NONE, corner_b = RES_NONE, endrow_a = RES_NONE, endrow_b = RES_NONE;
+    // the column tests below would otherwise be hoisted above the loop, where their masks
+    // (two to four per column) spill scalar registers for the whole sweep
+    asm volatile("" : "+v"(lqa), "+v"(lqb), "+v"(cbase));
     if (lr > 0) {
 #pragma unroll
         for (int kk = 0; kk < R; ++kk) {
