@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
+#include <cstring>
 #include <deque>
 #include <map>
 #include <memory>
@@ -53,6 +54,8 @@ struct Request {
     int status = GSW_OK;
     std::string error;
     bool done = false;
+    bool finishing = false;                   // its top-k is queued: the completer thread finishes it
+    hipEvent_t done_ev = nullptr;             // after its top-k results reach the host
     bool claimed = false;                     // a gsw_server_wait is waiting for it
     double t_submit = 0, t_admit = 0, t_done = 0;
     int64_t cells = 0;
@@ -64,9 +67,10 @@ struct Slot {
                                    // launches overlap, each one's workgroups taking CUs as the
                                    // other's finish
     hipEvent_t ev = nullptr;
-    void *items = nullptr;
+    void *items = nullptr;         // SLOT_HDR bytes (the work counter first), then the items
     size_t items_cap = 0;
-    int32_t *counters = nullptr;   // [0] work counter
+    char *h_items = nullptr;       // pinned staging of the same bytes
+    size_t h_cap = 0;
     int db = -1;
     std::vector<std::pair<Request *, int>> reqs;
     std::vector<CrossItem> host_items;
@@ -75,12 +79,16 @@ struct Slot {
 };
 
 constexpr int NSLOTS = 2;
+// A launch's work counter travels in the header of its item copy: a memset would be a fill
+// kernel, which cannot start before the other slot's persistent launch frees a CU.
+constexpr size_t SLOT_HDR = 64;
 
 }  // namespace
 
 struct Server {
-    DeviceCtx ctx;                       // private stream, events and scratch
+    DeviceCtx ctx;                       // private stream (query-code uploads), events and scratch
     hipEvent_t codes_ready = nullptr;    // after the latest query-code upload on ctx.stream
+    hipStream_t finish = nullptr;        // finished requests' top-k and result copies
     uint8_t *code_base = nullptr;
     uint8_t *score_base = nullptr;
     Arena code_arena, score_arena;
@@ -94,6 +102,15 @@ struct Server {
     Slot slots[NSLOTS];
     std::deque<int> inflight;
     std::thread worker;
+    // The completer thread waits for queued top-k results, so the worker never waits for a
+    // kernel that needs CUs while the slots' persistent launches hold them all.
+    std::deque<Request *> finishing;
+    std::condition_variable cv_finish;
+    std::thread completer;
+    bool stop_completer = false;
+    // buffers replaced by larger ones: kept to the end (hipFree and hipHostFree wait for the
+    // whole device, which would stall the other slot's launch)
+    std::vector<void *> graveyard, host_graveyard;
     bool stop = false;
     int64_t next_id = 1;
     std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
@@ -106,6 +123,8 @@ struct Server {
         return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
     void run();
+    void complete_loop();
+    int grow(void **buf, size_t *cap, size_t need, bool pinned);
     int admit_locked();
     int plan_locked(Request *r);
     Request *next_issuable_locked();
@@ -113,6 +132,20 @@ struct Server {
     int retire(int slot);
     int finalize(Request *r);
 };
+
+// Grow a device (or pinned host) buffer to `need` bytes without freeing the old one.
+int Server::grow(void **buf, size_t *cap, size_t need, bool pinned) {
+    if (need <= *cap && *buf) return GSW_OK;
+    size_t want = std::max(need, *cap + *cap / 2);
+    want = (want + 255) & ~size_t(255);
+    void *p = nullptr;
+    if (pinned) GSW_HIP_TRY(hipHostMalloc(&p, want, hipHostMallocDefault));
+    else GSW_HIP_TRY(hipMalloc(&p, want));
+    if (*buf) (pinned ? host_graveyard : graveyard).push_back(*buf);
+    *buf = p;
+    *cap = want;
+    return GSW_OK;
+}
 
 // ------------------------------------------------------------------ planning
 int Server::plan_locked(Request *r) {
@@ -299,11 +332,15 @@ int Server::launch_locked(int si, Request *head) {
         return (bp[x.rend] - bp[x.rbeg]) * (int64_t)(x.lqa + std::max(x.lqb, 0)) * x.nstr >
                (bp[y.rend] - bp[y.rbeg]) * (int64_t)(y.lqa + std::max(y.lqb, 0)) * y.nstr;
     });
-    const size_t bytes = sizeof(CrossItem) * sl.host_items.size();
-    GSW_TRY(ensure_buffer(&sl.items, &sl.items_cap, bytes));
+    // header (zeroed work counter) and items in one copy from pinned memory: the slot's previous
+    // copy was read before its launch, which has retired
+    const size_t bytes = SLOT_HDR + sizeof(CrossItem) * sl.host_items.size();
+    GSW_TRY(grow(&sl.items, &sl.items_cap, bytes, false));
+    GSW_TRY(grow(reinterpret_cast<void **>(&sl.h_items), &sl.h_cap, bytes, true));
+    std::memset(sl.h_items, 0, SLOT_HDR);
+    std::memcpy(sl.h_items + SLOT_HDR, sl.host_items.data(), bytes - SLOT_HDR);
     GSW_HIP_TRY(hipStreamWaitEvent(sl.st, codes_ready, 0));   // this launch's query codes are uploaded
-    GSW_HIP_TRY(hipMemcpyAsync(sl.items, sl.host_items.data(), bytes, hipMemcpyHostToDevice, sl.st));
-    GSW_HIP_TRY(hipMemsetAsync(sl.counters, 0, sizeof(int32_t), sl.st));
+    GSW_HIP_TRY(hipMemcpyAsync(sl.items, sl.h_items, bytes, hipMemcpyHostToDevice, sl.st));
     const size_t lds = cross_lds_bytes(v, d.s.ns);
     int grid = 0;
     GSW_TRY(occupancy_grid(&ctx, striped ? v.stripe_fn : v.cross_fn, CROSS_WAVES * 64, lds, &grid));
@@ -318,7 +355,7 @@ int Server::launch_locked(int si, Request *head) {
     if (striped) {
         for (const CrossItem &it : sl.host_items) rows = std::max(rows, d.rs->bpos[it.rend] - d.rs->bpos[it.rbeg]);
         // launches of both slots may run at once: each slot has its own half of the scratch
-        GSW_TRY(ensure_buffer(&sl.bnd, &sl.bnd_bytes, 2 * sizeof(uint32_t) * (size_t)rows * grid));
+        GSW_TRY(grow(&sl.bnd, &sl.bnd_bytes, 2 * sizeof(uint32_t) * (size_t)rows * grid, false));
     }
     KArgs a{};
     a.qcodes = code_base;
@@ -330,9 +367,9 @@ int Server::launch_locked(int si, Request *head) {
     a.k = SweepConst{d.s.mode, d.s.go, d.s.ge, d.s.free_q, d.s.free_r, d.s.ns};
     a.out = reinterpret_cast<int32_t *>(score_base);
     a.sat_check = 0;   // int16 only where no value can saturate
-    a.items = sl.items;
+    a.items = static_cast<char *>(sl.items) + SLOT_HDR;
     a.nitems = (int32_t)sl.host_items.size();
-    a.counter = sl.counters;
+    a.counter = static_cast<int32_t *>(sl.items);
     if (striped) {
         a.bnd = static_cast<uint32_t *>(sl.bnd);
         a.bnd_stride = rows;
@@ -348,6 +385,8 @@ int Server::launch_locked(int si, Request *head) {
     return GSW_OK;
 }
 
+// A request whose launches have all retired: its top-k and result copies are queued on the
+// finish stream, and the completer thread marks it done once they land.
 int Server::finalize(Request *r) {
     const SeqSet &rs = *dbs[r->db]->rs;
     const int32_t nr = rs.n;
@@ -357,25 +396,51 @@ int Server::finalize(Request *r) {
         int32_t *scores = reinterpret_cast<int32_t *>(score_base + r->scores_at);
         int32_t *d_idx = scores + (size_t)r->nq * nr;
         int32_t *d_sc = d_idx + (size_t)r->nq * r->kk;
-        topk_launch(scores, r->nq, nr, r->kk, d_idx, d_sc, ctx.stream);
+        topk_launch(scores, r->nq, nr, r->kk, d_idx, d_sc, finish);
         GSW_HIP_TRY(hipGetLastError());
-        GSW_HIP_TRY(hipMemcpyAsync(r->idx.data(), d_idx, 4 * r->idx.size(), hipMemcpyDeviceToHost, ctx.stream));
-        GSW_HIP_TRY(hipMemcpyAsync(r->sc.data(), d_sc, 4 * r->sc.size(), hipMemcpyDeviceToHost, ctx.stream));
-        GSW_HIP_TRY(hipStreamSynchronize(ctx.stream));
+        GSW_HIP_TRY(hipMemcpyAsync(r->idx.data(), d_idx, 4 * r->idx.size(), hipMemcpyDeviceToHost, finish));
+        GSW_HIP_TRY(hipMemcpyAsync(r->sc.data(), d_sc, 4 * r->sc.size(), hipMemcpyDeviceToHost, finish));
+        GSW_HIP_TRY(hipEventCreateWithFlags(&r->done_ev, hipEventDisableTiming));
+        GSW_HIP_TRY(hipEventRecord(r->done_ev, finish));
     }
-    code_arena.release(r->codes_at, r->codes_bytes);
-    score_arena.release(r->scores_at, r->scores_bytes);
     r->codes.clear();
     r->codes.shrink_to_fit();
     r->items.clear();
     r->items.shrink_to_fit();
-    r->done = true;
-    r->t_done = now_ms();
-    completed++;
-    cells_done += r->cells;
+    r->finishing = true;
     active.erase(std::find(active.begin(), active.end(), r));
-    cv_done.notify_all();
+    finishing.push_back(r);
+    cv_finish.notify_one();
     return GSW_OK;
+}
+
+void Server::complete_loop() {
+    (void)hipSetDevice(ctx.device);
+    std::unique_lock<std::mutex> lk(mu);
+    for (;;) {
+        cv_finish.wait(lk, [&] { return !finishing.empty() || stop_completer; });
+        if (finishing.empty()) return;   // stopped, nothing left
+        Request *r = finishing.front();
+        finishing.pop_front();
+        const hipEvent_t ev = r->done_ev;
+        lk.unlock();
+        const hipError_t e = ev ? hipEventSynchronize(ev) : hipSuccess;
+        lk.lock();
+        if (ev) (void)hipEventDestroy(ev);
+        r->done_ev = nullptr;
+        if (e != hipSuccess && r->status == GSW_OK) {
+            r->status = GSW_ERR_HIP;
+            r->error = hipGetErrorString(e);
+        }
+        code_arena.release(r->codes_at, r->codes_bytes);
+        score_arena.release(r->scores_at, r->scores_bytes);
+        r->done = true;
+        r->t_done = now_ms();
+        completed++;
+        cells_done += r->cells;
+        cv_done.notify_all();
+        cv_work.notify_all();   // memory came back: the worker may admit more
+    }
 }
 
 int Server::retire(int si) {
@@ -419,7 +484,7 @@ void Server::run() {
             fatal = gsw_last_error();
             for (auto &kv : reqs) {
                 Request *r = kv.second.get();
-                if (!r->done) {
+                if (!r->done && !r->finishing) {   // the completer finishes the others
                     r->status = rc;
                     r->error = fatal;
                     r->done = true;
@@ -453,6 +518,7 @@ int gsw_server_create(size_t mem_budget_bytes, double launch_cells, gsw_server *
     srv->ctx.num_cus = shared->num_cus;
     srv->ctx.max_smem = shared->max_smem;
     GSW_HIP_TRY(hipStreamCreateWithFlags(&srv->ctx.stream, hipStreamNonBlocking));
+    GSW_HIP_TRY(hipStreamCreateWithFlags(&srv->finish, hipStreamNonBlocking));
     GSW_HIP_TRY(hipEventCreate(&srv->ctx.ev0));
     GSW_HIP_TRY(hipEventCreate(&srv->ctx.ev1));
     GSW_HIP_TRY(hipEventCreateWithFlags(&srv->codes_ready, hipEventDisableTiming));
@@ -469,10 +535,10 @@ int gsw_server_create(size_t mem_budget_bytes, double launch_cells, gsw_server *
     for (int i = 0; i < NSLOTS; ++i) {
         GSW_HIP_TRY(hipStreamCreateWithFlags(&srv->slots[i].st, hipStreamNonBlocking));
         GSW_HIP_TRY(hipEventCreateWithFlags(&srv->slots[i].ev, hipEventDisableTiming));
-        GSW_HIP_TRY(hipMalloc(&srv->slots[i].counters, sizeof(int32_t)));
     }
     Server *p = srv.get();
     p->worker = std::thread([p] { p->run(); });
+    p->completer = std::thread([p] { p->complete_loop(); });
     *out = reinterpret_cast<gsw_server>(srv.release());
     return GSW_OK;
 }
@@ -595,6 +661,13 @@ int gsw_server_destroy(gsw_server server) {
         srv->cv_work.notify_all();
     }
     if (srv->worker.joinable()) srv->worker.join();
+    {
+        std::lock_guard<std::mutex> lk(srv->mu);
+        srv->stop_completer = true;
+        srv->cv_finish.notify_all();
+    }
+    if (srv->completer.joinable()) srv->completer.join();   // it drains the finishing queue first
+    (void)hipStreamSynchronize(srv->finish);
     (void)hipStreamSynchronize(srv->ctx.stream);
     for (int i = 0; i < NSLOTS; ++i) {
         if (srv->slots[i].st) {
@@ -602,10 +675,12 @@ int gsw_server_destroy(gsw_server server) {
             (void)hipStreamDestroy(srv->slots[i].st);
         }
         (void)hipEventDestroy(srv->slots[i].ev);
-        (void)hipFree(srv->slots[i].counters);
         if (srv->slots[i].items) (void)hipFree(srv->slots[i].items);
+        if (srv->slots[i].h_items) (void)hipHostFree(srv->slots[i].h_items);
         if (srv->slots[i].bnd) (void)hipFree(srv->slots[i].bnd);
     }
+    for (void *b : srv->graveyard) (void)hipFree(b);
+    for (void *b : srv->host_graveyard) (void)hipHostFree(b);
     (void)hipFree(srv->code_base);
     (void)hipFree(srv->score_base);
     (void)hipFree(srv->ctx.counters);
@@ -616,6 +691,7 @@ int gsw_server_destroy(gsw_server server) {
     (void)hipEventDestroy(srv->ctx.ev1);
     if (srv->codes_ready) (void)hipEventDestroy(srv->codes_ready);
     (void)hipStreamDestroy(srv->ctx.stream);
+    (void)hipStreamDestroy(srv->finish);
     delete srv;
     return GSW_OK;
 }
