@@ -7,11 +7,12 @@ scheme, int16 with int32 fallback.  One step = the full (2000, 20000) score matr
 computed into HBM plus a device top_k(5) of every row.  Inputs are resident in HBM
 before the timed region.  Also reported:
 
-* ``roofline``: packed-int16 VALU roofline of the cross kernel -- algorithmic int16 ops
+* ``roofline``: packed-int16 VALU roofline of the cross kernels -- algorithmic int16 ops
   (cells x 9 for local mode: 4 add/sub + 5 max, the clamp max(H, 0) folded into saturating
-  E/F updates; global/semiglobal 8) over the kernel's device time measured with HIP events
-  on the library stream, against the gfx950 peak of 78.6 T int16-ops/s (the 10-op count
-  that prices max(H, 0) separately is kept as ``survey_frac``);
+  E/F updates; global/semiglobal 8) over the device time of the call's batched launches,
+  which overlap on two streams (HIP events from the first launch's start to the last one's
+  end), against the gfx950 peak of 78.6 T int16-ops/s (the 10-op count that prices
+  max(H, 0) separately is kept as ``survey_frac``); the largest launch is reported beside it;
 * ``plain``: the one-thread-per-pair comparator on the same GPU (a query subsample);
 * ``cpu_baseline``: the oracle's C restatement on the host cores over a bounded sample
   of the same pairs (those pairs are also checked bit-exact against the GPU scores), and
@@ -212,24 +213,26 @@ def _call_traffic(workload: str):
     return (total, src) if src else (None, None)
 
 
-def _call_roofline(history, cells: int, kernel_s: float, steps: int) -> dict:
-    """Roofline of a protein top-k call as a whole.  Its tile launches run on two streams
+def _call_roofline(history, cells: int, kernel_s: float, steps: int, workload: str = "protein",
+                   label: str = "all batched launches of a top_k call (cross_kernel + stripe_kernel tiles, two streams)"
+                   ) -> dict:
+    """Roofline of a cross-product call as a whole.  Its tile launches run on two streams
     and overlap (each launch's workgroups take CUs as the other stream's finish), so no
     single launch owns the GPU for its interval: the dominant unit is the call's set of
     batched launches, timed from the first launch's start to the last one's end (HIP events
-    on the library stream, joined with the second stream)."""
+    on the library stream, joined with the second stream).  cells and kernel_s are per step."""
     ops, survey = OPS_PER_CELL["local"], SURVEY_OPS_PER_CELL["local"]
     achieved = cells * ops / kernel_s / 1e12
     calls = max(1, len(history) // max(steps, 1))
-    traffic, src = _call_traffic("protein")
+    traffic, src = _call_traffic(workload)
     out = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_INT16_TOPS, "unit": "TOP/s (int16)",
            "frac": round(achieved / PEAK_INT16_TOPS, 4),
            "traffic": None if traffic is None else round(traffic * calls, 1),
-           "kernel": "all batched launches of a top_k call (cross_kernel + stripe_kernel tiles, two streams)",
+           "kernel": label,
            "ops_per_cell": ops, "survey_ops_per_cell": survey,
            "survey_frac": round(achieved * survey / ops / PEAK_INT16_TOPS, 4),
            "kernels_ms_per_step": round(kernel_s * 1e3, 2), "calls_per_step": calls,
-           "largest_launch": _roofline(history, "local", "protein")}
+           "largest_launch": _roofline(history, "local", workload)}
     if traffic is not None:
         out["traffic_unit"] = "bytes per step (sum of the per-dispatch rocprofv3 rows of every batched kernel)"
         out["traffic_source"] = src
@@ -282,8 +285,9 @@ def run_dna_cross(args):
     value = cells / (wall / args.steps) / 1e9
     stats = b.last_stats
     kavg = float(np.mean(kernel_s))
-    roof = _roofline(history, "local", "dna16s")
-    roof.update({"kernels_ms_per_step": round(kavg * 1e3, 3), "launches_per_step": int(stats.launches)})
+    roof = _call_roofline(history, cells, kavg, args.steps, "dna16s",
+                          "all batched launches of a cross call (cross_kernel tiles by query length, two streams)")
+    roof["launches_per_step"] = int(stats.launches)
 
     # ---- correctness of the timed result (outside the timed region)
     host = scores.to_host()

@@ -39,8 +39,9 @@ __device__ __forceinline__ void emit(const KArgs &a, int64_t o, int q, int r, in
 // the pair's packed profile in LDS once, then its waves take blocks of 64/G references
 // from an LDS counter; each group of G lanes sweeps one reference.
 // Occupancy target: 4 waves per SIMD (<= 128 VGPRs) for the narrow local tiles (G <= 8,
-// R <= 32), whose loop fits in 128 registers; the other tiles are left free.
-template <bool LOCAL, int G, int R> struct WavesPerEU { static constexpr int value = (LOCAL && G <= 8 && R <= 32) ? 4 : 1; };
+// R <= 34), whose loop fits in 128 registers (R 33 and 34 spill two registers outside the
+// loop); the other tiles are left free.
+template <bool LOCAL, int G, int R> struct WavesPerEU { static constexpr int value = (LOCAL && G <= 8 && R <= 34) ? 4 : 1; };
 
 template <typename V, int G, int R, bool LOCAL, int WAVES>
 __global__ void __launch_bounds__(WAVES * 64)
@@ -388,7 +389,9 @@ std::vector<Variant> &variant_table() {
     static std::vector<Variant> table = [] {
         std::vector<Variant> v = {
             GSW_VARIANT4(8, 8),   GSW_VARIANT4(8, 12),  GSW_VARIANT4(8, 16),  GSW_VARIANT4(8, 19),  GSW_SVARIANT4(8, 20),
-            GSW_SVARIANT4(8, 24), GSW_SVARIANT4(8, 32), GSW_VARIANT4(8, 34),  GSW_VARIANT4(8, 36),
+            GSW_SVARIANT4(8, 24), GSW_VARIANT4(8, 30),  GSW_VARIANT4(8, 31),  GSW_SVARIANT4(8, 32), GSW_VARIANT4(8, 33),
+            GSW_VARIANT4(8, 34),
+            GSW_VARIANT4(8, 36),
             GSW_VARIANT4(8, 40),  GSW_VARIANT4(8, 44),  GSW_VARIANT4(8, 48),  GSW_VARIANT4(16, 24),
             GSW_VARIANT4(16, 28), GSW_VARIANT4(16, 32), GSW_VARIANT4(16, 36), GSW_VARIANT4(16, 40),
             GSW_VARIANT4(16, 44), GSW_VARIANT4(16, 48), GSW_VARIANT4(32, 32), GSW_VARIANT4(32, 48),
@@ -427,11 +430,13 @@ static double tile_cost(int G, int R, int lq, double lr) {
 
 size_t cross_lds_bytes(const Variant &v, int ns);
 
-// Occupancy of a cross or stripe tile relative to the 4 waves per SIMD of R <= 32: R 34-44
-// run 3 waves, R 48 two, and a profile above 80 KiB leaves one workgroup per CU.  (Measured
-// on BLOSUM62 top-k, 1,000 x 100,000: padded-column rates 5-8 % below the R <= 32 tiles.)
+// Occupancy of a cross or stripe tile relative to the 4 waves per SIMD of R <= 32 (local
+// G = 8: R <= 34): wider tiles run 3 waves up to R 44, R 48 two, and a profile above 80 KiB
+// leaves one workgroup per CU.  (Measured on BLOSUM62 top-k, 1,000 x 100,000: padded-column
+// rates 5-8 % below the 4-wave tiles.)
 static double tile_occupancy_cost(const Variant &v, int ns) {
-    double f = v.R > 44 ? 1.08 : v.R > 32 ? 1.05 : 1.0;
+    const int r4 = v.local && v.G <= 8 ? 34 : 32;
+    double f = v.R > 44 ? 1.08 : v.R > r4 ? 1.05 : 1.0;
     if (cross_lds_bytes(v, ns) > 80 * 1024) f *= 1.05;
     return f;
 }
