@@ -140,9 +140,12 @@ def _tile_name(variant: int) -> str | None:
 
     if variant < 0:
         return None
-    g, r = _hip.list_variants()[variant // 4]
-    kind = ("int16x2" if variant % 4 < 2 else "int32") + ("/local" if variant % 2 == 0 else "/nonlocal")
-    return f"G={g} lanes x R={r} columns ({kind})"
+    g, r, kind, local = _hip.variant_info(variant)
+    return f"G={g} lanes x R={r} columns ({kind}/{'local' if local else 'nonlocal'})"
+
+
+# element type of a variant kind as rocprofv3 prints it in a kernel name
+_ELEM_NAME = {"int16x2": "short __vector(2)", "half2": "_Float16 __vector(2)", "int32": "int"}
 
 
 def _kernel_name(variant: int, striped: int) -> str | None:
@@ -151,11 +154,9 @@ def _kernel_name(variant: int, striped: int) -> str | None:
 
     if variant < 0:
         return None
-    g, r = _hip.list_variants()[variant // 4]
-    elem = "short __vector(2)" if variant % 4 < 2 else "int"
-    local = "true" if variant % 2 == 0 else "false"
+    g, r, elem, local = _hip.variant_info(variant)
     kind, waves = {0: ("cross", 8), 1: ("stripe", 8), 2: ("pairs", 8), 3: ("pairs_tab", 4)}[int(striped)]
-    return f"void gsw::{kind}_kernel<{elem}, {g}, {r}, {local}, {waves}>(gsw::KArgs)"
+    return f"void gsw::{kind}_kernel<{_ELEM_NAME[elem]}, {g}, {r}, {str(local).lower()}, {waves}>(gsw::KArgs)"
 
 
 def _measured_traffic(workload: str, kernel: str | None):

@@ -113,6 +113,7 @@ _SIGNATURES = {
     "gsw_set_tile_table": [_vp, _i32],
     "gsw_force_variant": [_i32, _i32],
     "gsw_list_variants": [_vp, _i32, ctypes.POINTER(_i32)],
+    "gsw_variant_info": [_i32, _vp],
     "gsw_list_stripe_variants": [_vp, _i32, ctypes.POINTER(_i32)],
     "gsw_fasta_scan": [_vp, _i64, ctypes.POINTER(_i64), ctypes.POINTER(_i64)],
     "gsw_traceback_pairs": [ctypes.POINTER(GswScheme), _vp, _vp, _vp, _vp, _i64, _vp, _vp, _i64,
@@ -297,6 +298,16 @@ def list_variants() -> list[tuple[int, int]]:
     n = _i32(0)
     call("gsw_list_variants", ptr(buf), 256, ctypes.byref(n))
     return [(int(buf[2 * i]), int(buf[2 * i + 1])) for i in range(min(n.value, 256))]
+
+
+KIND_NAMES = {0: "int32", 1: "int16x2", 2: "half2"}
+
+
+def variant_info(variant: int) -> tuple[int, int, str, bool]:
+    """(G, R, element kind, local mode) of a variant id as gsw_stats reports it."""
+    info = np.zeros(4, dtype=np.int32)
+    call("gsw_variant_info", int(variant), ptr(info))
+    return int(info[0]), int(info[1]), KIND_NAMES[int(info[2])], bool(info[3])
 
 
 def list_stripe_variants() -> list[tuple[int, int]]:

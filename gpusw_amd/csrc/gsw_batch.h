@@ -59,25 +59,30 @@ constexpr int PAIRS_TAB_WAVES = 4;   // 4-wave workgroups: three fit a CU at 3 w
 
 typedef void (*LaunchFn)(const KArgs &, int grid, size_t lds, hipStream_t);
 
+const char *kind_name(int kind);   // "int32", "int16x2", "half2"
+
 struct Variant {
     int id, G, R;
-    bool packed, local;
+    bool packed, local;                             // packed: two alignments per lane
     const void *cross_fn, *pairs_fn, *stripe_fn;   // stripe_fn null: no stripe kernel
     LaunchFn cross, pairs, stripe;
     const void *pairs_tab_fn;                       // pairs with the substitution table in LDS
+    int kind;                                       // KIND_*
 };
+// The half2 rows of the variant table (gsw_sweep_f16.hip).
+std::vector<Variant> half_variants();
 // LDS of the pairs table kernel: ns rows of (ns+2)^2 packed words (int32 tiles: ns+2),
 // times pairs_tab_copies interleaved copies (32 when that stays within 48 KiB, else 1).
 int pairs_tab_copies(const Variant &v, int ns);
 size_t pairs_tab_lds_bytes(const Variant &v, int ns);
 
 std::vector<Variant> &variant_table();
-const Variant *pick_variant(int lq, double mean_lr, bool packed, bool local, int ns, size_t lds_limit,
+const Variant *pick_variant(int lq, double mean_lr, int kind, bool local, int ns, size_t lds_limit,
                             bool for_pairs);
 int max_width();
 // Tile of an explicit pair with query length lq; *tab: run it with the table kernel, else
 // with the profile kernel (profiles of prof_rows rows; prof_rows <= 0: table kernel only).
-const Variant *pick_pairs_tile(int lq, double mean_lr, bool packed, bool local, int ns, int prof_rows,
+const Variant *pick_pairs_tile(int lq, double mean_lr, int kind, bool local, int ns, int prof_rows,
                                size_t lds_limit, bool *tab);
 size_t cross_lds_bytes(const Variant &v, int ns);
 size_t pairs_wave_bytes(const Variant &v, int ns);               // one wave's private profiles
@@ -107,7 +112,7 @@ int plan_cross_units(const SchemeDev &s, int dtype_policy, const std::vector<Cro
                      std::vector<CrossQuery> *plain);
 // Tile for a cross query of length lq: the cheapest compiled tile covering it, or a tile
 // with a stripe kernel run as *nstr > 1 stripes when that is cheaper or nothing else fits.
-const Variant *pick_cross_tile(int lq, double mean_lr, bool packed, bool local, int ns, size_t lds_limit,
+const Variant *pick_cross_tile(int lq, double mean_lr, int kind, bool local, int ns, size_t lds_limit,
                                bool track_last_col, int *nstr);
 // One item per (unit, run of whole blocks of gpw sorted references holding about
 // target_rows rows); the appended items are ordered longest first.

@@ -68,6 +68,10 @@ int scheme_prepare(const gsw_scheme *s, DeviceCtx *ctx, SchemeDev *out);
 // score(q, r) under s equals score(r, q) under the transposed scheme, in every mode.
 int scheme_transpose(DeviceCtx *ctx, const SchemeDev &s, SchemeDev *out);
 
+// Element type of a tile: one int32 alignment per lane, or two per lane as int16 or as
+// binary16 (exact integers in [-2048, 2048], three-input max; see gsw_sweep.h).
+enum : int { KIND_I32 = 0, KIND_I16 = 1, KIND_F16 = 2 };
+
 // Static bound check used to choose int16 per pair (sound for every DP cell).
 bool int16_entries(const SchemeDev &s);   // table and gap scores fit int16
 bool fits_int16(const SchemeDev &s, int lq, int lr);
@@ -76,6 +80,23 @@ bool fits_int32(const SchemeDev &s, int lq, int lr);
 // (int16 only where no value can saturate); GSW_DTYPE_INT16 runs every local pair in int16
 // and relies on the kernels' saturation list; GSW_DTYPE_INT32 never uses int16.
 bool use_int16(const SchemeDev &s, int dtype_policy, int lq, int lr);
+
+// Binary16 tiles (exact integers in [-2048, 2048]): the table and gap scores fit, and for a
+// pair every DP value does (global/semiglobal; local pairs also run in half2 outside this
+// bound, their results of 2048 or more listed and rescored).  GSW_HALF=0 in the environment
+// turns the half2 tiles off (read per call).
+bool half_entries(const SchemeDev &s);
+bool fits_half(const SchemeDev &s, int lq, int lr);
+bool half_enabled();
+// Internal dtype policy: packed int16 where GSW_DTYPE_INT16 would use it, never half2 (the
+// rerun of local work whose half2 results overflowed the saturation list).
+constexpr int GSW_DTYPE_PK16 = 16;
+// Internal dtype policy: each pair on the narrowest type its envelope proves exact, so no
+// result is ever listed for rescoring (the serving loop, which has no rescoring pass).
+constexpr int GSW_DTYPE_SAFE = 17;
+// Tile element kind (KIND_I32 / KIND_I16 / KIND_F16, gsw_batch.h) of a pair under a policy,
+// from its int16 and half2 envelope checks; half: half2 tiles allowed for this call.
+int pair_kind(const SchemeDev &s, int dtype_policy, bool ok16, bool okh, bool half);
 
 // Closed-form score when one side is empty (src/gpusw/kernel.py:63-80).
 int32_t empty_edge_score(const SchemeDev &s, int lq, int lr);

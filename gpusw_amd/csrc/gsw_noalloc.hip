@@ -125,7 +125,8 @@ int run_pass(NoAllocDevice &dev, const SchemeDev &s, int dtype_policy, const Seq
                         (long long)i, lq, wmax);
         const bool use16 = dtype_policy == GSW_DTYPE_INT32 ? false : local16 ? true : fits_int16(s, lq, lr);
         bool tab = false;
-        const Variant *v = pick_pairs_tile(lq, mean_lr, use16, local, s.ns, 0, (size_t)dev.info.max_smem, &tab);
+        const Variant *v = pick_pairs_tile(lq, mean_lr, use16 ? KIND_I16 : KIND_I32, local, s.ns, 0,
+                                           (size_t)dev.info.max_smem, &tab);
         if (!v || !tab) return fail(GSW_ERR_LIMIT, "pair %lld has no table-kernel tile", (long long)i);
         members[v->id].push_back(i);
     }

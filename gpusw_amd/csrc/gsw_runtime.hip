@@ -2,6 +2,7 @@
 // preparation and indexed sequence sets.
 #include <algorithm>
 #include <cstdarg>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -188,6 +189,39 @@ bool use_int16(const SchemeDev &s, int dtype_policy, int lq, int lr) {
     // (32767) are listed by the kernel and rescored in int32
     if (dtype_policy == GSW_DTYPE_INT16 && s.mode == GSW_MODE_LOCAL) return int16_entries(s);
     return fits_int16(s, lq, lr);
+}
+
+bool half_entries(const SchemeDev &s) {
+    return s.sub_min >= -2048 && s.sub_max <= 2048 && s.go >= -2048 && s.ge >= -2048;
+}
+
+// As fits_int16 with the exact integer range of binary16, [-2048, 2048].
+bool fits_half(const SchemeDev &s, int lq, int lr) {
+    if (!half_entries(s)) return false;
+    int64_t lo, hi;
+    dp_bounds(s, lq, lr, &lo, &hi);
+    if (s.mode == GSW_MODE_LOCAL) return hi + std::max(s.sub_max, 0) + 16 <= 2047;
+    int64_t margin = -(int64_t)s.go - (int64_t)s.ge - (int64_t)std::min(s.sub_min, 0) + 16;
+    return lo - margin >= -2048 && hi + std::max(s.sub_max, 0) + 16 <= 2047;
+}
+
+bool half_enabled() {
+    const char *e = std::getenv("GSW_HALF");
+    return !(e && e[0] == '0');
+}
+
+int pair_kind(const SchemeDev &s, int dtype_policy, bool ok16, bool okh, bool half) {
+    if (dtype_policy == GSW_DTYPE_INT32) return KIND_I32;
+    const bool pk16 = dtype_policy == GSW_DTYPE_PK16;
+    if (s.mode == GSW_MODE_LOCAL && dtype_policy != GSW_DTYPE_SAFE) {
+        // int16 allowed: by the envelope, or everywhere when the caller vouched for it (its
+        // saturated results are listed); half2 wherever int16 is, results >= 2048 listed
+        const bool any16 = (dtype_policy == GSW_DTYPE_INT16 || pk16) ? int16_entries(s) : ok16;
+        if (!pk16 && half && half_entries(s) && (any16 || okh)) return KIND_F16;
+        return any16 ? KIND_I16 : KIND_I32;
+    }
+    if (!pk16 && half && okh) return KIND_F16;
+    return ok16 ? KIND_I16 : KIND_I32;
 }
 
 bool fits_int32(const SchemeDev &s, int lq, int lr) {

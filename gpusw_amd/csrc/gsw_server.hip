@@ -174,7 +174,7 @@ int Server::plan_locked(Request *r) {
     }
     std::vector<CrossUnit> units;
     std::vector<CrossQuery> plain;
-    GSW_TRY(plan_cross_units(db.s, GSW_DTYPE_AUTO, cq, rs.max_len, db.mean_lr, (size_t)ctx.max_smem, &units,
+    GSW_TRY(plan_cross_units(db.s, GSW_DTYPE_SAFE, cq, rs.max_len, db.mean_lr, (size_t)ctx.max_smem, &units,
                              &plain));
     // chunk references into small items: a launch gathers items of one tile from the
     // admitted requests, and small requests or rare tiles must still give it more items than
@@ -366,7 +366,7 @@ int Server::launch_locked(int si, Request *head) {
     a.ns = d.s.ns;
     a.k = SweepConst{d.s.mode, d.s.go, d.s.ge, d.s.free_q, d.s.free_r, d.s.ns};
     a.out = reinterpret_cast<int32_t *>(score_base);
-    a.sat_check = 0;   // int16 only where no value can saturate
+    a.sat_check = 0;   // int16 and half2 only where the envelope proves every value exact
     a.items = static_cast<char *>(sl.items) + SLOT_HDR;
     a.nitems = (int32_t)sl.host_items.size();
     a.counter = static_cast<int32_t *>(sl.items);

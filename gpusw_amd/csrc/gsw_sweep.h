@@ -13,12 +13,26 @@
 //   short2 (packed): two independent alignments per lane, query A in the low half and
 //                    query B in the high half, both against the group's reference; the
 //                    recurrence runs on v_pk_{add,sub,max}_{i16,u16} with saturation.
+//   half2 (packed):  the same two alignments carried as binary16 values, which hold every
+//                    integer of [-2048, 2048] exactly; the recurrence runs on v_pk_add_f16
+//                    and gfx950's three-input v_pk_maximum3_f16, so H = max(diag+s, E, F)
+//                    and each clamped E/F update take one instruction instead of two.
 //   int:             one alignment per lane in 32-bit.
 //
-// Local mode uses the clamped form: E, F and H-gap are kept >= 0 with unsigned
-// saturating subtraction.  Since local H = max(diag+s, E, F, 0), clamping E and F at 0
-// leaves every H unchanged and removes the separate max(H, 0).  In int16, a result equal
-// to 32767 may have saturated and is rescored in int32 by the host.
+// Local mode uses the clamped form: E, F and H-gap are kept >= 0 (int16: unsigned
+// saturating subtraction; half2: a third operand 0 of the three-input max).  Since local
+// H = max(diag+s, E, F, 0), clamping E and F at 0 leaves every H unchanged and removes the
+// separate max(H, 0).  In int16, a result equal to 32767 may have saturated; in half2, a
+// result of 2048 or more may have been rounded.  Either is rescored by the host.
+//
+// Why a half2 result below 2048 is exact (local mode): every E, F and H of a real column
+// is an integer in [0, best], and the only operation whose exact value can exceed best is
+// diag + s.  Sums of integers are exact up to 2048 in magnitude, and rounding is monotone
+// with 2048 representable, so the first inexact sum is >= 2048 after rounding, is an
+// operand of that cell's H = max3(...), and makes the lane's running maximum >= 2048.
+// Columns past a query's end carry the sentinel profile value (-30000): their sums round,
+// but they lie right of every real column and never feed one.  Global and semiglobal
+// pairs run in half2 only when the envelope bounds every value inside [-2048, 2048].
 //
 // Query profile.  prof(c, col) = sub[c][query[col]] (packed A|B for short2), stored in
 // LDS as 16-byte chunks; lane t's four values of chunk kc of row c sit at
@@ -39,6 +53,7 @@ namespace gsw {
 
 typedef short s2 __attribute__((ext_vector_type(2)));
 typedef unsigned short u2 __attribute__((ext_vector_type(2)));
+typedef _Float16 h2 __attribute__((ext_vector_type(2)));
 
 // Profile layout constants (bytes) of a (G, R) tile.
 template <int G, int R> struct ProfLayout {
@@ -54,14 +69,16 @@ template <typename V> struct VOps;
 
 template <> struct VOps<s2> {
     static constexpr int NEG = -32768;
-    static constexpr int TOP = 32767;
+    static constexpr int TOP = 32767;   // a result >= TOP may have saturated
     static constexpr bool PACKED = true;
+    static constexpr bool MAX3 = false;
     __device__ static __forceinline__ s2 add(s2 a, s2 b) { return __builtin_elementwise_add_sat(a, b); }
     __device__ static __forceinline__ s2 subu(s2 a, s2 b) {
         return __builtin_bit_cast(s2, __builtin_elementwise_sub_sat(__builtin_bit_cast(u2, a),
                                                                     __builtin_bit_cast(u2, b)));
     }
     __device__ static __forceinline__ s2 mx(s2 a, s2 b) { return __builtin_elementwise_max(a, b); }
+    __device__ static __forceinline__ s2 mx3(s2 a, s2 b, s2 c) { return mx(mx(a, b), c); }
     __device__ static __forceinline__ short clamp16(int v) { return (short)min(max(v, -32768), 32767); }
     __device__ static __forceinline__ s2 splat(int v) { short x = clamp16(v); return s2{x, x}; }
     __device__ static __forceinline__ int bits(s2 v) { return __builtin_bit_cast(int, v); }
@@ -73,15 +90,39 @@ template <> struct VOps<s2> {
     }
 };
 
+// Binary16 pairs: exact for integers in [-2048, 2048] (see the header comment).  The
+// IEEE-754-2019 maximum lowers to v_pk_maximum3_f16 with no operand canonicalisation
+// (maxNum would quiet each operand first: an extra instruction per input).
+template <> struct VOps<h2> {
+    static constexpr int NEG = -30000;   // sentinel: exact in binary16, far below any value
+    static constexpr int TOP = 2048;     // a result >= TOP may have been rounded
+    static constexpr bool PACKED = true;
+    static constexpr bool MAX3 = true;
+    __device__ static __forceinline__ h2 add(h2 a, h2 b) { return a + b; }
+    __device__ static __forceinline__ h2 mx(h2 a, h2 b) { return __builtin_elementwise_maximum(a, b); }
+    __device__ static __forceinline__ h2 mx3(h2 a, h2 b, h2 c) {
+        return __builtin_elementwise_maximum(__builtin_elementwise_maximum(a, b), c);
+    }
+    __device__ static __forceinline__ _Float16 cvt(int v) { return (_Float16)min(max(v, -30000), 30000); }
+    __device__ static __forceinline__ h2 splat(int v) { const _Float16 x = cvt(v); return h2{x, x}; }
+    __device__ static __forceinline__ int bits(h2 v) { return __builtin_bit_cast(int, v); }
+    __device__ static __forceinline__ h2 from_bits(int b) { return __builtin_bit_cast(h2, b); }
+    __device__ static __forceinline__ int lo(h2 v) { return (int)v.x; }
+    __device__ static __forceinline__ int hi(h2 v) { return (int)v.y; }
+    __device__ static __forceinline__ uint32_t pack(int a, int b) { return (uint32_t)bits(h2{cvt(a), cvt(b)}); }
+};
+
 template <> struct VOps<int> {
     static constexpr int NEG = -(1 << 30);   // the oracle's sentinel
     static constexpr int TOP = 0x7fffffff;
     static constexpr bool PACKED = false;
+    static constexpr bool MAX3 = false;
     __device__ static __forceinline__ int add(int a, int b) { return __builtin_elementwise_add_sat(a, b); }
     __device__ static __forceinline__ int subu(int a, int b) {
         return (int)__builtin_elementwise_sub_sat((unsigned)a, (unsigned)b);
     }
     __device__ static __forceinline__ int mx(int a, int b) { return max(a, b); }
+    __device__ static __forceinline__ int mx3(int a, int b, int c) { return max(max(a, b), c); }
     __device__ static __forceinline__ int splat(int v) { return v; }
     __device__ static __forceinline__ int bits(int v) { return v; }
     __device__ static __forceinline__ int from_bits(int b) { return b; }
@@ -129,6 +170,43 @@ __device__ __forceinline__ int col0_value(const SweepConst &k, int row) {
 }
 
 constexpr int RES_NONE = -0x40000000 - 7;   // below every reachable result
+
+// H(i, j-1) + gap_open for a row's first column.  The int16 local form clamps it at 0
+// (unsigned saturating subtraction of the penalty); the three-input-max form clamps E
+// and F instead, so it keeps the plain sum.
+template <typename V, bool LOCAL>
+__device__ __forceinline__ V row_hgo(V hin, V GOV, V GOP) {
+    using O = VOps<V>;
+    if constexpr (LOCAL && !O::MAX3) return O::subu(hin, GOP);
+    else return O::add(hin, GOV);
+}
+
+// One cell with the three-input max (column kk of a lane's R; see the header comment):
+//   local      E = max(E + ge, H + go, 0)   H = max(diag + s, F, E)   F = max(F + ge, H + go, 0)
+//   otherwise  E = max(E + ge, H + go)      H = max(diag + s, F, E)   F = max(F + ge, H + go)
+// 4 adds and 3 instructions of max per cell; local mode folds two columns' H into the
+// running maximum with one more three-input max.
+template <typename V, bool LOCAL, int R>
+__device__ __forceinline__ void cell_max3(int kk, V xd, V &e, V &h, V &hprev, V &best, V &hup, V &hgo, V &f,
+                                          V GOV, V GEV, V ZERO) {
+    using O = VOps<V>;
+    if (LOCAL) {
+        e = O::mx3(O::add(e, GEV), hgo, ZERO);
+        h = O::mx3(xd, f, e);
+        if (kk & 1) best = O::mx3(best, hprev, h);
+        else if (kk == R - 1) best = O::mx(best, h);
+        else hprev = h;
+        hup = h;
+        hgo = O::add(h, GOV);
+        f = O::mx3(O::add(f, GEV), hgo, ZERO);
+    } else {
+        e = O::mx(O::add(e, GEV), hgo);
+        h = O::mx3(xd, f, e);
+        hup = h;
+        hgo = O::add(h, GOV);
+        f = O::mx(O::add(f, GEV), hgo);
+    }
+}
 
 // Per-group results after a sweep.
 struct SweepOut {
@@ -246,15 +324,17 @@ __device__ __forceinline__ SweepOut sweep_group(const uint8_t *__restrict__ rseq
             if constexpr (PF) load_row(min(c_raw, k.ns - 1), pv);
             else load_row(code_at(row), pv);
             V e = ein;
-            V hgo = LOCAL ? O::subu(hin, GOP) : O::add(hin, GOV);
-            V h = hin;
+            V hgo = row_hgo<V, LOCAL>(hin, GOV, GOP);
+            V h = hin, hprev = ZERO;
             // xd = H(i-1, j-1) + s(i, j).  The sum for column kk+1 is taken before
             // hup[kk] is overwritten, so H(i, j) can replace H(i-1, j) in place.
             V xd = O::add(din, pv[0]);
 #pragma unroll
             for (int kk = 0; kk < R; ++kk) {
                 const V xn = (kk + 1 < R) ? O::add(hup[kk], pv[kk + 1]) : xd;
-                if (LOCAL) {
+                if constexpr (O::MAX3) {
+                    cell_max3<V, LOCAL, R>(kk, xd, e, h, hprev, best, hup[kk], hgo, fn[kk], GOV, GEV, ZERO);
+                } else if (LOCAL) {
                     e = O::mx(O::subu(e, GEP), hgo);
                     h = O::mx(O::mx(xd, fn[kk]), e);   // E last: the serial chain is E -> H -> H-gap -> E
                     best = O::mx(best, h);
@@ -420,13 +500,15 @@ __device__ __forceinline__ SweepOut sweep_pairs_pipe(const uint8_t *__restrict__
         load_row(cin, nxt);
         if ((unsigned)row < (unsigned)lr) {
             V e = ein;
-            V hgo = LOCAL ? O::subu(hin, GOP) : O::add(hin, GOV);
-            V h = hin;
+            V hgo = row_hgo<V, LOCAL>(hin, GOV, GOP);
+            V h = hin, hprev = ZERO;
             V xd = O::add(din, use[0]);
 #pragma unroll
             for (int kk = 0; kk < R; ++kk) {
                 const V xn = (kk + 1 < R) ? O::add(hup[kk], use[kk + 1]) : xd;
-                if (LOCAL) {
+                if constexpr (O::MAX3) {
+                    cell_max3<V, LOCAL, R>(kk, xd, e, h, hprev, best, hup[kk], hgo, fn[kk], GOV, GEV, ZERO);
+                } else if (LOCAL) {
                     e = O::mx(O::subu(e, GEP), hgo);
                     h = O::mx(O::mx(xd, fn[kk]), e);
                     best = O::mx(best, h);

@@ -21,327 +21,9 @@
 #include "gsw_internal.h"
 #include "gsw_sweep.h"
 #include "gsw_batch.h"
+#include "gsw_kernels.h"
 
 namespace gsw {
-
-// Store a result; with the saturation check on (packed local launches under
-// GSW_DTYPE_INT16), a result that reached 32767 is also listed for int32 rescoring.
-__device__ __forceinline__ void emit(const KArgs &a, int64_t o, int q, int r, int score) {
-    a.out[o] = score;
-    if (a.sat_check && score >= 32767) {
-        const int i = atomicAdd(a.sat_count, 1);
-        if (i < a.sat_cap) a.sat[i] = SatEntry{q, r, o};
-    }
-}
-
-// ---------------------------------------------------------------- cross kernel
-// A workgroup takes (query pair, reference range) items from a global counter, builds
-// the pair's packed profile in LDS once, then its waves take blocks of 64/G references
-// from an LDS counter; each group of G lanes sweeps one reference.
-// Occupancy target: 4 waves per SIMD (<= 128 VGPRs) for the narrow local tiles (G <= 8,
-// R <= 34), whose loop fits in 128 registers (R 33 and 34 spill two registers outside the
-// loop); the other tiles are left free.
-template <bool LOCAL, int G, int R> struct WavesPerEU { static constexpr int value = (LOCAL && G <= 8 && R <= 34) ? 4 : 1; };
-
-template <typename V, int G, int R, bool LOCAL, int WAVES>
-__global__ void __launch_bounds__(WAVES * 64)
-__attribute__((amdgpu_waves_per_eu(WavesPerEU<LOCAL, G, R>::value))) cross_kernel(KArgs a) {
-    using O = VOps<V>;
-    using L = ProfLayout<G, R>;
-    constexpr int LW = L::KC * 4;   // padded columns per lane
-    constexpr int GPW = 64 / G;
-    extern __shared__ __align__(16) char lds[];
-    __shared__ int s_item, s_blk;
-    const int lane = threadIdx.x & 63;
-    const int t = lane % G, g = lane / G;
-    const CrossItem *items = static_cast<const CrossItem *>(a.items);
-    for (;;) {
-        if (threadIdx.x == 0) {
-            s_item = atomicAdd(a.counter, 1);
-            s_blk = 0;
-        }
-        __syncthreads();
-        const int it = __builtin_amdgcn_readfirstlane(s_item);
-        if (it >= a.nitems) break;
-        const CrossItem item = items[it];
-        const int qsa = item.qsa, lqa = item.lqa;
-        const int qsb = item.lqb >= 0 ? item.qsb : qsa, lqb = item.lqb >= 0 ? item.lqb : lqa;
-        const int total = a.ns * G * LW;
-        for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-            const int c = idx / (G * LW);
-            const int rem = idx - c * (G * LW);
-            const int lt = rem / LW, kk = rem - lt * LW;
-            const int col = lt * R + kk - item.shift;   // front padding (col < 0) scores 0
-            const int va = kk >= R || col >= lqa ? O::NEG
-                           : col < 0 ? 0 : a.sub[c * a.ns + min((int)a.qcodes[qsa + col], a.ns - 1)];
-            const int vb = kk >= R || col >= lqb ? O::NEG
-                           : col < 0 ? 0 : a.sub[c * a.ns + min((int)a.qcodes[qsb + col], a.ns - 1)];
-            const uint32_t word = O::pack(va, vb);
-            char *dst = lds + c * L::ROWB + (kk >> 2) * L::CH + lt * 16 + (kk & 3) * 4;
-#pragma unroll
-            for (int cp = 0; cp < L::NSLOT; ++cp)   // one copy per group position in an LDS pass
-                *reinterpret_cast<uint32_t *>(dst + cp * G * 16) = word;
-        }
-        __syncthreads();
-        const char *prof_lane = lds + L::lane_offset(g, t);
-        for (;;) {
-            int blk = 0;
-            if (lane == 0) blk = atomicAdd(&s_blk, 1);
-            blk = __builtin_amdgcn_readfirstlane(blk);
-            const int r0 = item.rbeg + blk * GPW;
-            if (r0 >= item.rend) break;
-            const int rpos = r0 + g;
-            const int ref = rpos < item.rend ? a.rorder[rpos] : -1;
-            const int rs = ref >= 0 ? a.roff[ref] : 0;
-            const int lr = ref >= 0 ? a.roff[ref + 1] - rs : 0;
-            const int nsteps = __builtin_amdgcn_readfirstlane(wave_max_i32(lr)) + G - 1;
-            const SweepOut res = sweep_group<V, G, R, LOCAL>(a.rcodes + rs, lr, nsteps, prof_lane, t,
-                                                             a.k, lqa, lqb, -item.shift);
-            if (t == 0 && ref >= 0) {
-                emit(a, item.outa + ref, -1, ref, res.a);
-                if (item.outb >= 0) emit(a, item.outb + ref, -1, ref, res.b);
-            }
-        }
-        __syncthreads();
-    }
-}
-
-// ---------------------------------------------------------------- stripe kernel
-// Queries wider than a tile: the query is cut into stripes of W = G*R columns, and a
-// workgroup takes (query pair, reference range) items as the cross kernel does but runs
-// the item stripe by stripe: build the stripe's profile, let the waves sweep all the
-// item's references through it, then move right.  Each reference row's H and E at the
-// stripe edge are kept in the workgroup's boundary region of global memory (an H plane
-// and an E plane of bnd_stride words, one word per reference row of the item in sorted
-// order); results of a reference are combined over the stripes with max in the output,
-// and the saturation check sees the final value.
-template <typename V, int G, int R, bool LOCAL, int WAVES>
-__global__ void __launch_bounds__(WAVES * 64)
-__attribute__((amdgpu_waves_per_eu(WavesPerEU<LOCAL, G, R>::value))) stripe_kernel(KArgs a) {
-    using O = VOps<V>;
-    using L = ProfLayout<G, R>;
-    constexpr int LW = L::KC * 4;   // padded columns per lane
-    constexpr int GPW = 64 / G;
-    constexpr int W = G * R;
-    extern __shared__ __align__(16) char lds[];
-    __shared__ int s_item, s_blk;
-    const int lane = threadIdx.x & 63;
-    const int t = lane % G, g = lane / G;
-    const CrossItem *items = static_cast<const CrossItem *>(a.items);
-    uint32_t *bnd_wg = a.bnd + (size_t)blockIdx.x * 2 * (size_t)a.bnd_stride;
-    for (;;) {
-        if (threadIdx.x == 0) s_item = atomicAdd(a.counter, 1);
-        __syncthreads();
-        const int it = __builtin_amdgcn_readfirstlane(s_item);
-        if (it >= a.nitems) break;
-        const CrossItem item = items[it];
-        const int qsa = item.qsa, lqa = item.lqa;
-        const int qsb = item.lqb >= 0 ? item.qsb : qsa, lqb = item.lqb >= 0 ? item.lqb : lqa;
-        const int64_t row_base = a.bpos[item.rbeg];
-        for (int st = 0; st < item.nstr; ++st) {
-            const int cbase = st * W - item.shift;
-            if (threadIdx.x == 0) s_blk = 0;
-            const int total = a.ns * G * LW;
-            for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-                const int c = idx / (G * LW);
-                const int rem = idx - c * (G * LW);
-                const int lt = rem / LW, kk = rem - lt * LW;
-                const int col = cbase + lt * R + kk;   // front padding (col < 0) scores 0
-                const int va = kk >= R || col >= lqa ? O::NEG
-                               : col < 0 ? 0 : a.sub[c * a.ns + min((int)a.qcodes[qsa + col], a.ns - 1)];
-                const int vb = kk >= R || col >= lqb ? O::NEG
-                               : col < 0 ? 0 : a.sub[c * a.ns + min((int)a.qcodes[qsb + col], a.ns - 1)];
-                const uint32_t word = O::pack(va, vb);
-                char *dst = lds + c * L::ROWB + (kk >> 2) * L::CH + lt * 16 + (kk & 3) * 4;
-#pragma unroll
-                for (int cp = 0; cp < L::NSLOT; ++cp)
-                    *reinterpret_cast<uint32_t *>(dst + cp * G * 16) = word;
-            }
-            __syncthreads();
-            const bool first = st == 0, last = st + 1 == item.nstr;
-            const char *prof_lane = lds + L::lane_offset(g, t);
-            for (;;) {
-                int blk = 0;
-                if (lane == 0) blk = atomicAdd(&s_blk, 1);
-                blk = __builtin_amdgcn_readfirstlane(blk);
-                const int r0 = item.rbeg + blk * GPW;
-                if (r0 >= item.rend) break;
-                const int rpos = r0 + g;
-                const int ref = rpos < item.rend ? a.rorder[rpos] : -1;
-                const int rs = ref >= 0 ? a.roff[ref] : 0;
-                const int lr = ref >= 0 ? a.roff[ref + 1] - rs : 0;
-                const uint32_t boff = ref >= 0 ? (uint32_t)(a.bpos[rpos] - row_base) : 0u;
-                const int nsteps = __builtin_amdgcn_readfirstlane(wave_max_i32(lr)) + G - 1;
-                const SweepOut res = sweep_group<V, G, R, LOCAL, true>(a.rcodes + rs, lr, nsteps, prof_lane, t, a.k,
-                                                                       lqa, lqb, cbase, bnd_wg, bnd_wg + a.bnd_stride,
-                                                                       !first, !last, nullptr, 0, boff);
-                if (t == 0 && ref >= 0) {
-                    int va = res.a, vb = res.b;
-                    if (!first) {   // written by this workgroup in an earlier stripe
-                        va = max(va, a.out[item.outa + ref]);
-                        if (item.outb >= 0) vb = max(vb, a.out[item.outb + ref]);
-                    }
-                    if (last) {
-                        emit(a, item.outa + ref, -1, ref, va);
-                        if (item.outb >= 0) emit(a, item.outb + ref, -1, ref, vb);
-                    } else {
-                        a.out[item.outa + ref] = va;
-                        if (item.outb >= 0) a.out[item.outb + ref] = vb;
-                    }
-                }
-            }
-            __syncthreads();   // profile, partial results and boundaries complete
-        }
-    }
-}
-
-// ---------------------------------------------------------------- pairs kernel
-// Waves take blocks of 64/G tasks from a global counter.  Every lane writes the
-// profile entries of its own columns into the wave's LDS region and reads only those,
-// so no barrier is needed.  Tasks of one block share their front-padding shift (the host
-// pads blocks).  The profile holds rows 0..prow only (prow: the largest reference code of
-// the set, or the table's last row), so for DNA references without N a wave's profiles of
-// up to 20 columns per lane take 20 KiB and eight waves fit a CU; the sweep is pipelined
-// for that occupancy (sweep_pairs_pipe).
-template <typename V, int G, int R, bool LOCAL, int WAVES>
-__global__ void __launch_bounds__(WAVES * 64) pairs_kernel(KArgs a) {
-    using O = VOps<V>;
-    using L = ProfLayout<G, R>;
-    constexpr int KC = L::KC;
-    constexpr int GPW = 64 / G;
-    extern __shared__ __align__(16) char lds[];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int t = lane % G, g = lane / G;
-    const int nrows = a.prow + 1;
-    // blocks of NSLOT groups interleave their private profiles (see ProfLayout)
-    const int block_bytes = nrows * L::ROWB;
-    char *lane_lds = lds + (size_t)wave * (GPW / L::NSLOT) * block_bytes + (size_t)(g / L::NSLOT) * block_bytes +
-                     L::lane_offset(g, t);
-    // tables of up to 64 entries (ns <= 8, DNA) are held across the lanes of a register,
-    // lane l holding entry l, and read with ds_bpermute: no memory latency in the builds
-    const bool lane_table = a.ns * a.ns <= 64;
-    const int subv = lane_table && lane < a.ns * a.ns ? a.sub[lane] : 0;
-    const PairTask *tasks = static_cast<const PairTask *>(a.items);
-    const int nitems = a.nitems_ptr ? __builtin_amdgcn_readfirstlane(*a.nitems_ptr) : a.nitems;
-    const int nblocks = (nitems + GPW - 1) / GPW;
-    for (;;) {
-        int blk = 0;
-        if (lane == 0) blk = atomicAdd(a.counter, 1);
-        blk = __builtin_amdgcn_readfirstlane(blk);
-        if (blk >= nblocks) break;
-        const int ti = blk * GPW + g;
-        PairTask task{-1, -1, -1, 0, -1, -1};
-        if (ti < nitems) task = tasks[ti];
-        const int shift = __builtin_amdgcn_readfirstlane(tasks[blk * GPW].shift);
-        const bool valid = task.ref >= 0;
-        const int qa = valid ? task.qa : 0;
-        const int qb = (valid && task.qb >= 0) ? task.qb : qa;
-        const int qsa = valid ? a.qoff[qa] : 0, lqa = valid ? a.qoff[qa + 1] - qsa : 0;
-        const int qsb = valid ? a.qoff[qb] : 0, lqb = valid ? a.qoff[qb + 1] - qsb : 0;
-        const int rs = valid ? a.roff[task.ref] : 0;
-        const int lr = valid ? a.roff[task.ref + 1] - rs : 0;
-        // this lane's profile columns: all query codes are read first (one round trip), then
-        // each row's entries are looked up and written 4 columns at a time
-        int ca[KC * 4], cb[KC * 4];   // code, or -1 = back padding (NEG), -2 = front padding (0)
-#pragma unroll
-        for (int kk = 0; kk < KC * 4; ++kk) {
-            const int col = t * R + kk - shift;
-            ca[kk] = (kk >= R || col >= lqa) ? -1 : col < 0 ? -2 : min((int)a.qcodes[qsa + col], a.ns - 1);
-            cb[kk] = (kk >= R || col >= lqb) ? -1 : col < 0 ? -2 : min((int)a.qcodes[qsb + col], a.ns - 1);
-        }
-        auto entry = [&](int c, int code) -> int {
-            const int v = lane_table ? __builtin_amdgcn_ds_bpermute((c * a.ns + max(code, 0)) * 4, subv)
-                                     : a.sub[c * a.ns + max(code, 0)];
-            return code == -1 ? O::NEG : code == -2 ? 0 : v;
-        };
-#pragma unroll 1
-        for (int c = 0; c < nrows; ++c) {
-#pragma unroll
-            for (int kc = 0; kc < KC; ++kc) {
-                uint32_t w[4];
-#pragma unroll
-                for (int e = 0; e < 4; ++e) w[e] = O::pack(entry(c, ca[kc * 4 + e]), entry(c, cb[kc * 4 + e]));
-                *reinterpret_cast<int4 *>(lane_lds + c * L::ROWB + kc * L::CH) =
-                    int4{(int)w[0], (int)w[1], (int)w[2], (int)w[3]};
-            }
-        }
-        const int nsteps = __builtin_amdgcn_readfirstlane(wave_max_i32(lr)) + G - 1;
-        const SweepOut res = sweep_pairs_pipe<V, G, R, LOCAL>(a.rcodes + rs, lr, nsteps, lane_lds, t, a.k, lqa, lqb,
-                                                              -shift, a.prow);
-        if (t == 0 && valid) {
-            emit(a, task.oa, qa, task.ref, res.a);
-            if (task.ob >= 0) emit(a, task.ob, qb, task.ref, res.b);
-        }
-    }
-}
-
-// ---------------------------------------------------------------- pairs table kernel
-// Explicit pairs with the substitution table in LDS instead of per-lane profiles, for
-// alphabets whose profiles do not fit (protein beyond 160 columns) and to free LDS for
-// occupancy.  The workgroup builds T[c][a][b] = (sub[c][a] | sub[c][b] << 16) once, where
-// a, b run over the query codes plus two padding codes (NEG past a query's end, 0 for
-// front padding); int32 tiles use T[c][a].  Each lane keeps its R columns' byte offsets
-// into a table row in registers, so a column costs one LDS load and an address add.
-// Small tables are stored in pairs_tab_copies() interleaved copies (word w of copy c at
-// w*copies + c) and lane l reads copy l mod copies: the lanes of an LDS pass then hit
-// distinct banks whatever entries they look up.
-template <typename V, int G, int R, bool LOCAL, int WAVES>
-__global__ void __launch_bounds__(WAVES * 64) pairs_tab_kernel(KArgs a) {
-    using O = VOps<V>;
-    constexpr int GPW = 64 / G;
-    extern __shared__ __align__(16) char lds[];
-    const int lane = threadIdx.x & 63;
-    const int t = lane % G, g = lane / G;
-    const int ns = a.ns, nq = ns + 2, npad = ns, zpad = ns + 1;
-    const int row_words = O::PACKED ? nq * nq : nq;
-    const int copies = a.tab_copies;
-    for (int idx = threadIdx.x; idx < ns * row_words * copies; idx += blockDim.x) {
-        const int w = idx / copies;
-        const int c = w / row_words, rem = w - c * row_words;
-        const int qa = O::PACKED ? rem / nq : rem, qb = O::PACKED ? rem - (rem / nq) * nq : rem;
-        const int va = qa == npad ? O::NEG : qa == zpad ? 0 : a.sub[c * ns + qa];
-        const int vb = qb == npad ? O::NEG : qb == zpad ? 0 : a.sub[c * ns + qb];
-        reinterpret_cast<uint32_t *>(lds)[idx] = O::pack(va, vb);
-    }
-    __syncthreads();
-    const int trow = row_words * copies * 4;
-    const char *tab_lane = lds + (lane % copies) * 4;
-    const PairTask *tasks = static_cast<const PairTask *>(a.items);
-    const int nitems = a.nitems_ptr ? __builtin_amdgcn_readfirstlane(*a.nitems_ptr) : a.nitems;
-    const int nblocks = (nitems + GPW - 1) / GPW;
-    for (;;) {
-        int blk = 0;
-        if (lane == 0) blk = atomicAdd(a.counter, 1);
-        blk = __builtin_amdgcn_readfirstlane(blk);
-        if (blk >= nblocks) break;
-        const int ti = blk * GPW + g;
-        PairTask task{-1, -1, -1, 0, -1, -1};
-        if (ti < nitems) task = tasks[ti];
-        const int shift = __builtin_amdgcn_readfirstlane(tasks[blk * GPW].shift);
-        const bool valid = task.ref >= 0;
-        const int qa = valid ? task.qa : 0;
-        const int qb = (valid && task.qb >= 0) ? task.qb : qa;
-        const int qsa = valid ? a.qoff[qa] : 0, lqa = valid ? a.qoff[qa + 1] - qsa : 0;
-        const int qsb = valid ? a.qoff[qb] : 0, lqb = valid ? a.qoff[qb + 1] - qsb : 0;
-        const int rs = valid ? a.roff[task.ref] : 0;
-        const int lr = valid ? a.roff[task.ref + 1] - rs : 0;
-        int toff[R];
-#pragma unroll
-        for (int kk = 0; kk < R; ++kk) {
-            const int col = t * R + kk - shift;
-            const int ca = col >= lqa ? npad : col < 0 ? zpad : min((int)a.qcodes[qsa + col], ns - 1);
-            const int cb = col >= lqb ? npad : col < 0 ? zpad : min((int)a.qcodes[qsb + col], ns - 1);
-            toff[kk] = (O::PACKED ? ca * nq + cb : ca) * copies * 4;
-        }
-        const int nsteps = __builtin_amdgcn_readfirstlane(wave_max_i32(lr)) + G - 1;
-        const SweepOut res = sweep_group<V, G, R, LOCAL, false, true, true>(
-            a.rcodes + rs, lr, nsteps, tab_lane, t, a.k, lqa, lqb, -shift, nullptr, nullptr, false, false, toff, trow);
-        if (t == 0 && valid) {
-            emit(a, task.oa, qa, task.ref, res.a);
-            if (task.ob >= 0) emit(a, task.ob, qb, task.ref, res.b);
-        }
-    }
-}
 
 // ---------------------------------------------------------------- edge cells
 // Closed-form scores of pairs with an empty side, written to out[o[k]].
@@ -350,40 +32,6 @@ __global__ void scatter_kernel(int32_t *out, const int64_t *idx, const int32_t *
     if (k < n) out[idx[k]] = val[k];
 }
 
-// ---------------------------------------------------------------- variants
-
-template <typename V, int G, int R, bool L>
-static void launch_cross(const KArgs &a, int grid, size_t lds, hipStream_t st) {
-    hipLaunchKernelGGL((cross_kernel<V, G, R, L, CROSS_WAVES>), dim3(grid), dim3(CROSS_WAVES * 64), lds, st, a);
-}
-template <typename V, int G, int R, bool L>
-static void launch_pairs(const KArgs &a, int grid, size_t lds, hipStream_t st) {
-    hipLaunchKernelGGL((pairs_kernel<V, G, R, L, PAIRS_WAVES>), dim3(grid), dim3(PAIRS_WAVES * 64), lds, st, a);
-}
-
-template <typename V, int G, int R, bool L>
-static void launch_stripe(const KArgs &a, int grid, size_t lds, hipStream_t st) {
-    hipLaunchKernelGGL((stripe_kernel<V, G, R, L, CROSS_WAVES>), dim3(grid), dim3(CROSS_WAVES * 64), lds, st, a);
-}
-
-#define GSW_VARIANT(V, PK, G, R, L)                                                               \
-    Variant{0, G, R, PK, L, reinterpret_cast<const void *>(&cross_kernel<V, G, R, L, CROSS_WAVES>), \
-            reinterpret_cast<const void *>(&pairs_kernel<V, G, R, L, PAIRS_WAVES>), nullptr,      \
-            &launch_cross<V, G, R, L>, &launch_pairs<V, G, R, L>, nullptr,                        \
-            reinterpret_cast<const void *>(&pairs_tab_kernel<V, G, R, L, PAIRS_TAB_WAVES>)}
-#define GSW_VARIANT4(G, R)                                                                        \
-    GSW_VARIANT(s2, true, G, R, true), GSW_VARIANT(s2, true, G, R, false),                        \
-        GSW_VARIANT(int, false, G, R, true), GSW_VARIANT(int, false, G, R, false)
-// tiles that also run stripes of longer queries
-#define GSW_SVARIANT(V, PK, G, R, L)                                                              \
-    Variant{0, G, R, PK, L, reinterpret_cast<const void *>(&cross_kernel<V, G, R, L, CROSS_WAVES>), \
-            reinterpret_cast<const void *>(&pairs_kernel<V, G, R, L, PAIRS_WAVES>),               \
-            reinterpret_cast<const void *>(&stripe_kernel<V, G, R, L, CROSS_WAVES>),              \
-            &launch_cross<V, G, R, L>, &launch_pairs<V, G, R, L>, &launch_stripe<V, G, R, L>,     \
-            reinterpret_cast<const void *>(&pairs_tab_kernel<V, G, R, L, PAIRS_TAB_WAVES>)}
-#define GSW_SVARIANT4(G, R)                                                                       \
-    GSW_SVARIANT(s2, true, G, R, true), GSW_SVARIANT(s2, true, G, R, false),                      \
-        GSW_SVARIANT(int, false, G, R, true), GSW_SVARIANT(int, false, G, R, false)
 
 std::vector<Variant> &variant_table() {
     static std::vector<Variant> table = [] {
@@ -397,11 +45,22 @@ std::vector<Variant> &variant_table() {
             GSW_VARIANT4(16, 44), GSW_VARIANT4(16, 48), GSW_VARIANT4(32, 32), GSW_VARIANT4(32, 48),
             GSW_VARIANT4(64, 32),
         };
-        for (size_t i = 0; i < v.size(); ++i) v[i].id = (int)i;
-        return v;
+        // each tile's half2 rows follow its int16x2/int32 rows
+        const std::vector<Variant> h = half_variants();
+        std::vector<Variant> all;
+        for (size_t i = 0; i < v.size(); ++i) {
+            all.push_back(v[i]);
+            if (i + 1 == v.size() || v[i + 1].G != v[i].G || v[i + 1].R != v[i].R)
+                for (const Variant &x : h)
+                    if (x.G == v[i].G && x.R == v[i].R) all.push_back(x);
+        }
+        for (size_t i = 0; i < all.size(); ++i) all[i].id = (int)i;
+        return all;
     }();
     return table;
 }
+
+const char *kind_name(int kind) { return kind == KIND_F16 ? "half2" : kind == KIND_I16 ? "int16x2" : "int32"; }
 
 // Forced variant (G, R) for tuning; 0 = automatic.
 // Packed (G << 16 | R) in one atomic: planner threads read it while a caller may set it.
@@ -444,8 +103,9 @@ static double tile_occupancy_cost(const Variant &v, int ns) {
 
 // Cheapest compiled tile covering a query of length lq whose LDS profile fits
 // (cross: one profile per workgroup; pairs: one per lane group).
-const Variant *pick_variant(int lq, double mean_lr, bool packed, bool local, int ns,
+const Variant *pick_variant(int lq, double mean_lr, int kind, bool local, int ns,
                                    size_t lds_limit, bool for_pairs) {
+    const bool packed = kind != KIND_I32;
     const Variant *best = nullptr;
     double best_cost = 1e30;
     int g_force_g, g_force_r;
@@ -455,13 +115,13 @@ const Variant *pick_variant(int lq, double mean_lr, bool packed, bool local, int
         for (const TileRule &r : g_rules) {
             if (lq < r.lo || lq > r.hi || r.packed != (int)packed || r.local != (int)local) continue;
             for (const Variant &v : variant_table())
-                if (v.G == r.G && v.R == r.R && v.packed == packed && v.local == local && v.G * v.R >= lq &&
+                if (v.G == r.G && v.R == r.R && v.kind == kind && v.local == local && v.G * v.R >= lq &&
                     (for_pairs ? pairs_waves(v, ns, lds_limit) > 0 : cross_lds_bytes(v, ns) <= lds_limit))
                     return &v;
         }
     }
     for (const Variant &v : variant_table()) {
-        if (v.packed != packed || v.local != local) continue;
+        if (v.kind != kind || v.local != local) continue;
         if (v.G * v.R < lq) continue;
         if (for_pairs ? pairs_waves(v, ns, lds_limit) == 0 : cross_lds_bytes(v, ns) > lds_limit) continue;
         if (g_force_g && (v.G != g_force_g || v.R != g_force_r)) continue;
@@ -491,13 +151,14 @@ static int stripes_for(const Variant &v, int lq, bool track_last_col) {
     return (lq + shift + w - 1) / w;
 }
 
-const Variant *pick_cross_tile(int lq, double mean_lr, bool packed, bool local, int ns, size_t lds_limit,
+const Variant *pick_cross_tile(int lq, double mean_lr, int kind, bool local, int ns, size_t lds_limit,
                                bool track_last_col, int *nstr) {
     *nstr = 1;
+    const bool packed = kind != KIND_I32;
     int g_force_g, g_force_r;
     forced_tile(&g_force_g, &g_force_r);
     auto usable = [&](const Variant &v, int G, int R) {
-        return v.G == G && v.R == R && v.packed == packed && v.local == local && cross_lds_bytes(v, ns) <= lds_limit;
+        return v.G == G && v.R == R && v.kind == kind && v.local == local && cross_lds_bytes(v, ns) <= lds_limit;
     };
     auto take = [&](const Variant &v) -> const Variant * {
         if (v.G * v.R >= lq) return &v;
@@ -518,10 +179,10 @@ const Variant *pick_cross_tile(int lq, double mean_lr, bool packed, bool local, 
                 if (usable(v, r.G, r.R) && (v.G * v.R >= lq || v.stripe_fn)) return take(v);
         }
     }
-    const Variant *best = pick_variant(lq, mean_lr, packed, local, ns, lds_limit, false);
+    const Variant *best = pick_variant(lq, mean_lr, kind, local, ns, lds_limit, false);
     double best_cost = best ? tile_cost(best->G, best->R, lq, mean_lr) * tile_occupancy_cost(*best, ns) : 1e30;
     for (const Variant &v : variant_table()) {
-        if (!v.stripe_fn || v.packed != packed || v.local != local || v.G * v.R >= lq) continue;
+        if (!v.stripe_fn || v.kind != kind || v.local != local || v.G * v.R >= lq) continue;
         if (cross_lds_bytes(v, ns) > lds_limit) continue;
         const int n = stripes_for(v, lq, track_last_col);
         const double c = stripe_cost(v.G, v.R, lq, mean_lr, n) * tile_occupancy_cost(v, ns);
@@ -573,7 +234,7 @@ static int pairs_kernel_pref() {
     return e && e[0] == '1' ? 1 : e && e[0] == '0' ? 0 : -1;
 }
 
-const Variant *pick_pairs_tile(int lq, double mean_lr, bool packed, bool local, int ns, int prof_rows,
+const Variant *pick_pairs_tile(int lq, double mean_lr, int kind, bool local, int ns, int prof_rows,
                                size_t lds_limit, bool *tab) {
     const int pref = prof_rows <= 0 ? 1 : pairs_kernel_pref();
     const Variant *best = nullptr;
@@ -582,7 +243,7 @@ const Variant *pick_pairs_tile(int lq, double mean_lr, bool packed, bool local, 
     int g_force_g, g_force_r;
     forced_tile(&g_force_g, &g_force_r);
     for (const Variant &v : variant_table()) {
-        if (v.packed != packed || v.local != local || v.G * v.R < lq) continue;
+        if (v.kind != kind || v.local != local || v.G * v.R < lq) continue;
         if (g_force_g && (v.G != g_force_g || v.R != g_force_r)) continue;
         const int pw = prof_rows > 0 ? pairs_waves(v, prof_rows, lds_limit) : 0;
         const bool tab_ok = v.pairs_tab_fn && pairs_tab_lds_bytes(v, ns) <= lds_limit;
@@ -730,43 +391,47 @@ int plan_cross_units(const SchemeDev &s, int dtype_policy, const std::vector<Cro
     (void)wmax;
     struct QInfo { CrossQuery q; const Variant *v; int shift, nstr; };
     std::vector<QInfo> packed_q, single_q;
-    std::vector<CrossQuery> packed_free;   // int16 queries paired before their tile is chosen
+    std::vector<CrossQuery> packed_free[3];   // packed queries (by kind) paired before their tile is chosen
+    const bool half = half_enabled();
     for (const CrossQuery &q : queries) {
         if (!fits_int32(s, q.len, max_lr))
             return fail(GSW_ERR_OVERFLOW, "score range of query %d exceeds the int32 kernels", q.id);
-        const bool use16 = use_int16(s, dtype_policy, q.len, max_lr);
-        if (use16 && !track_last_col) {
-            packed_free.push_back(q);
+        const int kind = pair_kind(s, dtype_policy, fits_int16(s, q.len, max_lr), fits_half(s, q.len, max_lr), half);
+        if (kind != KIND_I32 && !track_last_col) {
+            packed_free[kind].push_back(q);
             continue;
         }
         int nstr = 1;
-        const Variant *v = pick_cross_tile(q.len, mean_lr, use16, local, s.ns, lds_limit, track_last_col, &nstr);
+        const Variant *v = pick_cross_tile(q.len, mean_lr, kind, local, s.ns, lds_limit, track_last_col, &nstr);
         if (!v) {
             plain->push_back(q);
             continue;
         }
-        (use16 ? packed_q : single_q)
+        (kind != KIND_I32 ? packed_q : single_q)
             .push_back(QInfo{q, v, track_last_col ? (v->R - q.len % v->R) % v->R : 0, nstr});
     }
-    // Without last-column tracking any two int16 queries can share a unit: pair neighbours
-    // in length order and give the pair the tile of the longer one (the shorter one's extra
-    // columns are padding), so every tile and stripe count is left with no lone query.
-    std::stable_sort(packed_free.begin(), packed_free.end(),
-                     [](const CrossQuery &x, const CrossQuery &y) { return x.len < y.len; });
+    // Without last-column tracking any two packed queries of a kind can share a unit: pair
+    // neighbours in length order and give the pair the tile of the longer one (the shorter
+    // one's extra columns are padding), so every tile and stripe count is left with no lone
+    // query.
     const CrossQuery none{0, 0, -1, -1};
-    for (size_t i = 0; i < packed_free.size(); i += 2) {
-        const CrossQuery &a = packed_free[i];
-        const bool has_b = i + 1 < packed_free.size();
-        const CrossQuery &b = has_b ? packed_free[i + 1] : none;
-        const int lq = std::max(a.len, has_b ? b.len : 0);
-        int nstr = 1;
-        const Variant *v = pick_cross_tile(lq, mean_lr, true, local, s.ns, lds_limit, false, &nstr);
-        if (!v) {
-            plain->push_back(a);
-            if (has_b) plain->push_back(b);
-            continue;
+    for (int kind = KIND_I16; kind <= KIND_F16; ++kind) {
+        std::vector<CrossQuery> &pf = packed_free[kind];
+        std::stable_sort(pf.begin(), pf.end(), [](const CrossQuery &x, const CrossQuery &y) { return x.len < y.len; });
+        for (size_t i = 0; i < pf.size(); i += 2) {
+            const CrossQuery &a = pf[i];
+            const bool has_b = i + 1 < pf.size();
+            const CrossQuery &b = has_b ? pf[i + 1] : none;
+            const int lq = std::max(a.len, has_b ? b.len : 0);
+            int nstr = 1;
+            const Variant *v = pick_cross_tile(lq, mean_lr, kind, local, s.ns, lds_limit, false, &nstr);
+            if (!v) {
+                plain->push_back(a);
+                if (has_b) plain->push_back(b);
+                continue;
+            }
+            units->push_back(CrossUnit{v, has_b ? b : a, has_b ? a : none, has_b, 0, nstr});
         }
-        units->push_back(CrossUnit{v, has_b ? b : a, has_b ? a : none, has_b, 0, nstr});
     }
     // pair packed queries of the same tile, stripe count and shift (wave-uniform)
     auto key = [&](const QInfo &x) { return std::make_tuple(x.v->id, x.nstr, track_last_col ? x.shift : 0, x.q.len); };
@@ -960,18 +625,19 @@ int batch_cross(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
     // type.  Not with last-column tracking, whose front padding depends on R.
     if (!(s.mode == GSW_MODE_SEMIGLOBAL && s.free_q)) {
         double cells = 0;
-        const Variant *wide[2] = {nullptr, nullptr};   // int32, int16x2
+        const Variant *wide[3] = {nullptr, nullptr, nullptr};   // by kind
         std::set<const Variant *> used;
         for (const CrossUnit &u : units) {
             cells += (double)(u.a.len + (u.has_b ? u.b.len : 0)) * (double)rs.total;
             if (u.nstr != 1) continue;
             used.insert(u.v);
-            const Variant *&w = wide[u.v->packed ? 1 : 0];
+            const Variant *&w = wide[u.v->kind];
             if (!w || u.v->G * u.v->R > w->G * w->R) w = u.v;
         }
-        if (cells < 1e10 && used.size() > (size_t)((wide[0] != nullptr) + (wide[1] != nullptr))) {
+        const size_t nkinds = (wide[0] != nullptr) + (wide[1] != nullptr) + (wide[2] != nullptr);
+        if (cells < 1e10 && used.size() > nkinds) {
             for (CrossUnit &u : units)
-                if (u.nstr == 1) u.v = wide[u.v->packed ? 1 : 0];
+                if (u.nstr == 1) u.v = wide[u.v->kind];
             std::stable_sort(units.begin(), units.end(), [](const CrossUnit &x, const CrossUnit &y) {
                 return std::make_pair(x.nstr == 1, x.v->id) < std::make_pair(y.nstr == 1, y.v->id);
             });
@@ -1116,7 +782,7 @@ int batch_cross(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
         }
         if (trace)
             std::fprintf(stderr, "[gsw] launch %zu: G=%d R=%d %s %s%s units=%zu items=%zu cells=%.3e ms=%.2f "
-                         "GCUPS=%.0f padded_GCUPS=%.0f\n", gi, v.G, v.R, v.packed ? "int16x2" : "int32",
+                         "GCUPS=%.0f padded_GCUPS=%.0f\n", gi, v.G, v.R, kind_name(v.kind),
                          v.local ? "local" : "nonlocal", g.striped ? " striped" : "", g.u1 - g.u0, g.i1 - g.i0,
                          (double)cells, gms, (double)cells / (gms * 1e6), width / (gms * 1e6));
     }
@@ -1148,21 +814,25 @@ int batch_cross(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
     bool overflowed = false;
     GSW_TRY(read_saturated(ctx, sb, &sat, &overflowed));
     if (overflowed) {
-        // too many saturated results to list: every row of a packed local unit again, on the
-        // int32 tiles (the rows' empty-reference cells are written again with the same values)
-        std::vector<int32_t> rows, ql;
-        for (const CrossUnit &u : units) {
-            if (!u.v->packed) continue;
-            rows.push_back((int32_t)(u.a.out / nr)); ql.push_back(u.a.id);
-            if (u.has_b) { rows.push_back((int32_t)(u.b.out / nr)); ql.push_back(u.b.id); }
+        // too many listed results: every row of a packed local unit again (the rows'
+        // empty-reference cells are written again with the same values), half2 rows on the
+        // int16 tiles (whose own saturated results are listed in turn), int16 rows on int32
+        for (int kind = KIND_I16; kind <= KIND_F16; ++kind) {
+            std::vector<int32_t> rows, ql;
+            for (const CrossUnit &u : units) {
+                if (u.v->kind != kind) continue;
+                rows.push_back((int32_t)(u.a.out / nr)); ql.push_back(u.a.id);
+                if (u.has_b) { rows.push_back((int32_t)(u.b.out / nr)); ql.push_back(u.b.id); }
+            }
+            if (ql.empty()) continue;
+            BatchResult r;
+            GSW_TRY(batch_cross(ctx, s, kind == KIND_F16 ? GSW_DTYPE_PK16 : GSW_DTYPE_INT32, qs, ql.data(),
+                                (int32_t)ql.size(), rs, d_out, &r, rows.data()));
+            res->kernel_ms += r.kernel_ms;
+            res->launches += r.launches;
+            res->plain_pairs += r.plain_pairs;
+            res->rescored += (int64_t)ql.size() * nr + r.rescored;
         }
-        BatchResult r;
-        GSW_TRY(batch_cross(ctx, s, GSW_DTYPE_INT32, qs, ql.data(), (int32_t)ql.size(), rs, d_out, &r,
-                            rows.data()));
-        res->kernel_ms += r.kernel_ms;
-        res->launches += r.launches;
-        res->plain_pairs += r.plain_pairs;
-        res->rescored += (int64_t)ql.size() * nr;
     } else if (!sat.empty()) {
         // the listed results as explicit pairs: the output row names the query
         std::vector<std::pair<int64_t, int32_t>> row_query((size_t)nq);
@@ -1297,7 +967,7 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
     const int wmax = max_width();
     const double mean_lr = rs.n_nonempty ? (double)rs.total / rs.n_nonempty : 1.0;
     const int T = n >= 65536 ? host_threads() : 1;
-    const bool local16 = dtype_policy == GSW_DTYPE_INT16 && local && int16_entries(s);
+    const bool half = half_enabled();
     // profile rows of the profile kernel: codes past the set's largest one never occur
     const int prow = std::min(s.ns - 1, rs.max_code);
     const size_t nv = 2 * variant_table().size();   // kinds: 2 * variant id + (table kernel)
@@ -1362,9 +1032,9 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
         P.cells = 0;
         P.bad = -1;
         P.bad_code = GSW_OK;
-        std::vector<const Variant *> vcache(2 * (wmax + 1), nullptr);
-        std::vector<char> vdone(2 * (wmax + 1), 0), vtab(2 * (wmax + 1), 0);
-        std::vector<int> lim16(wmax + 1, -1), lim32(wmax + 1, -1);   // -1: not computed
+        std::vector<const Variant *> vcache(3 * (wmax + 1), nullptr);
+        std::vector<char> vdone(3 * (wmax + 1), 0), vtab(3 * (wmax + 1), 0);
+        std::vector<int> limh(wmax + 1, -1), lim16(wmax + 1, -1), lim32(wmax + 1, -1);   // -1: not computed
         for (int64_t k = a; k < b; ++k) {
             const int q = h_qi[k], r = h_rj[k];
             vid[k] = -1;
@@ -1380,32 +1050,35 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
                 P.e_val.push_back(empty_edge_score(s, lq, lr));
                 continue;
             }
-            bool ok32, ok16;
+            bool ok32, ok16, okh;
             if (lq <= wmax) {
                 if (lim32[lq] < 0) {
                     lim32[lq] = lr_limit(fits_int32, lq);
                     lim16[lq] = lr_limit(fits_int16, lq);
+                    limh[lq] = lr_limit(fits_half, lq);
                 }
                 ok32 = lr <= lim32[lq];
                 ok16 = lr <= lim16[lq];
+                okh = lr <= limh[lq];
             } else {
                 ok32 = fits_int32(s, lq, lr);
                 ok16 = fits_int16(s, lq, lr);
+                okh = fits_half(s, lq, lr);
             }
             if (!ok32) {
                 P.bad = k;
                 P.bad_code = GSW_ERR_OVERFLOW;
                 return;
             }
-            // GSW_DTYPE_INT16 runs every local pair packed (saturated results are listed)
-            const bool use16 = dtype_policy == GSW_DTYPE_INT32 ? false : local16 ? true : ok16;
+            // GSW_DTYPE_INT16 runs every local pair packed (results at the type's top are listed)
+            const int kind = pair_kind(s, dtype_policy, ok16, okh, half);
             const Variant *v = nullptr;
             bool tab = false;
             if (lq <= wmax) {
-                const size_t ci = (size_t)(use16 ? 1 : 0) * (wmax + 1) + lq;
+                const size_t ci = (size_t)kind * (wmax + 1) + lq;
                 if (!vdone[ci]) {
                     bool tb = false;
-                    vcache[ci] = pick_pairs_tile(lq, mean_lr, use16, local, s.ns, prow + 1, (size_t)ctx->max_smem,
+                    vcache[ci] = pick_pairs_tile(lq, mean_lr, kind, local, s.ns, prow + 1, (size_t)ctx->max_smem,
                                                  &tb);
                     vtab[ci] = tb;
                     vdone[ci] = 1;
@@ -1732,11 +1405,12 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
         }
     }
 
-    // saturated int16 results (GSW_DTYPE_INT16, local) are listed before any other call
-    // resets the list; they run again on the int32 tiles below: the listed ones or, when more
-    // saturated than the list holds, every pair that ran packed
-    std::vector<int32_t> rqi, rrj;
-    std::vector<int64_t> roi;
+    // local results at the top of a packed type (int16 32767, half2 2048) are listed before
+    // any other call resets the list; they run again below: the listed ones on the int32
+    // tiles or, when more were listed than the list holds, every pair that ran packed (half2
+    // ones on the int16 tiles, whose own saturated results are listed in turn)
+    std::vector<int32_t> rqi, rrj, hqi, hrj;
+    std::vector<int64_t> roi, hoi;
     {
         std::vector<SatEntry> sat;
         bool overflowed = false;
@@ -1747,9 +1421,10 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
                 if (vid[k] < 0) continue;
                 const Variant &v = vt[vid[k] / 2];
                 if (!v.packed || !v.local) continue;
-                rqi.push_back(h_qi[k]);
-                rrj.push_back(h_rj[k]);
-                roi.push_back(oidx(k));
+                const bool h = v.kind == KIND_F16;
+                (h ? hqi : rqi).push_back(h_qi[k]);
+                (h ? hrj : rrj).push_back(h_rj[k]);
+                (h ? hoi : roi).push_back(oidx(k));
             }
         } else {
             for (const SatEntry &e : sat) {
@@ -1807,6 +1482,15 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
         }
     }
     GSW_TRY(scatter_values(ctx, d_out, e_idx, e_val));
+    if (!hqi.empty()) {
+        BatchResult r;
+        GSW_TRY(batch_pairs(ctx, s, GSW_DTYPE_PK16, qs, rs, hqi.data(), hrj.data(), (int64_t)hqi.size(), d_out, &r,
+                            hoi.data()));
+        res->kernel_ms += r.kernel_ms;
+        res->launches += r.launches;
+        res->plain_pairs += r.plain_pairs;
+        res->rescored += (int64_t)hqi.size() + r.rescored;
+    }
     return rescore_pairs_int32(ctx, s, qs, rs, rqi, rrj, roi, d_out, res);
 }
 
@@ -1931,6 +1615,16 @@ extern "C" int gsw_list_stripe_variants(int32_t *gr, int32_t cap, int32_t *n) {
         ++cnt;
     }
     *n = cnt;
+    return GSW_OK;
+}
+
+extern "C" int gsw_variant_info(int32_t id, int32_t *info) {
+    const std::vector<Variant> &vt = variant_table();
+    if (!info || id < 0 || id >= (int32_t)vt.size()) return fail(GSW_ERR_INVALID, "no variant %d", id);
+    info[0] = vt[id].G;
+    info[1] = vt[id].R;
+    info[2] = vt[id].kind;
+    info[3] = vt[id].local ? 1 : 0;
     return GSW_OK;
 }
 

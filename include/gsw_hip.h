@@ -171,6 +171,10 @@ int gsw_topk_rows(const int32_t *d_scores, int32_t nrows, int32_t ncols, int32_t
 int gsw_set_tile_table(const int32_t *rules, int32_t n);
 int gsw_force_variant(int32_t G, int32_t R);
 int gsw_list_variants(int32_t *gr, int32_t cap, int32_t *n);   /* (G, R) pairs */
+/* The tile of a variant id as reported in gsw_stats (variant, top_variant): info[0..3] =
+ * G, R, element kind (0 int32, 1 int16x2, 2 half2: binary16 pairs, exact for integers in
+ * [-2048, 2048]), local mode (1) or global/semiglobal (0). */
+int gsw_variant_info(int32_t id, int32_t *info);
 /* Tiles that also run queries wider than G*R as stripes (forcing one of them stripes
  * every query it does not cover). */
 int gsw_list_stripe_variants(int32_t *gr, int32_t cap, int32_t *n);

@@ -1,4 +1,7 @@
-"""Shared helpers for the tests (seeded sequence generators, scheme table)."""
+"""Shared helpers for the tests (seeded sequence generators, scheme table, C ABI calls
+with an explicit dtype policy)."""
+import ctypes
+
 import numpy as np
 
 from gpusw_amd import Matrix, Scheme
@@ -32,3 +35,36 @@ def schemes():
 def stable_seed(name: str) -> int:
     """Process-independent seed (the reference used hash(), which varies per process)."""
     return sum((i + 1) * ord(c) for i, c in enumerate(name)) % (2**32)
+
+
+def call_pairs(al, qi, rj, policy):
+    """gsw_score_pairs through the C ABI with an explicit dtype policy: (scores, stats)."""
+    from gpusw_amd import _hip
+    from gpusw_amd.backends.hip import scheme_struct_cached
+
+    qset = _hip.seqset_for(al._d_qbuf, al._d_qoff)
+    rset = _hip.seqset_for(al._d_rbuf, al._d_roff)
+    s, _table = scheme_struct_cached(al.scheme)
+    n = int(qi.size)
+    out = _hip.DeviceBuffer(4 * n, np.int32, (n,))
+    stats = _hip.GswStats()
+    _hip.call("gsw_score_pairs", ctypes.byref(s), policy, qset.handle, rset.handle, _hip.ptr(qi),
+              _hip.ptr(rj), n, out.ptr, ctypes.byref(stats))
+    return out.to_host(), stats
+
+
+def call_cross(al, policy):
+    """gsw_score_cross of all queries against all references with an explicit dtype policy."""
+    from gpusw_amd import _hip
+    from gpusw_amd.backends.hip import scheme_struct_cached
+
+    qset = _hip.seqset_for(al._d_qbuf, al._d_qoff)
+    rset = _hip.seqset_for(al._d_rbuf, al._d_roff)
+    s, _table = scheme_struct_cached(al.scheme)
+    nq, nr = al._queries.n, al._refs.n
+    qlist = np.arange(nq, dtype=np.int32)
+    out = _hip.DeviceBuffer(4 * nq * nr, np.int32, (nq, nr))
+    stats = _hip.GswStats()
+    _hip.call("gsw_score_cross", ctypes.byref(s), policy, qset.handle, _hip.ptr(qlist), nq,
+              rset.handle, out.ptr, ctypes.byref(stats))
+    return out.to_host(), stats

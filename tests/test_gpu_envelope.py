@@ -6,7 +6,9 @@ The reference resolves one element type per call from the longest query and refe
 could leave +/-30,000 (``backends/cuda.py:68-85``).  ``dtype="auto"`` here applies that
 envelope to every pair; ``GSW_DTYPE_INT16`` (a C caller vouching for int16) runs every
 local pair packed and rescores the results that reach 32767 -- the listed ones, or, when
-more than the list holds (2^20) saturate, every pair that ran packed.
+more than the list holds (2^20) saturate, every pair that ran packed.  The int16 tests here
+turn the half2 tiles off (``GSW_HALF=0``), which would otherwise take these pairs first;
+``test_gpu_half.py`` covers that route.
 """
 import ctypes
 
@@ -87,9 +89,11 @@ def test_heavy_pairs_auto_route_to_int32(heavy_pairs):
     _check_heavy(al, qi, rj, same, got)
 
 
-def test_heavy_pairs_int16_overflow_rescored(heavy_pairs):
-    """The same pairs forced to int16 through the C ABI: every identical pair saturates,
-    more than the saturation list holds, so every packed pair is rescored in int32."""
+def test_heavy_pairs_int16_overflow_rescored(heavy_pairs, monkeypatch):
+    """The same pairs forced to int16 through the C ABI (half2 tiles off): every identical
+    pair saturates, more than the saturation list holds, so every packed pair is rescored
+    in int32."""
+    monkeypatch.setenv("GSW_HALF", "0")
     al, qi, rj, same = heavy_pairs
     got, st = _call_pairs(al, qi, rj, _hip.DTYPE_INT16)
     assert st.plain_pairs == 0
@@ -97,8 +101,10 @@ def test_heavy_pairs_int16_overflow_rescored(heavy_pairs):
     _check_heavy(al, qi, rj, same, got)
 
 
-def test_pairs_int16_listed_saturation_rescored():
-    """A few saturating pairs among many: only the listed ones are rescored."""
+def test_pairs_int16_listed_saturation_rescored(monkeypatch):
+    """A few saturating pairs among many (int16 tiles, half2 off): only the listed ones are
+    rescored."""
+    monkeypatch.setenv("GSW_HALF", "0")
     rng = np.random.default_rng(99)
     seqs = rand_seqs(rng, 40, 100, 160, "ACGT")
     al = Aligner(HEAVY, backend="hip").index(seqs).set_queries(seqs)
@@ -127,7 +133,8 @@ def test_cross_envelope_int32_no_comparator():
     assert np.array_equal(got.astype(np.int64), want)
 
 
-def test_cross_int16_listed_and_overflow():
+def test_cross_int16_listed_and_overflow(monkeypatch):
+    monkeypatch.setenv("GSW_HALF", "0")
     rng = np.random.default_rng(6)
     # listed: a few saturating cells
     queries = rand_seqs(rng, 30, 120, 160, "ACGT")
@@ -171,9 +178,10 @@ def test_long_query_pairs_run_transposed(mode, fq, fr):
     assert st.plain_pairs == 2
 
 
-def test_long_query_saturation_rescored_transposed():
-    """A saturated long-query pair (forced int16) is rescored on the int32 tiles through
-    the transposed route."""
+def test_long_query_saturation_rescored_transposed(monkeypatch):
+    """A saturated long-query pair (forced int16, half2 off) is rescored on the int32 tiles
+    through the transposed route."""
+    monkeypatch.setenv("GSW_HALF", "0")
     rng = np.random.default_rng(77)
     ref = rand_seqs(rng, 1, 150, 151, "ACGT")[0]
     queries = ["".join(rng.choice(list("ACGT"), 1500)) + ref + "".join(rng.choice(list("ACGT"), 1000))]

@@ -68,10 +68,12 @@ def test_protein_beyond_widest_tile(mode):
     assert st.plain_pairs == 0
 
 
-def test_saturation_in_late_stripe_rescored():
+def test_saturation_in_late_stripe_rescored(monkeypatch):
     """A local score past int16 that builds up over several stripes.  With dtype "auto" the
-    envelope sends the pair to the int32 stripes; forced to int16 through the C ABI, the
-    int16 result reaches 32767 only in a late stripe and is rescored on the int32 tiles."""
+    envelope sends the pair to the int32 stripes; forced to int16 through the C ABI (half2
+    tiles off), the int16 result reaches 32767 only in a late stripe and is rescored on the
+    int32 tiles."""
+    monkeypatch.setenv("GSW_HALF", "0")
     scheme = Scheme.preset("blosum62")
     rng = np.random.default_rng(21)
     w = "W" * 3100                      # self score 11 per residue: 34100 > 32767

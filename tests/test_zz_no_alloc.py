@@ -58,7 +58,8 @@ def test_noalloc_equals_batched_two_calls(mode):
 
 
 @pytest.mark.gpu
-def test_noalloc_int16_saturation_rescored():
+def test_noalloc_int16_saturation_rescored(monkeypatch):
+    monkeypatch.setenv("GSW_HALF", "0")   # both entry points on the int16 tiles
     heavy = Scheme(mode="local", match=300, mismatch=-3, gap_open=-5, gap_extend=-2)
     rng = np.random.default_rng(33)
     seqs = rand_seqs(rng, 50, 100, 170, "ACGT")
