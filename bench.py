@@ -7,12 +7,15 @@ scheme, int16 with int32 fallback.  One step = the full (2000, 20000) score matr
 computed into HBM plus a device top_k(5) of every row.  Inputs are resident in HBM
 before the timed region.  Also reported:
 
-* ``roofline``: packed-int16 VALU roofline of the cross kernels -- algorithmic int16 ops
-  (cells x 9 for local mode: 4 add/sub + 5 max, the clamp max(H, 0) folded into saturating
-  E/F updates; global/semiglobal 8) over the device time of the call's batched launches,
-  which overlap on two streams (HIP events from the first launch's start to the last one's
-  end), against the gfx950 peak of 78.6 T int16-ops/s (the 10-op count that prices
-  max(H, 0) separately is kept as ``survey_frac``); the largest launch is reported beside it;
+* ``roofline``: packed 16-bit VALU roofline of the cross kernels -- the VALU lane-slots the
+  recurrence needs per cell on the tiles' element type (half2: 7.5 in local mode = 4 adds,
+  3 three-input maxima and half a maximum for the running best, 7 otherwise; int16x2: 9 and
+  8), times cells, over the device time of the call's batched launches, which overlap on
+  two streams (HIP events from the first launch's start to the last one's end), against
+  the gfx950 peak of 78.6 T lane-slots/s; ``recurrence_ops_rate`` counts add/sub/max
+  operations instead (local 9, the clamp max(H, 0) folded; the survey's 10 in
+  ``survey_ops_per_cell``), which a three-input max retires two at a time; the largest
+  launch is reported beside it;
 * ``plain``: the one-thread-per-pair comparator on the same GPU (a query subsample);
 * ``cpu_baseline``: the oracle's C restatement on the host cores over a bounded sample
   of the same pairs (those pairs are also checked bit-exact against the GPU scores), and
@@ -47,12 +50,47 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 METRIC = "bit-exact GCUPS on 1 MI355X (batched vs plain); p50/p99 latency when served"
-PEAK_INT16_TOPS = 78.6          # 256 CU x 64 lanes x 2 (packed) x 2.4 GHz, int16 ops/s / 1e12
-# int16 ops per cell update: local 4 add/sub + 5 max (E, F and H-gap kept >= 0 by unsigned
-# saturating subtraction, so max(H, 0) needs no op of its own), global/semiglobal 4 + 4
-OPS_PER_CELL = {"local": 9, "global": 8, "semiglobal": 8}
-# the survey's count, which prices max(H, 0) as a separate op in local mode
-SURVEY_OPS_PER_CELL = {"local": 10, "global": 8, "semiglobal": 8}
+PEAK_SLOTS = 78.6   # T packed-16-bit VALU lane-slots/s: 256 CU x 64 lanes x 2 halves x 2.4 GHz
+# VALU lane-slots per cell update, by tile element kind and local mode.  half2 (binary16
+# pairs): 4 adds + E, H, F each one v_pk_maximum3_f16 (local: clamps at 0 as third operand)
+# + the running best over two columns per three-input max (local) = 7.5; global/semiglobal
+# 4 adds + 2 two-input maxima + 1 three-input = 7.  int16x2: local 4 add/sub + 5 max = 9
+# (E, F and H-gap kept >= 0 by saturating subtraction, so max(H, 0) costs nothing), else 8.
+# int32: one alignment per lane, twice the slots.
+SLOTS_PER_CELL = {("half2", True): 7.5, ("half2", False): 7.0, ("int16x2", True): 9.0,
+                  ("int16x2", False): 8.0, ("int32", True): 18.0, ("int32", False): 16.0}
+# add/sub/max operations of the recurrence per cell (local: clamp folded), and the survey's
+# count, which prices max(H, 0) as an operation of its own
+RECURRENCE_OPS = {True: 9, False: 8}
+SURVEY_OPS = {True: 10, False: 8}
+ROOF_UNIT = "T VALU lane-slots/s (packed 16-bit)"
+# arithmetic type of a tile kind, and what the binary16 one means for the scores
+DTYPE_OF = {"half2": "f16 (exact integers)", "int16x2": "int16", "int32": "int32"}
+DTYPE_NOTE = ("half2 tiles carry two alignments per lane as binary16 values, which hold every integer "
+              "in [-2048, 2048] exactly; a local result of 2048 or more is rescored in int32, and "
+              "global/semiglobal pairs run in half2 only where every DP value stays within 2048, so "
+              "every score is the exact int32 (checked against the oracle in 'checks')")
+
+
+def _valu_roof(cells: float, seconds: float, elem: str, local: bool) -> dict:
+    """VALU roofline fields of `cells` cell updates in `seconds` on tiles of kind `elem`."""
+    slots = SLOTS_PER_CELL[(elem, local)]
+    achieved = cells * slots / seconds / 1e12
+    return {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_SLOTS, "unit": ROOF_UNIT,
+            "frac": round(achieved / PEAK_SLOTS, 4), "elem": elem, "slots_per_cell": slots,
+            "recurrence_ops_per_cell": RECURRENCE_OPS[local], "survey_ops_per_cell": SURVEY_OPS[local],
+            "recurrence_ops_rate": round(cells * RECURRENCE_OPS[local] / seconds / 1e12, 3)}
+
+
+def _dominant_kind(history) -> tuple[str, bool]:
+    """Element kind and local mode of the most frequent dominant variant of the calls."""
+    from gpusw_amd import _hip
+
+    ids = [h.variant for h in history if h.variant >= 0]
+    if not ids:
+        return "int16x2", True
+    _g, _r, elem, local = _hip.variant_info(max(set(ids), key=ids.count))
+    return elem, local
 
 
 def log(msg: str) -> None:
@@ -175,10 +213,9 @@ def _measured_traffic(workload: str, kernel: str | None):
 def _roofline(history, mode: str, workload: str) -> dict:
     """Roofline of the dominant kernel: the batched launch with the most cells in each timed
     call (its cells x ops over its device time from HIP events on the stream it ran on)."""
-    ops, survey = OPS_PER_CELL[mode], SURVEY_OPS_PER_CELL[mode]
     top = [h for h in history if h.top_variant >= 0 and h.top_ms > 0]
     if not top:
-        return {"bound": "valu", "achieved": None, "peak": PEAK_INT16_TOPS, "unit": "TOP/s (int16)",
+        return {"bound": "valu", "achieved": None, "peak": PEAK_SLOTS, "unit": ROOF_UNIT,
                 "frac": None, "traffic": None}
     # the most frequent (variant, striped) among the calls' dominant launches (most cells)
     keys = [(h.top_variant, h.top_striped) for h in top]
@@ -186,14 +223,14 @@ def _roofline(history, mode: str, workload: str) -> dict:
     sel = [h for h in top if (h.top_variant, h.top_striped) == (var, striped)]
     ms = float(np.mean([h.top_ms for h in sel]))
     cells = float(np.mean([h.top_cells for h in sel]))
-    achieved = cells * ops / (ms * 1e-3) / 1e12
+    from gpusw_amd import _hip
+
+    _g, _r, elem, local = _hip.variant_info(var)
     kernel = _kernel_name(var, striped)
     traffic, source = _measured_traffic(workload, kernel)
-    out = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_INT16_TOPS, "unit": "TOP/s (int16)",
-           "frac": round(achieved / PEAK_INT16_TOPS, 4), "traffic": traffic,
-           "kernel": kernel, "tile": _tile_name(var), "launch_ms": round(ms, 3), "launch_cells": int(cells),
-           "ops_per_cell": ops, "survey_ops_per_cell": survey,
-           "survey_frac": round(achieved * survey / ops / PEAK_INT16_TOPS, 4), "launches_timed": len(sel)}
+    out = _valu_roof(cells, ms * 1e-3, elem, local)
+    out.update({"traffic": traffic, "kernel": kernel, "tile": _tile_name(var), "launch_ms": round(ms, 3),
+                "launch_cells": int(cells), "launches_timed": len(sel)})
     if traffic is not None:
         out["traffic_unit"] = "bytes/launch (rocprofv3 FETCH_SIZE, WRITE_SIZE, copy-calibrated)"
         out["traffic_source"] = source
@@ -222,18 +259,13 @@ def _call_roofline(history, cells: int, kernel_s: float, steps: int, workload: s
     single launch owns the GPU for its interval: the dominant unit is the call's set of
     batched launches, timed from the first launch's start to the last one's end (HIP events
     on the library stream, joined with the second stream).  cells and kernel_s are per step."""
-    ops, survey = OPS_PER_CELL["local"], SURVEY_OPS_PER_CELL["local"]
-    achieved = cells * ops / kernel_s / 1e12
+    elem, local = _dominant_kind(history)
     calls = max(1, len(history) // max(steps, 1))
     traffic, src = _call_traffic(workload)
-    out = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_INT16_TOPS, "unit": "TOP/s (int16)",
-           "frac": round(achieved / PEAK_INT16_TOPS, 4),
-           "traffic": None if traffic is None else round(traffic * calls, 1),
-           "kernel": label,
-           "ops_per_cell": ops, "survey_ops_per_cell": survey,
-           "survey_frac": round(achieved * survey / ops / PEAK_INT16_TOPS, 4),
-           "kernels_ms_per_step": round(kernel_s * 1e3, 2), "calls_per_step": calls,
-           "largest_launch": _roofline(history, "local", workload)}
+    out = _valu_roof(cells, kernel_s, elem, local)
+    out.update({"traffic": None if traffic is None else round(traffic * calls, 1), "kernel": label,
+                "kernels_ms_per_step": round(kernel_s * 1e3, 2), "calls_per_step": calls,
+                "largest_launch": _roofline(history, "local", workload)})
     if traffic is not None:
         out["traffic_unit"] = "bytes per step (sum of the per-dispatch rocprofv3 rows of every batched kernel)"
         out["traffic_source"] = src
@@ -339,7 +371,8 @@ def run_dna_cross(args):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "int16",
+        "dtype": DTYPE_OF[roof["elem"]],
+        "dtype_note": DTYPE_NOTE,
         "data": "synthetic",
         "config": {
             "workload": "16S-shape DNA cross: 2000 queries (240-260 nt) x 20000 references "
@@ -468,13 +501,15 @@ def protein_leg(args, steps: int | None = None, warmup: int | None = None):
     hrj = res.topk_idx[hq].reshape(-1).astype(np.int32)
     want = native.score_encoded_pairs(al._queries, al._refs, hqi, hrj, scheme)
     hit_mism = int((res.topk_scores[hq].reshape(-1).astype(np.int64) != want).sum())
+    roof = _call_roofline(history, cells, kernel_s, steps)
     out = {
         "value": round(value, 2), "unit": "GCUPS", "ms_per_step": round(wall * 1e3, 2), "steps": steps,
-        "warmup": warmup, "dtype": "int16",
+        "dtype": DTYPE_OF[roof["elem"]], "dtype_note": DTYPE_NOTE, "rescored": int(history[-1].rescored),
+        "warmup": warmup,
         "config": {"workload": "protein top-k: 1000 queries (lognormal median 350, 30-2000) x 100000 "
                                "references (lognormal median 300, 30-3000), BLOSUM62 local -11/-1, top_k(5)",
                    "cells": cells, "query_batch": qb or "Aligner default (2 GiB score block)"},
-        "roofline": _call_roofline(history, cells, kernel_s, steps),
+        "roofline": roof,
         "plain": {"gcups": round(plain_gcups, 2), "queries": int(len(qsel)), "store_dtype": dtype,
                   "speedup": round(value / plain_gcups, 2)},
         "checks": {"topk_equals_comparator_rows": ok, "oracle_hits_checked": int(hqi.size),
@@ -581,8 +616,8 @@ def run_protein_topk(args):
     prot, _pool = protein_leg(args, steps=args.steps, warmup=args.warmup)
     line = {"metric": METRIC, "value": prot["value"], "unit": "GCUPS", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": prot["ms_per_step"], "higher_is_better": True,
-            "scaling": "weak", "vs_baseline": None, "dtype": "int16", "data": "synthetic"}
-    line.update({k: prot[k] for k in ("config", "roofline", "plain", "checks")})
+            "scaling": "weak", "vs_baseline": None, "data": "synthetic"}
+    line.update({k: prot[k] for k in ("dtype", "dtype_note", "config", "roofline", "plain", "checks")})
     return line
 
 
@@ -624,15 +659,16 @@ def run_read_pairs(args):
                                   al._d_rbuf, al._d_roff, qi[sub], rj[sub], sub.size)
         pcells = int((al._queries.lengths[qi[sub]].astype(np.int64) * al._refs.lengths[rj[sub]]).sum())
         value = cells / wall / 1e9
+        roof = dict(_roofline(history, mode, "readpairs"), all_batches_ms=round(st.kernel_ms, 2),
+                    host_wall_ms=round(st.wall_ms, 2), launches=int(st.launches))
         out.append({
             "metric": METRIC, "value": round(value, 2), "unit": "GCUPS", "n_gpus": 1, "steps": args.steps,
             "warmup": warmup, "ms_per_step": round(wall * 1e3, 2), "higher_is_better": True,
-            "scaling": "weak", "vs_baseline": None, "dtype": "int16", "data": "synthetic",
+            "scaling": "weak", "vs_baseline": None, "dtype": DTYPE_OF.get(roof.get("elem"), "int16"),
+            "dtype_note": DTYPE_NOTE, "data": "synthetic",
             "config": {"workload": f"read pairs: 1,000,000 (150-nt read, 200-400-nt window) pairs, dna "
-                                   f"{mode}, int16, HIP-graph batches of 131,072", "cells": cells},
-            "roofline": dict(_roofline(history, mode, "readpairs"),
-                             all_batches_ms=round(st.kernel_ms, 2), host_wall_ms=round(st.wall_ms, 2),
-                             launches=int(st.launches)),
+                                   f"{mode}, int16 envelope, HIP-graph batches of 131,072", "cells": cells},
+            "roofline": roof,
             "plain": {"gcups": round(pcells / pdt / 1e9, 2), "pairs": int(sub.size),
                       "speedup": round(value / (pcells / pdt / 1e9), 2),
                       "equal_to_batched": bool(np.array_equal(ph, got[sub]))},
@@ -662,7 +698,8 @@ def run_serving(args):
             lines.append({
                 "metric": METRIC, "value": s["gcups"], "unit": "GCUPS", "n_gpus": 1, "steps": 1, "warmup": 0,
                 "ms_per_step": round(wall * 1e3, 2), "higher_is_better": True, "scaling": "weak",
-                "vs_baseline": None, "dtype": "int16", "data": "synthetic",
+                "vs_baseline": None, "dtype": "f16 (exact integers) / int16", "dtype_note": DTYPE_NOTE,
+                "data": "synthetic",
                 "config": {"workload": "serving mix: Poisson arrivals, 1-512 queries per request (log-uniform), "
                                        "70% DNA vs 16S-shape DB (20,000 refs), 30% protein vs 100,000-ref DB, top_k(5)",
                            "offered_rate_rps": rate, "requests": args.requests,
