@@ -36,11 +36,14 @@ def _policy(scheme) -> int:
 
     ``"auto"`` applies the reference's int16 envelope (``src/gpusw/scheme.py:116-141``)
     to every pair with the pair's own lengths instead of the call's longest ones: a pair
-    runs on the packed int16 tiles only where no DP value can saturate, otherwise on the
+    runs on packed 16-bit tiles only where no DP value can saturate int16, otherwise on the
     int32 tiles.  Every pair the call-wide envelope sends to int32 that could saturate is
-    therefore on the int32 tiles, as in ``backends/cuda.py:68-85``.  ``"int16"`` reaches
-    the library only after ``resolve_dtype`` accepted it; local results that reach 32767
-    would be rescored on the int32 tiles.  ``"int32"``: 32-bit tiles throughout.
+    therefore on the int32 tiles, as in ``backends/cuda.py:68-85``.  The packed tiles are
+    binary16 pairs (exact for integers up to 2048; local results from 2048 up are rescored
+    in int32) or, for global/semiglobal pairs past that range, int16.  ``"int16"`` reaches
+    the library only after ``resolve_dtype`` accepted it; local results at the top of the
+    packed type would be rescored on the int32 tiles.  ``"int32"``: 32-bit tiles
+    throughout.
     """
     if scheme.dtype == "int32":
         return _hip.DTYPE_INT32
