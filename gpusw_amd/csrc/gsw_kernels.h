@@ -41,7 +41,7 @@ __attribute__((amdgpu_waves_per_eu(WavesPerEU<LOCAL, G, R>::value))) cross_kerne
     extern __shared__ __align__(16) char lds[];
     __shared__ int s_item, s_blk;
     const int lane = threadIdx.x & 63;
-    const int t = lane % G, g = lane / G;
+    const int t = GroupLanes<G>::t_of(lane), g = GroupLanes<G>::g_of(lane);
     const CrossItem *items = static_cast<const CrossItem *>(a.items);
     for (;;) {
         if (threadIdx.x == 0) {
@@ -114,7 +114,7 @@ __attribute__((amdgpu_waves_per_eu(WavesPerEU<LOCAL, G, R>::value))) stripe_kern
     extern __shared__ __align__(16) char lds[];
     __shared__ int s_item, s_blk;
     const int lane = threadIdx.x & 63;
-    const int t = lane % G, g = lane / G;
+    const int t = GroupLanes<G>::t_of(lane), g = GroupLanes<G>::g_of(lane);
     const CrossItem *items = static_cast<const CrossItem *>(a.items);
     uint32_t *bnd_wg = a.bnd + (size_t)blockIdx.x * 2 * (size_t)a.bnd_stride;
     for (;;) {
@@ -199,7 +199,7 @@ __global__ void __launch_bounds__(WAVES * 64) pairs_kernel(KArgs a) {
     constexpr int GPW = 64 / G;
     extern __shared__ __align__(16) char lds[];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int t = lane % G, g = lane / G;
+    const int t = GroupLanes<G>::t_of(lane), g = GroupLanes<G>::g_of(lane);
     const int nrows = a.prow + 1;
     // blocks of NSLOT groups interleave their private profiles (see ProfLayout)
     const int block_bytes = nrows * L::ROWB;
@@ -279,7 +279,7 @@ __global__ void __launch_bounds__(WAVES * 64) pairs_tab_kernel(KArgs a) {
     constexpr int GPW = 64 / G;
     extern __shared__ __align__(16) char lds[];
     const int lane = threadIdx.x & 63;
-    const int t = lane % G, g = lane / G;
+    const int t = GroupLanes<G>::t_of(lane), g = GroupLanes<G>::g_of(lane);
     const int ns = a.ns, nq = ns + 2, npad = ns, zpad = ns + 1;
     const int row_words = O::PACKED ? nq * nq : nq;
     const int copies = a.tab_copies;

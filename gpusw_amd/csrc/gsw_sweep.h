@@ -48,6 +48,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 namespace gsw {
 
@@ -142,10 +143,40 @@ __device__ __forceinline__ int wave_max_i32(int v) {
     return v;
 }
 
+__device__ __forceinline__ int wave_min_i32(int v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// Wave lanes of a lane group.  The boundary H and E of a group's lane t come from its lane
+// t-1 through a DPP shift.  G = 16: a group is one 16-lane DPP row, and row_shr:1 with
+// bound_ctrl zero-fills each group's lane 0.  G = 8: the two groups of a DPP row
+// interleave (lane 16*(g/2) + 2t + g%2), so row_shr:2 moves each value to the next lane of
+// its own group and zero-fills both groups' lane 0.  Wider groups span DPP rows
+// (wave_shr:1, lane 0 of a group fixed up by a select).  The zero fill is the local-mode
+// boundary (H = E = 0 at column 0), so those tiles need no select.
+template <int G> struct GroupLanes {
+    static constexpr bool INTERLEAVED = G == 8;
+    static constexpr bool ZERO_FILL = G == 8 || G == 16;   // the shift gives lane 0 zeros
+    static constexpr int STRIDE = INTERLEAVED ? 2 : 1;     // wave-lane distance of t, t+1
+    __device__ static __forceinline__ int t_of(int lane) { return INTERLEAVED ? (lane & 15) >> 1 : lane % G; }
+    __device__ static __forceinline__ int g_of(int lane) {
+        return INTERLEAVED ? ((lane >> 4) << 1) | (lane & 1) : lane / G;
+    }
+    // value of the group's lane t-1 (0 for t = 0 when ZERO_FILL)
+    __device__ static __forceinline__ int shift(int v) {
+        if constexpr (INTERLEAVED) return __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, true);   // row_shr:2
+        else if constexpr (G == 16) return __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, true);  // row_shr:1
+        else return wave_shr1(v);
+    }
+};
+
 template <int G>
 __device__ __forceinline__ int group_max_i32(int v) {
+    constexpr int S = GroupLanes<G>::STRIDE;
 #pragma unroll
-    for (int off = G / 2; off >= 1; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+    for (int off = G / 2; off >= 1; off >>= 1) v = max(v, __shfl_xor(v, off * S, 64));
     return v;
 }
 
@@ -313,13 +344,68 @@ __device__ __forceinline__ SweepOut sweep_group(const uint8_t *__restrict__ rseq
     // lane 0's boundary words of its next row (stale when there is none: lane 0 is idle then)
     uint2 nb{0u, 0u};
 
-    for (int s = 0; s < nsteps; ++s) {
+    auto finish = [&]() -> SweepOut {
+        SweepOut out;
+        if (LOCAL) {
+            int ba = O::lo(best), bb = O::hi(best);
+            ba = group_max_i32<G>(ba);
+            bb = O::PACKED ? group_max_i32<G>(bb) : ba;
+            out.a = ba;
+            out.b = bb;
+            return out;
+        }
+        // non-local.  A lane stops updating after its last row, so hup[] now holds
+        // H(lr, j) of the lane's columns: the corner and the last-row maximum over the
+        // real columns are read once here.  (The column tests would otherwise be hoisted
+        // above the step loop, where their masks spill scalar registers for the sweep.)
+        asm volatile("" : "+v"(lqa), "+v"(lqb), "+v"(cbase));
+        if (lr > 0) {
+#pragma unroll
+            for (int kk = 0; kk < R; ++kk) {
+                const int col = cbase + t * R + kk;
+                const int va = O::lo(hup[kk]);
+                const int vb = O::hi(hup[kk]);
+                if (col == lqa - 1) corner_a = va;
+                if (col == lqb - 1) corner_b = vb;
+                if (col < lqa) endrow_a = max(endrow_a, va);
+                if (col < lqb) endrow_b = max(endrow_b, vb);
+            }
+        }
+        // each lane offers what it holds; the group maximum is the result
+        int ca = corner_a, cb = corner_b;
+        if (semi) {
+            if (track_endcol) {
+                // the lane whose last register is column lq-1 (none in other stripes)
+                if (cbase + t * R + R - 1 == lqa - 1) ca = max(ca, O::lo(endcol));
+                if (cbase + t * R + R - 1 == lqb - 1) cb = max(cb, O::hi(endcol));
+            }
+            if (k.free_r) {
+                ca = max(ca, endrow_a);
+                cb = max(cb, endrow_b);
+            }
+        }
+        ca = group_max_i32<G>(ca);
+        cb = O::PACKED ? group_max_i32<G>(cb) : ca;
+        if (semi && k.free_q && k.free_r) {
+            ca = max(ca, 0);
+            cb = max(cb, 0);
+        }
+        out.a = ca;
+        out.b = cb;
+        return out;
+    };
+    // One step: lane t on row s - t.  GUARD: rows outside [0, lr) are skipped (the pipeline
+    // fill and drain); without it every lane of the wave is on a real row, or in an idle
+    // group (lr = 0) whose results are never stored and whose code reads stay inside the
+    // code buffer (row < the shortest active reference of the wave).
+    auto step = [&](int s, auto guard) {
+        constexpr bool GUARD = decltype(guard)::value;
         const int row = s - t;
         if (STRIPE && bnd_in && t == 0 && row + 1 < lr)   // lane 0's next row
             nb = uint2{*bnd(bh, boff + (uint32_t)(row + 1)), *bnd(be, boff + (uint32_t)(row + 1))};
         int c_nxt = 0;
         if constexpr (PF) c_nxt = (int)rseq[min(max(row + 1, 0), rlast)];
-        if ((unsigned)row < (unsigned)lr) {
+        if (!GUARD || (unsigned)row < (unsigned)lr) {
             V pv[KC * 4];
             if constexpr (PF) load_row(min(c_raw, k.ns - 1), pv);
             else load_row(code_at(row), pv);
@@ -371,66 +457,44 @@ __device__ __forceinline__ SweepOut sweep_group(const uint8_t *__restrict__ rseq
             }
         }
         din = hin;
-        hin = O::from_bits(wave_shr1(O::bits(hout)));
-        ein = O::from_bits(wave_shr1(O::bits(eout)));
+        hin = O::from_bits(GroupLanes<G>::shift(O::bits(hout)));
+        ein = O::from_bits(GroupLanes<G>::shift(O::bits(eout)));
         if (t == 0) {
             if (STRIPE && bnd_in) {
                 hin = O::from_bits((int)nb.x);
                 ein = O::from_bits((int)nb.y);
-            } else {
+            } else if (!(LOCAL && GroupLanes<G>::ZERO_FILL)) {   // zero fill: local boundary already
                 hin = c0v;
                 ein = LOCAL ? ZERO : O::splat(O::NEG);
             }
         }
         if (ladder) c0v = O::add(c0v, GEV);
+    };
+    // Steps [G-1, s_hi) have every active lane of the wave on a real row (s_hi: the
+    // shortest active reference of the wave): they run unguarded, two per iteration.  Not
+    // in stripes, whose boundary registers leave too few VGPRs for the duplicated step: the
+    // scheduler then serialises the E -> H chain with a wait state per column.
+    if constexpr (STRIPE) {
+        for (int s = 0; s < nsteps; ++s) step(s, std::true_type{});
+        return finish();
+    }
+    const int s_lo = min(G - 1, nsteps);
+    const int lrmin = __builtin_amdgcn_readfirstlane(wave_min_i32(lr > 0 ? lr : 0x7fffffff));
+    const int s_hi = max(s_lo, min(lrmin, nsteps));
+    for (int s = 0; s < nsteps;) {
+        if (s == s_lo) {
+            for (; s + 2 <= s_hi; s += 2) {
+                step(s, std::false_type{});
+                step(s + 1, std::false_type{});
+            }
+            if (s < s_hi) step(s++, std::false_type{});
+            if (s >= nsteps) break;
+        }
+        step(s, std::true_type{});
+        ++s;
     }
 
-    SweepOut out;
-    if (LOCAL) {
-        int ba = O::lo(best), bb = O::hi(best);
-        ba = group_max_i32<G>(ba);
-        bb = O::PACKED ? group_max_i32<G>(bb) : ba;
-        out.a = ba;
-        out.b = bb;
-        return out;
-    }
-    // non-local.  A lane stops updating after its last row, so hup[] now holds
-    // H(lr, j) of the lane's columns: the corner and the last-row maximum over the
-    // real columns are read once here.
-    if (lr > 0) {
-#pragma unroll
-        for (int kk = 0; kk < R; ++kk) {
-            const int col = cbase + t * R + kk;
-            const int va = O::lo(hup[kk]);
-            const int vb = O::hi(hup[kk]);
-            if (col == lqa - 1) corner_a = va;
-            if (col == lqb - 1) corner_b = vb;
-            if (col < lqa) endrow_a = max(endrow_a, va);
-            if (col < lqb) endrow_b = max(endrow_b, vb);
-        }
-    }
-    // each lane offers what it holds; the group maximum is the result
-    int ca = corner_a, cb = corner_b;
-    if (semi) {
-        if (track_endcol) {
-            // the lane whose last register is column lq-1 (none in other stripes)
-            if (cbase + t * R + R - 1 == lqa - 1) ca = max(ca, O::lo(endcol));
-            if (cbase + t * R + R - 1 == lqb - 1) cb = max(cb, O::hi(endcol));
-        }
-        if (k.free_r) {
-            ca = max(ca, endrow_a);
-            cb = max(cb, endrow_b);
-        }
-    }
-    ca = group_max_i32<G>(ca);
-    cb = O::PACKED ? group_max_i32<G>(cb) : ca;
-    if (semi && k.free_q && k.free_r) {
-        ca = max(ca, 0);
-        cb = max(cb, 0);
-    }
-    out.a = ca;
-    out.b = cb;
-    return out;
+    return finish();
 }
 
 // The sweep of sweep_group for explicit pairs with private profiles, pipelined for two
@@ -532,9 +596,9 @@ __device__ __forceinline__ SweepOut sweep_pairs_pipe(const uint8_t *__restrict__
             if (!LOCAL) endcol = O::mx(endcol, h);
         }
         din = hin;
-        hin = O::from_bits(wave_shr1(O::bits(hout)));
-        ein = O::from_bits(wave_shr1(O::bits(eout)));
-        if (t == 0) {
+        hin = O::from_bits(GroupLanes<G>::shift(O::bits(hout)));
+        ein = O::from_bits(GroupLanes<G>::shift(O::bits(eout)));
+        if (!(LOCAL && GroupLanes<G>::ZERO_FILL) && t == 0) {   // zero fill: local boundary already
             hin = c0v;
             ein = LOCAL ? ZERO : O::splat(O::NEG);
         }
