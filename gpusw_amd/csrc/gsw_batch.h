@@ -69,8 +69,9 @@ struct Variant {
     const void *pairs_tab_fn;                       // pairs with the substitution table in LDS
     int kind;                                       // KIND_*
 };
-// The half2 rows of the variant table (gsw_sweep_f16.hip).
+// The half2 and int32 rows of the variant table (gsw_sweep_f16.hip, gsw_sweep_i32.hip).
 std::vector<Variant> half_variants();
+std::vector<Variant> int32_variants();
 // LDS of the pairs table kernel: ns rows of (ns+2)^2 packed words (int32 tiles: ns+2),
 // times pairs_tab_copies interleaved copies (32 when that stays within 48 KiB, else 1).
 int pairs_tab_copies(const Variant &v, int ns);

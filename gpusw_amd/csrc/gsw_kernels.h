@@ -353,10 +353,8 @@ static void launch_stripe(const KArgs &a, int grid, size_t lds, hipStream_t st) 
             reinterpret_cast<const void *>(&pairs_kernel<V, G, R, L, PAIRS_WAVES>), nullptr,      \
             &launch_cross<V, G, R, L>, &launch_pairs<V, G, R, L>, nullptr,                        \
             reinterpret_cast<const void *>(&pairs_tab_kernel<V, G, R, L, PAIRS_TAB_WAVES>), KIND}
-#define GSW_VARIANT4(G, R)                                                                        \
-    GSW_VARIANT(s2, KIND_I16, G, R, true), GSW_VARIANT(s2, KIND_I16, G, R, false),                \
-        GSW_VARIANT(int, KIND_I32, G, R, true), GSW_VARIANT(int, KIND_I32, G, R, false)
-#define GSW_HVARIANT2(G, R) GSW_VARIANT(h2, KIND_F16, G, R, true), GSW_VARIANT(h2, KIND_F16, G, R, false)
+// local and global/semiglobal rows of a tile for one element type
+#define GSW_VARIANT2(V, KIND, G, R) GSW_VARIANT(V, KIND, G, R, true), GSW_VARIANT(V, KIND, G, R, false)
 // tiles that also run stripes of longer queries
 #define GSW_SVARIANT(V, KIND, G, R, L)                                                            \
     Variant{0, G, R, KIND != KIND_I32, L,                                                         \
@@ -365,9 +363,19 @@ static void launch_stripe(const KArgs &a, int grid, size_t lds, hipStream_t st) 
             reinterpret_cast<const void *>(&stripe_kernel<V, G, R, L, CROSS_WAVES>),              \
             &launch_cross<V, G, R, L>, &launch_pairs<V, G, R, L>, &launch_stripe<V, G, R, L>,     \
             reinterpret_cast<const void *>(&pairs_tab_kernel<V, G, R, L, PAIRS_TAB_WAVES>), KIND}
-#define GSW_SVARIANT4(G, R)                                                                       \
-    GSW_SVARIANT(s2, KIND_I16, G, R, true), GSW_SVARIANT(s2, KIND_I16, G, R, false),              \
-        GSW_SVARIANT(int, KIND_I32, G, R, true), GSW_SVARIANT(int, KIND_I32, G, R, false)
-#define GSW_HSVARIANT2(G, R) GSW_SVARIANT(h2, KIND_F16, G, R, true), GSW_SVARIANT(h2, KIND_F16, G, R, false)
+#define GSW_SVARIANT2(V, KIND, G, R) GSW_SVARIANT(V, KIND, G, R, true), GSW_SVARIANT(V, KIND, G, R, false)
+
+// The rows of one element type over every compiled tile, in the table's tile order (each
+// element type's instantiations compile in a translation unit of their own).
+#define GSW_TILE_ROWS(V, KIND)                                                                    \
+    GSW_VARIANT2(V, KIND, 8, 8), GSW_VARIANT2(V, KIND, 8, 12), GSW_VARIANT2(V, KIND, 8, 16),      \
+        GSW_VARIANT2(V, KIND, 8, 19), GSW_SVARIANT2(V, KIND, 8, 20), GSW_SVARIANT2(V, KIND, 8, 24), \
+        GSW_VARIANT2(V, KIND, 8, 30), GSW_VARIANT2(V, KIND, 8, 31), GSW_SVARIANT2(V, KIND, 8, 32), \
+        GSW_VARIANT2(V, KIND, 8, 33), GSW_VARIANT2(V, KIND, 8, 34), GSW_VARIANT2(V, KIND, 8, 36),  \
+        GSW_VARIANT2(V, KIND, 8, 40), GSW_VARIANT2(V, KIND, 8, 44), GSW_VARIANT2(V, KIND, 8, 48),  \
+        GSW_VARIANT2(V, KIND, 16, 24), GSW_VARIANT2(V, KIND, 16, 28), GSW_VARIANT2(V, KIND, 16, 32), \
+        GSW_VARIANT2(V, KIND, 16, 36), GSW_VARIANT2(V, KIND, 16, 40), GSW_VARIANT2(V, KIND, 16, 44), \
+        GSW_VARIANT2(V, KIND, 16, 48), GSW_VARIANT2(V, KIND, 32, 32), GSW_VARIANT2(V, KIND, 32, 48), \
+        GSW_VARIANT2(V, KIND, 64, 32)
 
 }  // namespace gsw

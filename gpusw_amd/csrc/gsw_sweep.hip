@@ -35,24 +35,16 @@ __global__ void scatter_kernel(int32_t *out, const int64_t *idx, const int32_t *
 
 std::vector<Variant> &variant_table() {
     static std::vector<Variant> table = [] {
-        std::vector<Variant> v = {
-            GSW_VARIANT4(8, 8),   GSW_VARIANT4(8, 12),  GSW_VARIANT4(8, 16),  GSW_VARIANT4(8, 19),  GSW_SVARIANT4(8, 20),
-            GSW_SVARIANT4(8, 24), GSW_VARIANT4(8, 30),  GSW_VARIANT4(8, 31),  GSW_SVARIANT4(8, 32), GSW_VARIANT4(8, 33),
-            GSW_VARIANT4(8, 34),
-            GSW_VARIANT4(8, 36),
-            GSW_VARIANT4(8, 40),  GSW_VARIANT4(8, 44),  GSW_VARIANT4(8, 48),  GSW_VARIANT4(16, 24),
-            GSW_VARIANT4(16, 28), GSW_VARIANT4(16, 32), GSW_VARIANT4(16, 36), GSW_VARIANT4(16, 40),
-            GSW_VARIANT4(16, 44), GSW_VARIANT4(16, 48), GSW_VARIANT4(32, 32), GSW_VARIANT4(32, 48),
-            GSW_VARIANT4(64, 32),
-        };
-        // each tile's half2 rows follow its int16x2/int32 rows
-        const std::vector<Variant> h = half_variants();
+        // int16x2 rows here; each tile's int32 and half2 rows follow its int16x2 rows
+        const std::vector<Variant> v = {GSW_TILE_ROWS(s2, KIND_I16)};
+        const std::vector<Variant> w = int32_variants(), h = half_variants();
         std::vector<Variant> all;
         for (size_t i = 0; i < v.size(); ++i) {
             all.push_back(v[i]);
             if (i + 1 == v.size() || v[i + 1].G != v[i].G || v[i + 1].R != v[i].R)
-                for (const Variant &x : h)
-                    if (x.G == v[i].G && x.R == v[i].R) all.push_back(x);
+                for (const std::vector<Variant> *o : {&w, &h})
+                    for (const Variant &x : *o)
+                        if (x.G == v[i].G && x.R == v[i].R) all.push_back(x);
         }
         for (size_t i = 0; i < all.size(); ++i) all[i].id = (int)i;
         return all;
