@@ -170,6 +170,13 @@ template <int G> struct GroupLanes {
         else if constexpr (G == 16) return __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, true);  // row_shr:1
         else return wave_shr1(v);
     }
+    // ZERO_FILL tiles: value of lane t-1, lane 0 keeping `old` (no bound_ctrl: a lane whose
+    // source is outside its DPP row is not written)
+    __device__ static __forceinline__ int shift_keep(int old, int v) {
+        static_assert(ZERO_FILL, "row shifts only");
+        if constexpr (INTERLEAVED) return __builtin_amdgcn_update_dpp(old, v, 0x112, 0xf, 0xf, false);
+        else return __builtin_amdgcn_update_dpp(old, v, 0x111, 0xf, 0xf, false);
+    }
 };
 
 template <int G>
@@ -264,9 +271,9 @@ struct SweepOut {
 // row is read one step before lane 0 needs it and written G-1 steps after, so one buffer
 // serves both.  The result then covers the stripe's columns only (the caller combines
 // stripes with max).
-// PF: for low occupancy (one or two waves per SIMD) the reference code of the next row is
-// loaded one step ahead, from a clamped index so the load needs no branch and nothing
-// waits for it until the next step's profile address.
+// PF: the reference code of a row is loaded two steps ahead, from a clamped index so the
+// load needs no branch and nothing waits for it until that step's table or profile
+// address (a global load takes longer than one step of a narrow tile).
 // TAB: no profile; prof_lane is a substitution table of trow bytes per reference code
 // and toff[kk] the byte offset of column kk's (query A code, query B code) entry in a row,
 // so each column costs one LDS load (see the pairs table kernel).
@@ -339,8 +346,30 @@ __device__ __forceinline__ SweepOut sweep_group(const uint8_t *__restrict__ rseq
     // column-0 boundary H(i, 0) of lane 0's next row, kept as a running (saturating) value
     const bool ladder = !LOCAL && (k.mode == 1 || (k.mode == 2 && !k.free_q));
     V c0v = ladder ? O::splat(col0_value(k, 2)) : ZERO;
+    // Row shifts of ZERO_FILL tiles give every lane t > 0 lane t-1's H and E of the row.
+    // Lane 0 takes the column-0 boundary: local H = E = 0 (the zero fill itself); otherwise
+    // E stays the sentinel it holds and H steps down the column-0 ladder by gap_extend (or
+    // stays 0 with free query ends), so each is the kept old value of one DPP move.
+    const V LADV = ladder ? GEV : ZERO;
+    auto shift_boundary = [&](V &hi, V &ei, V ho, V eo) {
+        if constexpr (!GroupLanes<G>::ZERO_FILL) {
+            (void)hi, (void)ei, (void)ho, (void)eo;
+        } else if constexpr (LOCAL) {
+            hi = O::from_bits(GroupLanes<G>::shift(O::bits(ho)));
+            ei = O::from_bits(GroupLanes<G>::shift(O::bits(eo)));
+        } else {
+            hi = O::from_bits(GroupLanes<G>::shift_keep(O::bits(O::add(hi, LADV)), O::bits(ho)));
+            ei = O::from_bits(GroupLanes<G>::shift_keep(O::bits(ei), O::bits(eo)));
+        }
+    };
     const int rlast = max(lr - 1, 0);   // idle groups (lr = 0) read their row 0 harmlessly
-    int c_raw = PF ? (int)rseq[min(max(-t, 0), rlast)] : 0;
+    // PF codes: stripes hold the code of the current row (c_raw) and load the next one each
+    // step; other sweeps keep one register per step parity (cA even steps, cB odd ones),
+    // each reloaded with the code two rows on once its step has used it, so the load stays
+    // in flight through the next step with no register copy to wait for it
+    int c_raw = STRIPE ? (int)rseq[min(max(-t, 0), rlast)] : 0;
+    int cA = PF && !STRIPE ? (int)rseq[min(max(-t, 0), rlast)] : 0;
+    int cB = PF && !STRIPE ? (int)rseq[min(max(1 - t, 0), rlast)] : 0;
     // lane 0's boundary words of its next row (stale when there is none: lane 0 is idle then)
     uint2 nb{0u, 0u};
 
@@ -398,17 +427,19 @@ __device__ __forceinline__ SweepOut sweep_group(const uint8_t *__restrict__ rseq
     // fill and drain); without it every lane of the wave is on a real row, or in an idle
     // group (lr = 0) whose results are never stored and whose code reads stay inside the
     // code buffer (row < the shortest active reference of the wave).
-    auto step = [&](int s, auto guard) {
+    auto step = [&](int s, auto guard, int &c) {
         constexpr bool GUARD = decltype(guard)::value;
         const int row = s - t;
         if (STRIPE && bnd_in && t == 0 && row + 1 < lr)   // lane 0's next row
             nb = uint2{*bnd(bh, boff + (uint32_t)(row + 1)), *bnd(be, boff + (uint32_t)(row + 1))};
         int c_nxt = 0;
-        if constexpr (PF) c_nxt = (int)rseq[min(max(row + 1, 0), rlast)];
+        if constexpr (STRIPE) c_nxt = (int)rseq[min(max(row + 1, 0), rlast)];
         if (!GUARD || (unsigned)row < (unsigned)lr) {
             V pv[KC * 4];
-            if constexpr (PF) load_row(min(c_raw, k.ns - 1), pv);
+            if constexpr (STRIPE) load_row(min(c_raw, k.ns - 1), pv);
+            else if constexpr (PF) load_row(min(c, k.ns - 1), pv);
             else load_row(code_at(row), pv);
+            if constexpr (PF && !STRIPE) c = (int)rseq[min(max(row + 2, 0), rlast)];
             V e = ein;
             V hgo = row_hgo<V, LOCAL>(hin, GOV, GOP);
             V h = hin, hprev = ZERO;
@@ -440,7 +471,10 @@ __device__ __forceinline__ SweepOut sweep_group(const uint8_t *__restrict__ rseq
             eout = e;
             if (!LOCAL && track_endcol) endcol = O::mx(endcol, h);   // lane's last column
         }
-        if constexpr (PF) c_raw = c_nxt;
+        if constexpr (STRIPE) c_raw = c_nxt;
+        else if constexpr (PF && GUARD) {
+            if ((unsigned)row >= (unsigned)lr) c = (int)rseq[min(max(row + 2, 0), rlast)];
+        }
         if constexpr (STRIPE) {
             // Loads and stores share one wait counter, and the compiler waits for all of
             // them once both kinds are pending: take this step's loads (empty asm reading
@@ -457,41 +491,46 @@ __device__ __forceinline__ SweepOut sweep_group(const uint8_t *__restrict__ rseq
             }
         }
         din = hin;
-        hin = O::from_bits(GroupLanes<G>::shift(O::bits(hout)));
-        ein = O::from_bits(GroupLanes<G>::shift(O::bits(eout)));
-        if (t == 0) {
-            if (STRIPE && bnd_in) {
-                hin = O::from_bits((int)nb.x);
-                ein = O::from_bits((int)nb.y);
-            } else if (!(LOCAL && GroupLanes<G>::ZERO_FILL)) {   // zero fill: local boundary already
-                hin = c0v;
-                ein = LOCAL ? ZERO : O::splat(O::NEG);
+        if constexpr (GroupLanes<G>::ZERO_FILL && !STRIPE) {
+            shift_boundary(hin, ein, hout, eout);
+        } else {
+            hin = O::from_bits(GroupLanes<G>::shift(O::bits(hout)));
+            ein = O::from_bits(GroupLanes<G>::shift(O::bits(eout)));
+            if (t == 0) {
+                if (STRIPE && bnd_in) {
+                    hin = O::from_bits((int)nb.x);
+                    ein = O::from_bits((int)nb.y);
+                } else if (!(LOCAL && GroupLanes<G>::ZERO_FILL)) {   // zero fill: local boundary already
+                    hin = c0v;
+                    ein = LOCAL ? ZERO : O::splat(O::NEG);
+                }
             }
+            if (ladder) c0v = O::add(c0v, GEV);
         }
-        if (ladder) c0v = O::add(c0v, GEV);
     };
-    // Steps [G-1, s_hi) have every active lane of the wave on a real row (s_hi: the
-    // shortest active reference of the wave): they run unguarded, two per iteration.  Not
-    // in stripes, whose boundary registers leave too few VGPRs for the duplicated step: the
-    // scheduler then serialises the E -> H chain with a wait state per column.
+    // Steps from G-1 to the shortest active reference of the wave have every active lane on
+    // a real row: they run unguarded.  Not in stripes, whose boundary registers leave too
+    // few VGPRs for the duplicated step: the scheduler then serialises the E -> H chain
+    // with a wait state per column.
     if constexpr (STRIPE) {
-        for (int s = 0; s < nsteps; ++s) step(s, std::true_type{});
+        for (int s = 0; s < nsteps; ++s) step(s, std::true_type{}, c_raw);
         return finish();
     }
-    const int s_lo = min(G - 1, nsteps);
+    // Steps run in pairs from even s (cA, then cB); the unguarded pairs cover [s_lo, s_hi),
+    // s_lo the first even step >= G-1.
+    const int s_lo = min(G, nsteps);
     const int lrmin = __builtin_amdgcn_readfirstlane(wave_min_i32(lr > 0 ? lr : 0x7fffffff));
-    const int s_hi = max(s_lo, min(lrmin, nsteps));
-    for (int s = 0; s < nsteps;) {
+    const int s_hi = s_lo + ((max(s_lo, min(lrmin, nsteps)) - s_lo) & ~1);
+    for (int s = 0; s < nsteps; s += 2) {
         if (s == s_lo) {
-            for (; s + 2 <= s_hi; s += 2) {
-                step(s, std::false_type{});
-                step(s + 1, std::false_type{});
+            for (; s < s_hi; s += 2) {
+                step(s, std::false_type{}, cA);
+                step(s + 1, std::false_type{}, cB);
             }
-            if (s < s_hi) step(s++, std::false_type{});
             if (s >= nsteps) break;
         }
-        step(s, std::true_type{});
-        ++s;
+        step(s, std::true_type{}, cA);
+        if (s + 1 < nsteps) step(s + 1, std::true_type{}, cB);
     }
 
     return finish();
@@ -538,6 +577,22 @@ __device__ __forceinline__ SweepOut sweep_pairs_pipe(const uint8_t *__restrict__
     V endcol = O::splat(O::NEG);
     const bool ladder = !LOCAL && (k.mode == 1 || (k.mode == 2 && !k.free_q));
     V c0v = ladder ? O::splat(col0_value(k, 2)) : ZERO;
+    // Row shifts of ZERO_FILL tiles give every lane t > 0 lane t-1's H and E of the row.
+    // Lane 0 takes the column-0 boundary: local H = E = 0 (the zero fill itself); otherwise
+    // E stays the sentinel it holds and H steps down the column-0 ladder by gap_extend (or
+    // stays 0 with free query ends), so each is the kept old value of one DPP move.
+    const V LADV = ladder ? GEV : ZERO;
+    auto shift_boundary = [&](V &hi, V &ei, V ho, V eo) {
+        if constexpr (!GroupLanes<G>::ZERO_FILL) {
+            (void)hi, (void)ei, (void)ho, (void)eo;
+        } else if constexpr (LOCAL) {
+            hi = O::from_bits(GroupLanes<G>::shift(O::bits(ho)));
+            ei = O::from_bits(GroupLanes<G>::shift(O::bits(eo)));
+        } else {
+            hi = O::from_bits(GroupLanes<G>::shift_keep(O::bits(O::add(hi, LADV)), O::bits(ho)));
+            ei = O::from_bits(GroupLanes<G>::shift_keep(O::bits(ei), O::bits(eo)));
+        }
+    };
     const int rlast = max(lr - 1, 0);   // idle groups (lr = 0) read their row 0 harmlessly
     auto code = [&](int row) -> int { return min((int)rseq[min(max(row, 0), rlast)], prow); };
     auto load_row = [&](int c, V *dst) {
@@ -596,13 +651,17 @@ __device__ __forceinline__ SweepOut sweep_pairs_pipe(const uint8_t *__restrict__
             if (!LOCAL) endcol = O::mx(endcol, h);
         }
         din = hin;
-        hin = O::from_bits(GroupLanes<G>::shift(O::bits(hout)));
-        ein = O::from_bits(GroupLanes<G>::shift(O::bits(eout)));
-        if (!(LOCAL && GroupLanes<G>::ZERO_FILL) && t == 0) {   // zero fill: local boundary already
-            hin = c0v;
-            ein = LOCAL ? ZERO : O::splat(O::NEG);
+        if constexpr (GroupLanes<G>::ZERO_FILL) {
+            shift_boundary(hin, ein, hout, eout);
+        } else {
+            hin = O::from_bits(GroupLanes<G>::shift(O::bits(hout)));
+            ein = O::from_bits(GroupLanes<G>::shift(O::bits(eout)));
+            if (t == 0) {
+                hin = c0v;
+                ein = LOCAL ? ZERO : O::splat(O::NEG);
+            }
+            if (ladder) c0v = O::add(c0v, GEV);
         }
-        if (ladder) c0v = O::add(c0v, GEV);
     };
     for (int s = 0; s < nsteps; s += 2) {   // a step past nsteps has no row in range: harmless
         step(s, pa, pb, cx, cy);
