@@ -35,17 +35,18 @@ __global__ void scatter_kernel(int32_t *out, const int64_t *idx, const int32_t *
 
 std::vector<Variant> &variant_table() {
     static std::vector<Variant> table = [] {
-        // int16x2 rows here; each tile's int32 and half2 rows follow its int16x2 rows
+        // int16x2 rows here, int32 and half2 rows from their own units; the table is ordered
+        // by tile (G, then R), each tile's rows int16x2, int32, half2
         const std::vector<Variant> v = {GSW_TILE_ROWS(s2, KIND_I16)};
         const std::vector<Variant> w = int32_variants(), h = half_variants();
+        std::set<std::pair<int, int>> tiles;
+        for (const std::vector<Variant> *o : {&v, &w, &h})
+            for (const Variant &x : *o) tiles.insert({x.G, x.R});
         std::vector<Variant> all;
-        for (size_t i = 0; i < v.size(); ++i) {
-            all.push_back(v[i]);
-            if (i + 1 == v.size() || v[i + 1].G != v[i].G || v[i + 1].R != v[i].R)
-                for (const std::vector<Variant> *o : {&w, &h})
-                    for (const Variant &x : *o)
-                        if (x.G == v[i].G && x.R == v[i].R) all.push_back(x);
-        }
+        for (const auto &gr : tiles)
+            for (const std::vector<Variant> *o : {&v, &w, &h})
+                for (const Variant &x : *o)
+                    if (x.G == gr.first && x.R == gr.second) all.push_back(x);
         for (size_t i = 0; i < all.size(); ++i) all[i].id = (int)i;
         return all;
     }();
@@ -71,12 +72,17 @@ static std::vector<TileRule> g_rules;
 
 // Estimated relative cost per cell of a (G, R) tile for a query of length lq against
 // references of mean length lr: padded columns, per-step overhead and pipeline fill.
-static double tile_cost(int G, int R, int lq, double lr) {
+// Per-step cost of a tile's columns and its fixed work: half2 7.5 lane-slots per column
+// and about 6 per step; int16/int32 9 and 14.
+static double step_factor(int kind, int R, double extra) {
+    const double col = kind == KIND_F16 ? 7.5 : 9.0, fixed = kind == KIND_F16 ? 6.0 : 14.0;
+    return (col * R + fixed + extra) / (col * R);
+}
+static double tile_cost(int G, int R, int lq, double lr, int kind) {
     const double width = (double)G * R;
     const double pad = width / std::max(lq, 1);
-    const double step = (9.0 * R + 14.0) / (9.0 * R);
     const double fill = (lr + G - 1) / std::max(lr, 1.0);
-    return pad * step * fill;
+    return pad * step_factor(kind, R, 0.0) * fill;
 }
 
 size_t cross_lds_bytes(const Variant &v, int ns);
@@ -108,16 +114,16 @@ const Variant *pick_variant(int lq, double mean_lr, int kind, bool local, int ns
             if (lq < r.lo || lq > r.hi || r.packed != (int)packed || r.local != (int)local) continue;
             for (const Variant &v : variant_table())
                 if (v.G == r.G && v.R == r.R && v.kind == kind && v.local == local && v.G * v.R >= lq &&
-                    (for_pairs ? pairs_waves(v, ns, lds_limit) > 0 : cross_lds_bytes(v, ns) <= lds_limit))
+                    (for_pairs ? v.pairs_fn && pairs_waves(v, ns, lds_limit) > 0 : cross_lds_bytes(v, ns) <= lds_limit))
                     return &v;
         }
     }
     for (const Variant &v : variant_table()) {
         if (v.kind != kind || v.local != local) continue;
         if (v.G * v.R < lq) continue;
-        if (for_pairs ? pairs_waves(v, ns, lds_limit) == 0 : cross_lds_bytes(v, ns) > lds_limit) continue;
+        if (for_pairs ? !v.pairs_fn || pairs_waves(v, ns, lds_limit) == 0 : cross_lds_bytes(v, ns) > lds_limit) continue;
         if (g_force_g && (v.G != g_force_g || v.R != g_force_r)) continue;
-        const double c = tile_cost(v.G, v.R, lq, mean_lr) * (for_pairs ? 1.0 : tile_occupancy_cost(v, ns));
+        const double c = tile_cost(v.G, v.R, lq, mean_lr, kind) * (for_pairs ? 1.0 : tile_occupancy_cost(v, ns));
         if (c < best_cost) {
             best_cost = c;
             best = &v;
@@ -130,11 +136,10 @@ const Variant *pick_variant(int lq, double mean_lr, int kind, bool local, int ns
 // tile: padding of the last stripe, per-step overhead including the boundary load and
 // store, pipeline fill per stripe, and the per-stripe profile build, boundary traffic
 // and barrier (measured: about 12 % more per padded column than a single tile).
-static double stripe_cost(int G, int R, int lq, double lr, int nstr) {
+static double stripe_cost(int G, int R, int lq, double lr, int nstr, int kind) {
     const double pad = (double)nstr * G * R / std::max(lq, 1);
-    const double step = (9.0 * R + 17.0) / (9.0 * R);
     const double fill = (lr + G - 1) / std::max(lr, 1.0);
-    return pad * step * fill * 1.10;
+    return pad * step_factor(kind, R, 3.0) * fill * 1.10;
 }
 
 static int stripes_for(const Variant &v, int lq, bool track_last_col) {
@@ -172,12 +177,12 @@ const Variant *pick_cross_tile(int lq, double mean_lr, int kind, bool local, int
         }
     }
     const Variant *best = pick_variant(lq, mean_lr, kind, local, ns, lds_limit, false);
-    double best_cost = best ? tile_cost(best->G, best->R, lq, mean_lr) * tile_occupancy_cost(*best, ns) : 1e30;
+    double best_cost = best ? tile_cost(best->G, best->R, lq, mean_lr, kind) * tile_occupancy_cost(*best, ns) : 1e30;
     for (const Variant &v : variant_table()) {
         if (!v.stripe_fn || v.kind != kind || v.local != local || v.G * v.R >= lq) continue;
         if (cross_lds_bytes(v, ns) > lds_limit) continue;
         const int n = stripes_for(v, lq, track_last_col);
-        const double c = stripe_cost(v.G, v.R, lq, mean_lr, n) * tile_occupancy_cost(v, ns);
+        const double c = stripe_cost(v.G, v.R, lq, mean_lr, n, kind) * tile_occupancy_cost(v, ns);
         if (c < best_cost) {
             best_cost = c;
             best = &v;
@@ -235,7 +240,7 @@ const Variant *pick_pairs_tile(int lq, double mean_lr, int kind, bool local, int
     int g_force_g, g_force_r;
     forced_tile(&g_force_g, &g_force_r);
     for (const Variant &v : variant_table()) {
-        if (v.kind != kind || v.local != local || v.G * v.R < lq) continue;
+        if (v.kind != kind || v.local != local || v.G * v.R < lq || !v.pairs_fn) continue;
         if (g_force_g && (v.G != g_force_g || v.R != g_force_r)) continue;
         const int pw = prof_rows > 0 ? pairs_waves(v, prof_rows, lds_limit) : 0;
         const bool tab_ok = v.pairs_tab_fn && pairs_tab_lds_bytes(v, ns) <= lds_limit;
@@ -250,7 +255,7 @@ const Variant *pick_pairs_tile(int lq, double mean_lr, int kind, bool local, int
             if (pw == 0 && !tab_ok) continue;
             use_tab = tab_ok;
         }
-        const double c = tile_cost(v.G, v.R, lq, mean_lr);
+        const double c = tile_cost(v.G, v.R, lq, mean_lr, kind);
         if (c < best_cost) {
             best_cost = c;
             best = &v;
