@@ -94,6 +94,8 @@ constexpr int GSW_DTYPE_PK16 = 16;
 // Internal dtype policy: each pair on the narrowest type its envelope proves exact, so no
 // result is ever listed for rescoring (the serving loop, which has no rescoring pass).
 constexpr int GSW_DTYPE_SAFE = 17;
+// As GSW_DTYPE_SAFE without half2 tiles (the serving loop's rerun of listed half2 results).
+constexpr int GSW_DTYPE_SAFE16 = 18;
 // Tile element kind (KIND_I32 / KIND_I16 / KIND_F16, gsw_batch.h) of a pair under a policy,
 // from its int16 and half2 envelope checks; half: half2 tiles allowed for this call.
 int pair_kind(const SchemeDev &s, int dtype_policy, bool ok16, bool okh, bool half);

@@ -213,7 +213,8 @@ bool half_enabled() {
 int pair_kind(const SchemeDev &s, int dtype_policy, bool ok16, bool okh, bool half) {
     if (dtype_policy == GSW_DTYPE_INT32) return KIND_I32;
     const bool pk16 = dtype_policy == GSW_DTYPE_PK16;
-    if (s.mode == GSW_MODE_LOCAL && dtype_policy != GSW_DTYPE_SAFE) {
+    if (dtype_policy == GSW_DTYPE_SAFE16) half = false;
+    if (s.mode == GSW_MODE_LOCAL && dtype_policy != GSW_DTYPE_SAFE && dtype_policy != GSW_DTYPE_SAFE16) {
         // int16 allowed: by the envelope, or everywhere when the caller vouched for it (its
         // saturated results are listed); half2 wherever int16 is, results >= 2048 listed
         const bool any16 = (dtype_policy == GSW_DTYPE_INT16 || pk16) ? int16_entries(s) : ok16;

@@ -11,9 +11,12 @@
 // in flight on the two slots' streams, so the second takes the CUs as the first one's last
 // workgroups finish.  When a request's last item completes, a device top-k of its score rows
 // (score descending, ties to the lower reference index) is copied back and its memory is
-// released.  Queries run in int16 only where the score envelope proves that no value
-// saturates (src/gpusw/scheme.py:116-141, applied per query), otherwise on the int32 tiles, so
-// launches need no saturation list.
+// released.  Local queries whose values fit int16 (src/gpusw/scheme.py:116-141, applied per
+// query) run on the half2 tiles; a launch lists the results of 2048 or more (binary16 may have
+// rounded them) in its slot, and when it retires the listed query rows are planned again on
+// the int16 tiles (or int32, by the envelope), whose results the envelope proves exact, so a
+// request's top-k waits for them.  Global/semiglobal queries run on half2 or int16 only where
+// the envelope proves every value exact, otherwise on the int32 tiles.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -76,12 +79,17 @@ struct Slot {
     std::vector<CrossItem> host_items;
     void *bnd = nullptr;           // stripe boundaries of this slot's launch
     size_t bnd_bytes = 0;
+    SatEntry *sat = nullptr;       // results listed by this slot's launch (half2 >= 2048)
+    SatEntry *h_sat = nullptr;     // pinned copy of the listed entries
+    int32_t *h_hdr = nullptr;      // pinned copy of the launch's header (counters) once it ran
 };
 
 constexpr int NSLOTS = 2;
-// A launch's work counter travels in the header of its item copy: a memset would be a fill
-// kernel, which cannot start before the other slot's persistent launch frees a CU.
+// A launch's work counter (word 0) and listed-result counter (word 2) travel in the header of
+// its item copy: a memset would be a fill kernel, which cannot start before the other slot's
+// persistent launch frees a CU.
 constexpr size_t SLOT_HDR = 64;
+constexpr int32_t SLOT_SAT_CAP = 1 << 16;   // listed results per launch (more: rerun its rows)
 
 }  // namespace
 
@@ -127,6 +135,8 @@ struct Server {
     int grow(void **buf, size_t *cap, size_t need, bool pinned);
     int admit_locked();
     int plan_locked(Request *r);
+    int append_units_locked(Request *r, const std::vector<CrossUnit> &units, const std::vector<CrossQuery> &plain);
+    int rerun_rows_locked(Request *r, std::vector<int> rows);
     Request *next_issuable_locked();
     int launch_locked(int slot, Request *head);
     int retire(int slot);
@@ -174,12 +184,22 @@ int Server::plan_locked(Request *r) {
     }
     std::vector<CrossUnit> units;
     std::vector<CrossQuery> plain;
-    GSW_TRY(plan_cross_units(db.s, GSW_DTYPE_SAFE, cq, rs.max_len, db.mean_lr, (size_t)ctx.max_smem, &units,
+    GSW_TRY(plan_cross_units(db.s, GSW_DTYPE_AUTO, cq, rs.max_len, db.mean_lr, (size_t)ctx.max_smem, &units,
                              &plain));
+    GSW_TRY(append_units_locked(r, units, plain));
+    if (!e_idx.empty()) GSW_TRY(scatter_values(&ctx, reinterpret_cast<int32_t *>(score_base), e_idx, e_val));
+    return GSW_OK;
+}
+
+// Cross items of planned units appended to the request (grouped by tile); queries no tile
+// covers are scored now with the comparator.
+int Server::append_units_locked(Request *r, const std::vector<CrossUnit> &units, const std::vector<CrossQuery> &plain) {
+    const ServerDb &db = *dbs[r->db];
+    const SeqSet &rs = *db.rs;
+    const int32_t nr = rs.n;
     // chunk references into small items: a launch gathers items of one tile from the
     // admitted requests, and small requests or rare tiles must still give it more items than
     // workgroups (with 1e8-cell items, 29 of 120 launches of the serving mix had fewer)
-    const int64_t nref = rs.n_nonempty;
     for (size_t u0 = 0; u0 < units.size();) {
         size_t u1 = u0;
         const bool striped = units[u0].nstr > 1;
@@ -198,40 +218,60 @@ int Server::plan_locked(Request *r) {
             int64_t rows = (int64_t)(5e7 / std::max(qcols, 1.0));
             rows = std::max<int64_t>(rows, (int64_t)(2.0 * gpw * CROSS_WAVES * db.mean_lr));
             expand_cross_items(units.data() + u0, units.data() + u1, rs, rows, gpw, &r->items);
-            (void)nref;
         }
         r->item_tile.resize(r->items.size(), 2 * v.id + (striped ? 1 : 0));
         u0 = u1;
     }
-    // queries no tile covers, and edge cells, are scored now with the comparator
-    if (!plain.empty() || !e_idx.empty()) {
-        if (!plain.empty()) {
-            std::vector<int32_t> qs, ql, rj;
-            std::vector<int64_t> oi;
-            int maxq = 1;
-            for (const CrossQuery &q : plain) {
-                maxq = std::max(maxq, q.len);
-                for (int b = 0; b < nr; ++b) {
-                    if (rs.len[b] == 0) continue;
-                    qs.push_back(q.start);
-                    ql.push_back(q.len);
-                    rj.push_back(b);
-                    oi.push_back(q.out + b);
-                }
+    // queries no tile covers are scored now with the comparator
+    if (!plain.empty()) {
+        std::vector<int32_t> qs, ql, rj;
+        std::vector<int64_t> oi;
+        int maxq = 1;
+        for (const CrossQuery &q : plain) {
+            maxq = std::max(maxq, q.len);
+            for (int b = 0; b < nr; ++b) {
+                if (rs.len[b] == 0) continue;
+                qs.push_back(q.start);
+                ql.push_back(q.len);
+                rj.push_back(b);
+                oi.push_back(q.out + b);
             }
-            PlainJob job{};
-            job.qcodes = code_base; job.qoff = nullptr; job.rcodes = rs.d_codes; job.roff = rs.d_off;
-            job.dtype = GSW_DTYPE_INT32; job.threads = 256; job.maxq = maxq;
-            job.h_qstart = qs.data(); job.h_qlen = ql.data(); job.h_rj = rj.data(); job.h_outidx = oi.data();
-            job.npairs = (int64_t)qs.size();
-            job.d_out = reinterpret_cast<int32_t *>(score_base);
-            float ms = 0.f;
-            GSW_TRY(plain_run(&ctx, db.s, job, &ms));
-            gpu_ms += ms;
         }
-        if (!e_idx.empty())
-            GSW_TRY(scatter_values(&ctx, reinterpret_cast<int32_t *>(score_base), e_idx, e_val));
+        PlainJob job{};
+        job.qcodes = code_base; job.qoff = nullptr; job.rcodes = rs.d_codes; job.roff = rs.d_off;
+        job.dtype = GSW_DTYPE_INT32; job.threads = 256; job.maxq = maxq;
+        job.h_qstart = qs.data(); job.h_qlen = ql.data(); job.h_rj = rj.data(); job.h_outidx = oi.data();
+        job.npairs = (int64_t)qs.size();
+        job.d_out = reinterpret_cast<int32_t *>(score_base);
+        float ms = 0.f;
+        GSW_TRY(plain_run(&ctx, db.s, job, &ms));
+        gpu_ms += ms;
     }
+    return GSW_OK;
+}
+
+// Query rows of a request whose half2 results were listed: planned again on tiles whose
+// results the envelope proves exact (int16, else int32); the new items follow the request's
+// others, so its top-k waits for them.  A half2 launch still running other items of a row
+// can only write exact values below 2048, or list its own.
+int Server::rerun_rows_locked(Request *r, std::vector<int> rows) {
+    std::sort(rows.begin(), rows.end());
+    rows.erase(std::unique(rows.begin(), rows.end()), rows.end());
+    const ServerDb &db = *dbs[r->db];
+    const SeqSet &rs = *db.rs;
+    std::vector<CrossQuery> cq;
+    for (int a : rows) {
+        const int32_t len = r->off[a + 1] - r->off[a];
+        if (len == 0 || rs.n_nonempty == 0) continue;
+        cq.push_back(CrossQuery{(int32_t)r->codes_at + r->off[a], len,
+                                (int64_t)(r->scores_at / 4) + (int64_t)a * rs.n, a});
+    }
+    std::vector<CrossUnit> units;
+    std::vector<CrossQuery> plain;
+    GSW_TRY(plan_cross_units(db.s, GSW_DTYPE_SAFE16, cq, rs.max_len, db.mean_lr, (size_t)ctx.max_smem, &units,
+                             &plain));
+    GSW_TRY(append_units_locked(r, units, plain));
+    rescored += (int64_t)cq.size() * rs.n;
     return GSW_OK;
 }
 
@@ -366,7 +406,10 @@ int Server::launch_locked(int si, Request *head) {
     a.ns = d.s.ns;
     a.k = SweepConst{d.s.mode, d.s.go, d.s.ge, d.s.free_q, d.s.free_r, d.s.ns};
     a.out = reinterpret_cast<int32_t *>(score_base);
-    a.sat_check = 0;   // int16 and half2 only where the envelope proves every value exact
+    a.sat = sl.sat;
+    a.sat_count = reinterpret_cast<int32_t *>(static_cast<char *>(sl.items) + 8);
+    a.sat_cap = SLOT_SAT_CAP;
+    a.sat_check = (v.kind == KIND_F16 && d.s.mode == GSW_MODE_LOCAL) ? 1 : 0;
     a.items = static_cast<char *>(sl.items) + SLOT_HDR;
     a.nitems = (int32_t)sl.host_items.size();
     a.counter = static_cast<int32_t *>(sl.items);
@@ -379,6 +422,8 @@ int Server::launch_locked(int si, Request *head) {
         v.cross(a, grid, lds, sl.st);
     }
     GSW_HIP_TRY(hipGetLastError());
+    // the header's counters come back with the launch (DMA: no CU needed)
+    GSW_HIP_TRY(hipMemcpyAsync(sl.h_hdr, sl.items, 16, hipMemcpyDeviceToHost, sl.st));
     GSW_HIP_TRY(hipEventRecord(sl.ev, sl.st));
     launches++;
     inflight.push_back(si);
@@ -446,7 +491,34 @@ void Server::complete_loop() {
 int Server::retire(int si) {
     Slot &sl = slots[si];
     GSW_HIP_TRY(hipEventSynchronize(sl.ev));
+    const int32_t listed = sl.h_hdr[2];
+    if (listed > 0 && listed <= SLOT_SAT_CAP) {
+        GSW_HIP_TRY(hipMemcpyAsync(sl.h_sat, sl.sat, sizeof(SatEntry) * (size_t)listed, hipMemcpyDeviceToHost, sl.st));
+        GSW_HIP_TRY(hipStreamSynchronize(sl.st));
+    }
     std::unique_lock<std::mutex> lk(mu);
+    if (listed > 0) {
+        // the listed output cells name their request and query row; more than the list
+        // holds: every query row of the launch's items
+        std::vector<int64_t> outs;
+        if (listed <= SLOT_SAT_CAP) {
+            for (int32_t i = 0; i < listed; ++i) outs.push_back(sl.h_sat[i].o);
+        } else {
+            for (const CrossItem &it : sl.host_items) {
+                outs.push_back(it.outa);
+                if (it.outb >= 0) outs.push_back(it.outb);
+            }
+        }
+        for (auto &pr : sl.reqs) {
+            Request *r = pr.first;
+            const int32_t nr = dbs[r->db]->rs->n;
+            const int64_t base = (int64_t)(r->scores_at / 4), end = base + (int64_t)r->nq * nr;
+            std::vector<int> rows;
+            for (int64_t o : outs)
+                if (o >= base && o < end) rows.push_back((int)((o - base) / nr));
+            if (!rows.empty()) GSW_TRY(rerun_rows_locked(r, std::move(rows)));
+        }
+    }
     for (auto &pr : sl.reqs) {
         Request *r = pr.first;
         r->inflight--;
@@ -533,8 +605,12 @@ int gsw_server_create(size_t mem_budget_bytes, double launch_cells, gsw_server *
     srv->code_arena.init(code_cap);
     srv->score_arena.init(score_cap);
     for (int i = 0; i < NSLOTS; ++i) {
-        GSW_HIP_TRY(hipStreamCreateWithFlags(&srv->slots[i].st, hipStreamNonBlocking));
-        GSW_HIP_TRY(hipEventCreateWithFlags(&srv->slots[i].ev, hipEventDisableTiming));
+        Slot &sl = srv->slots[i];
+        GSW_HIP_TRY(hipStreamCreateWithFlags(&sl.st, hipStreamNonBlocking));
+        GSW_HIP_TRY(hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming));
+        GSW_HIP_TRY(hipMalloc(&sl.sat, sizeof(SatEntry) * SLOT_SAT_CAP));
+        GSW_HIP_TRY(hipHostMalloc(&sl.h_sat, sizeof(SatEntry) * SLOT_SAT_CAP, hipHostMallocDefault));
+        GSW_HIP_TRY(hipHostMalloc(&sl.h_hdr, SLOT_HDR, hipHostMallocDefault));
     }
     Server *p = srv.get();
     p->worker = std::thread([p] { p->run(); });
@@ -678,6 +754,9 @@ int gsw_server_destroy(gsw_server server) {
         if (srv->slots[i].items) (void)hipFree(srv->slots[i].items);
         if (srv->slots[i].h_items) (void)hipHostFree(srv->slots[i].h_items);
         if (srv->slots[i].bnd) (void)hipFree(srv->slots[i].bnd);
+        if (srv->slots[i].sat) (void)hipFree(srv->slots[i].sat);
+        if (srv->slots[i].h_sat) (void)hipHostFree(srv->slots[i].h_sat);
+        if (srv->slots[i].h_hdr) (void)hipHostFree(srv->slots[i].h_hdr);
     }
     for (void *b : srv->graveyard) (void)hipFree(b);
     for (void *b : srv->host_graveyard) (void)hipHostFree(b);

@@ -153,3 +153,33 @@ def test_serving_mix_at_full_database_sizes():
                                   sch.substitution_table(), sch.mode, sch.gap_open, sch.gap_extend,
                                   sch.free_end_gaps_query, sch.free_end_gaps_ref)
         assert np.array_equal(want, h[:, 2])
+
+
+def test_listed_half2_results_rerun_exact(dbs):
+    """Protein queries that are copies of long references score 2,048 or more: their half2
+    results are listed, the rows run again on the int16 tiles before the top-k, and every
+    served row equals the batched top-k (and the oracle on its best hit), alone and beside
+    other requests."""
+    from oracle import native
+    from gpusw_amd.encode import funnel
+
+    rng, _, prot_refs = dbs
+    prot = Scheme.preset("blosum62")
+    heavy = rand_seqs(rng, 6, 700, 1100, PROT)   # self scores ~5 x length
+    refs = prot_refs + heavy
+    queries = heavy[:3] + [h[50:650] for h in heavy[3:]] + rand_seqs(rng, 10, 40, 500, PROT)
+    want_idx, want_sc = _expected(prot, refs, queries, 4)
+    assert int(want_sc[:, 0].max()) >= 2048 and int(want_sc[:, 0].min()) < 2048
+    with Server(memory_budget=1 << 30, launch_cells=3e8) as srv:
+        d1 = srv.add_database(refs, prot)
+        side = srv.submit(d1, rand_seqs(rng, 30, 30, 600, PROT), k=4)
+        res = srv.result(srv.submit(d1, queries, k=4))
+        srv.result(side)
+        st = srv.stats()
+    assert np.array_equal(res.result.topk_idx, want_idx)
+    assert np.array_equal(res.result.topk_scores, want_sc)
+    assert st["rescored"] > 0
+    qi = np.arange(len(queries), dtype=np.int32)
+    best = native.score_encoded_pairs(funnel(queries, prot), funnel(refs, prot), qi,
+                                      want_idx[:, 0].astype(np.int32), prot)
+    assert np.array_equal(best, want_sc[:, 0].astype(np.int64))
