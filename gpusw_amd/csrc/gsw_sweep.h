@@ -423,87 +423,68 @@ __device__ __forceinline__ SweepOut sweep_group(const uint8_t *__restrict__ rseq
         out.b = cb;
         return out;
     };
-    // One step: lane t on row s - t.  GUARD: rows outside [0, lr) are skipped (the pipeline
-    // fill and drain); without it every lane of the wave is on a real row, or in an idle
-    // group (lr = 0) whose results are never stored and whose code reads stay inside the
-    // code buffer (row < the shortest active reference of the wave).
+    // One row of the lane's R columns from its profile row pv.
+    auto row_cells = [&](const V *pv) {
+        V e = ein;
+        V hgo = row_hgo<V, LOCAL>(hin, GOV, GOP);
+        V h = hin, hprev = ZERO;
+        // xd = H(i-1, j-1) + s(i, j).  The sum for column kk+1 is taken before
+        // hup[kk] is overwritten, so H(i, j) can replace H(i-1, j) in place.
+        V xd = O::add(din, pv[0]);
+#pragma unroll
+        for (int kk = 0; kk < R; ++kk) {
+            const V xn = (kk + 1 < R) ? O::add(hup[kk], pv[kk + 1]) : xd;
+            if constexpr (O::MAX3) {
+                cell_max3<V, LOCAL, R>(kk, xd, e, h, hprev, best, hup[kk], hgo, fn[kk], GOV, GEV, ZERO);
+            } else if (LOCAL) {
+                e = O::mx(O::subu(e, GEP), hgo);
+                h = O::mx(O::mx(xd, fn[kk]), e);   // E last: the serial chain is E -> H -> H-gap -> E
+                best = O::mx(best, h);
+                hup[kk] = h;
+                hgo = O::subu(h, GOP);
+                fn[kk] = O::mx(O::subu(fn[kk], GEP), hgo);
+            } else {
+                e = O::mx(O::add(e, GEV), hgo);
+                h = O::mx(O::mx(xd, fn[kk]), e);
+                hup[kk] = h;
+                hgo = O::add(h, GOV);
+                fn[kk] = O::mx(O::add(fn[kk], GEV), hgo);
+            }
+            xd = xn;
+        }
+        hout = h;
+        eout = e;
+        if (!LOCAL && track_endcol) endcol = O::mx(endcol, h);   // lane's last column
+    };
+    // One step of a tile (not a stripe): lane t on row s - t.  GUARD: rows outside [0, lr)
+    // are skipped (the pipeline fill and drain); without it every lane of the wave is on a
+    // real row, or in an idle group (lr = 0) whose results are never stored and whose code
+    // reads stay inside the code buffer (row < the shortest active reference of the wave).
+    // c: this step parity's prefetched code (PF).
     auto step = [&](int s, auto guard, int &c) {
         constexpr bool GUARD = decltype(guard)::value;
         const int row = s - t;
-        if (STRIPE && bnd_in && t == 0 && row + 1 < lr)   // lane 0's next row
-            nb = uint2{*bnd(bh, boff + (uint32_t)(row + 1)), *bnd(be, boff + (uint32_t)(row + 1))};
-        int c_nxt = 0;
-        if constexpr (STRIPE) c_nxt = (int)rseq[min(max(row + 1, 0), rlast)];
         if (!GUARD || (unsigned)row < (unsigned)lr) {
             V pv[KC * 4];
-            if constexpr (STRIPE) load_row(min(c_raw, k.ns - 1), pv);
-            else if constexpr (PF) load_row(min(c, k.ns - 1), pv);
-            else load_row(code_at(row), pv);
-            if constexpr (PF && !STRIPE) c = (int)rseq[min(max(row + 2, 0), rlast)];
-            V e = ein;
-            V hgo = row_hgo<V, LOCAL>(hin, GOV, GOP);
-            V h = hin, hprev = ZERO;
-            // xd = H(i-1, j-1) + s(i, j).  The sum for column kk+1 is taken before
-            // hup[kk] is overwritten, so H(i, j) can replace H(i-1, j) in place.
-            V xd = O::add(din, pv[0]);
-#pragma unroll
-            for (int kk = 0; kk < R; ++kk) {
-                const V xn = (kk + 1 < R) ? O::add(hup[kk], pv[kk + 1]) : xd;
-                if constexpr (O::MAX3) {
-                    cell_max3<V, LOCAL, R>(kk, xd, e, h, hprev, best, hup[kk], hgo, fn[kk], GOV, GEV, ZERO);
-                } else if (LOCAL) {
-                    e = O::mx(O::subu(e, GEP), hgo);
-                    h = O::mx(O::mx(xd, fn[kk]), e);   // E last: the serial chain is E -> H -> H-gap -> E
-                    best = O::mx(best, h);
-                    hup[kk] = h;
-                    hgo = O::subu(h, GOP);
-                    fn[kk] = O::mx(O::subu(fn[kk], GEP), hgo);
-                } else {
-                    e = O::mx(O::add(e, GEV), hgo);
-                    h = O::mx(O::mx(xd, fn[kk]), e);
-                    hup[kk] = h;
-                    hgo = O::add(h, GOV);
-                    fn[kk] = O::mx(O::add(fn[kk], GEV), hgo);
-                }
-                xd = xn;
+            if constexpr (PF) {
+                load_row(min(c, k.ns - 1), pv);
+                c = (int)rseq[min(max(row + 2, 0), rlast)];
+            } else {
+                load_row(code_at(row), pv);
             }
-            hout = h;
-            eout = e;
-            if (!LOCAL && track_endcol) endcol = O::mx(endcol, h);   // lane's last column
-        }
-        if constexpr (STRIPE) c_raw = c_nxt;
-        else if constexpr (PF && GUARD) {
-            if ((unsigned)row >= (unsigned)lr) c = (int)rseq[min(max(row + 2, 0), rlast)];
-        }
-        if constexpr (STRIPE) {
-            // Loads and stores share one wait counter, and the compiler waits for all of
-            // them once both kinds are pending: take this step's loads (empty asm reading
-            // their registers; memory clobber keeps the store below it), then store lane
-            // G-1's boundary row, which drains during the next step's columns.
-            asm volatile("" ::"v"(nb.x), "v"(nb.y), "v"(c_nxt) : "memory");
-            if (bnd_out && t == G - 1 && (unsigned)row < (unsigned)lr) {
-                // two planes: the stores read H and E where they live (loop-carried, not
-                // rewritten until late in the next step); one 64-bit store would need an
-                // adjacent copy that the next step's first profile load overwrites,
-                // forcing an early wait
-                *bnd(bh, boff + (uint32_t)row) = (uint32_t)O::bits(hout);
-                *bnd(be, boff + (uint32_t)row) = (uint32_t)O::bits(eout);
-            }
+            row_cells(pv);
+        } else if constexpr (PF) {
+            c = (int)rseq[min(max(row + 2, 0), rlast)];
         }
         din = hin;
-        if constexpr (GroupLanes<G>::ZERO_FILL && !STRIPE) {
+        if constexpr (GroupLanes<G>::ZERO_FILL) {
             shift_boundary(hin, ein, hout, eout);
         } else {
             hin = O::from_bits(GroupLanes<G>::shift(O::bits(hout)));
             ein = O::from_bits(GroupLanes<G>::shift(O::bits(eout)));
             if (t == 0) {
-                if (STRIPE && bnd_in) {
-                    hin = O::from_bits((int)nb.x);
-                    ein = O::from_bits((int)nb.y);
-                } else if (!(LOCAL && GroupLanes<G>::ZERO_FILL)) {   // zero fill: local boundary already
-                    hin = c0v;
-                    ein = LOCAL ? ZERO : O::splat(O::NEG);
-                }
+                hin = c0v;
+                ein = LOCAL ? ZERO : O::splat(O::NEG);
             }
             if (ladder) c0v = O::add(c0v, GEV);
         }
@@ -513,7 +494,46 @@ __device__ __forceinline__ SweepOut sweep_group(const uint8_t *__restrict__ rseq
     // few VGPRs for the duplicated step: the scheduler then serialises the E -> H chain
     // with a wait state per column.
     if constexpr (STRIPE) {
-        for (int s = 0; s < nsteps; ++s) step(s, std::true_type{}, c_raw);
+        // (the step written out here, not through the step lambda: with the lambda the
+        // compiler schedules the global/semiglobal stripes' E -> H chain with a wait state
+        // per column)
+        for (int s = 0; s < nsteps; ++s) {
+            const int row = s - t;
+            if (bnd_in && t == 0 && row + 1 < lr)   // lane 0's next row
+                nb = uint2{*bnd(bh, boff + (uint32_t)(row + 1)), *bnd(be, boff + (uint32_t)(row + 1))};
+            const int c_nxt = (int)rseq[min(max(row + 1, 0), rlast)];
+            if ((unsigned)row < (unsigned)lr) {
+                V pv[KC * 4];
+                load_row(min(c_raw, k.ns - 1), pv);
+                row_cells(pv);
+            }
+            c_raw = c_nxt;
+            // Loads and stores share one wait counter, and the compiler waits for all of
+            // them once both kinds are pending: take this step's loads (empty asm reading
+            // their registers; memory clobber keeps the store below it), then store lane
+            // G-1's boundary row, which drains during the next step's columns.  Two planes:
+            // the stores read H and E where they live (loop-carried, not rewritten until
+            // late in the next step); one 64-bit store would need an adjacent copy that the
+            // next step's first profile load overwrites, forcing an early wait.
+            asm volatile("" ::"v"(nb.x), "v"(nb.y), "v"(c_nxt) : "memory");
+            if (bnd_out && t == G - 1 && (unsigned)row < (unsigned)lr) {
+                *bnd(bh, boff + (uint32_t)row) = (uint32_t)O::bits(hout);
+                *bnd(be, boff + (uint32_t)row) = (uint32_t)O::bits(eout);
+            }
+            din = hin;
+            hin = O::from_bits(GroupLanes<G>::shift(O::bits(hout)));
+            ein = O::from_bits(GroupLanes<G>::shift(O::bits(eout)));
+            if (t == 0) {
+                if (bnd_in) {
+                    hin = O::from_bits((int)nb.x);
+                    ein = O::from_bits((int)nb.y);
+                } else if (!(LOCAL && GroupLanes<G>::ZERO_FILL)) {
+                    hin = c0v;
+                    ein = LOCAL ? ZERO : O::splat(O::NEG);
+                }
+            }
+            if (ladder) c0v = O::add(c0v, GEV);
+        }
         return finish();
     }
     // Steps run in pairs from even s (cA, then cB); the unguarded pairs cover [s_lo, s_hi),
