@@ -364,11 +364,16 @@ static void launch_stripe(const KArgs &a, int grid, size_t lds, hipStream_t st) 
             &launch_cross<V, G, R, L>, &launch_pairs<V, G, R, L>, &launch_stripe<V, G, R, L>,     \
             reinterpret_cast<const void *>(&pairs_tab_kernel<V, G, R, L, PAIRS_TAB_WAVES>), KIND}
 #define GSW_SVARIANT2(V, KIND, G, R) GSW_SVARIANT(V, KIND, G, R, true), GSW_SVARIANT(V, KIND, G, R, false)
-// a tile for cross products only (no explicit-pairs kernels)
+// a tile for cross products only (no explicit-pairs kernels), and one that also runs stripes
 #define GSW_XVARIANT(V, KIND, G, R, L)                                                            \
     Variant{0, G, R, KIND != KIND_I32, L,                                                         \
             reinterpret_cast<const void *>(&cross_kernel<V, G, R, L, CROSS_WAVES>), nullptr,      \
             nullptr, &launch_cross<V, G, R, L>, nullptr, nullptr, nullptr, KIND}
+#define GSW_XSVARIANT(V, KIND, G, R, L)                                                           \
+    Variant{0, G, R, KIND != KIND_I32, L,                                                         \
+            reinterpret_cast<const void *>(&cross_kernel<V, G, R, L, CROSS_WAVES>), nullptr,      \
+            reinterpret_cast<const void *>(&stripe_kernel<V, G, R, L, CROSS_WAVES>),              \
+            &launch_cross<V, G, R, L>, nullptr, &launch_stripe<V, G, R, L>, nullptr, KIND}
 
 // The rows of one element type over every compiled tile, in the table's tile order (each
 // element type's instantiations compile in a translation unit of their own).

@@ -53,6 +53,29 @@ def test_forced_stripes_every_scheme(name, dtype):
     assert st.plain_pairs == 0
 
 
+@pytest.mark.parametrize("tile", [(8, 28), (8, 30), (8, 36)])
+@pytest.mark.parametrize("preset", ["dna", "blosum62"])
+def test_forced_half2_local_stripe_widths(tile, preset):
+    """The local stripe widths compiled for half2 only (224, 240 and 288 columns): queries
+    at, one past and well beyond the stripe edges, paired with short unstriped queries."""
+    scheme = Scheme.preset(preset).replace(mode="local")
+    alpha = PROT if preset == "blosum62" else "ACGT"
+    W = tile[0] * tile[1]
+    rng = np.random.default_rng(stable_seed("h2stripes" + preset + str(W)))
+    qlens = [W - 1, W, W + 1, 2 * W + 3, 3 * W - 7, 700, 1301, 33, 150]
+    queries = ["".join(rng.choice(list(alpha), L)) for L in qlens]
+    refs = rand_seqs(rng, 40, 1, 500, alpha) + ["", queries[4][100:600]]
+    got, st = _score(queries, refs, scheme, tile)
+    want = _oracle(queries, refs, scheme)
+    bad = np.argwhere(got.astype(np.int64) != want)
+    assert bad.size == 0, f"{len(bad)} mismatches, first {bad[:5].tolist()}"
+    # the protein query against its own 500-residue stretch scores above 2048: that one
+    # listed result is rescored in int32 as an explicit pair, and explicit pairs do not
+    # stripe, so a pair of 713 x 500 residues wider than the forced tile takes the comparator
+    listed = 1 if preset == "blosum62" else 0
+    assert st.plain_pairs == listed
+
+
 @pytest.mark.parametrize("mode", ["local", "global", "semiglobal"])
 def test_protein_beyond_widest_tile(mode):
     """No comparator fallback: 1500-3300 aa protein queries are striped by the cost model."""
