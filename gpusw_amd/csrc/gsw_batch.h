@@ -48,6 +48,7 @@ struct KArgs {
     uint32_t *bnd;               // stripe kernel: boundary regions, one per workgroup,
     int64_t bnd_stride;          //   each an H plane then an E plane of bnd_stride words (>= item rows)
     const int64_t *bpos;         //   row offsets of the sorted references (n + 1)
+    const int64_t *oidx;         //   explicit pairs: output index of each position (else null)
     int32_t tab_copies;          // pairs table kernel: interleaved copies of the table
     int32_t prow;                // pairs kernel: last profile row (rows 0..prow are built)
 };

@@ -164,16 +164,19 @@ __attribute__((amdgpu_waves_per_eu(WavesPerEU<LOCAL, G, R>::value))) stripe_kern
                                                                        lqa, lqb, cbase, bnd_wg, bnd_wg + a.bnd_stride,
                                                                        !first, !last, nullptr, 0, boff);
                 if (t == 0 && ref >= 0) {
+                    // explicit pairs: a position is one (query A, reference) pair with an
+                    // output index of its own (and no query B)
+                    const int64_t oa = a.oidx ? a.oidx[rpos] : item.outa + ref;
                     int va = res.a, vb = res.b;
                     if (!first) {   // written by this workgroup in an earlier stripe
-                        va = max(va, a.out[item.outa + ref]);
+                        va = max(va, a.out[oa]);
                         if (item.outb >= 0) vb = max(vb, a.out[item.outb + ref]);
                     }
                     if (last) {
-                        emit<V>(a, item.outa + ref, -1, ref, va);
+                        emit<V>(a, oa, -1, ref, va);
                         if (item.outb >= 0) emit<V>(a, item.outb + ref, -1, ref, vb);
                     } else {
-                        a.out[item.outa + ref] = va;
+                        a.out[oa] = va;
                         if (item.outb >= 0) a.out[item.outb + ref] = vb;
                     }
                 }

@@ -850,6 +850,151 @@ int batch_cross(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
     return GSW_OK;
 }
 
+// ---------------------------------------------------------------- long explicit pairs
+// Explicit pairs that no pairs tile takes on either side (both longer than the widest
+// tile) run on the int32 stripe kernels.  The pairs of one query become a list of
+// positions, its references longest first, each position with its own output index
+// (KArgs::oidx), and the list is cut into items of whole references as a cross product's
+// reference range is; the stripe kernel then sweeps them as it sweeps a reference range.
+// int32 throughout: no result is listed for rescoring.  One launch per tile, on the
+// library stream.  Pairs of queries that no int32 stripe tile takes (a forced tile
+// without stripes, or a pair outside the int32 range) are returned in rq/rr/ro.
+static int batch_long_pairs(DeviceCtx *ctx, const SchemeDev &s, const SeqSet &qs, const SeqSet &rs,
+                            const std::vector<int32_t> &pq, const std::vector<int32_t> &pr,
+                            const std::vector<int64_t> &po, int32_t *d_out, BatchResult *res,
+                            std::vector<int32_t> *rq, std::vector<int32_t> *rr, std::vector<int64_t> *ro) {
+    const bool local = s.mode == GSW_MODE_LOCAL;
+    const bool track_last_col = s.mode == GSW_MODE_SEMIGLOBAL && s.free_q;
+    const size_t n = pq.size();
+    std::vector<size_t> idx(n);
+    std::iota(idx.begin(), idx.end(), (size_t)0);
+    std::stable_sort(idx.begin(), idx.end(), [&](size_t x, size_t y) {
+        return pq[x] != pq[y] ? pq[x] < pq[y] : rs.len[pr[x]] > rs.len[pr[y]];
+    });
+    struct QGroup { size_t i0, i1; const Variant *v; int nstr, shift; };
+    std::vector<QGroup> qg;
+    for (size_t i0 = 0; i0 < n;) {
+        const int q = pq[idx[i0]];
+        size_t i1 = i0;
+        int64_t rows = 0;
+        bool ok = true;
+        for (; i1 < n && pq[idx[i1]] == q; ++i1) {
+            rows += rs.len[pr[idx[i1]]];
+            ok = ok && fits_int32(s, qs.len[q], rs.len[pr[idx[i1]]]);
+        }
+        int nstr = 1;
+        const Variant *v = ok ? pick_cross_tile(qs.len[q], (double)rows / (double)(i1 - i0), KIND_I32, local, s.ns,
+                                                (size_t)ctx->max_smem, track_last_col, &nstr)
+                              : nullptr;
+        if (v && v->stripe_fn) {
+            qg.push_back(QGroup{i0, i1, v, nstr, track_last_col ? (v->R - qs.len[q] % v->R) % v->R : 0});
+        } else {
+            for (size_t i = i0; i < i1; ++i) {
+                rq->push_back(pq[idx[i]]);
+                rr->push_back(pr[idx[i]]);
+                ro->push_back(po[idx[i]]);
+            }
+        }
+        i0 = i1;
+    }
+    if (qg.empty()) return GSW_OK;
+    std::stable_sort(qg.begin(), qg.end(), [](const QGroup &x, const QGroup &y) { return x.v->id < y.v->id; });
+
+    // positions in group order; items per launch, each of whole references and about an
+    // equal share of the launch's rows (two items per workgroup), longest first
+    std::vector<int32_t> vorder;
+    std::vector<int64_t> vbpos{0}, voidx;
+    std::vector<CrossItem> items;
+    struct Launch { const Variant *v; size_t it0, it1; int grid; int64_t widest; };
+    std::vector<Launch> launches;
+    int64_t bnd_words = 0;
+    for (size_t g0 = 0; g0 < qg.size();) {
+        size_t g1 = g0;
+        int64_t rows = 0;
+        int max_lr = 0;
+        for (; g1 < qg.size() && qg[g1].v == qg[g0].v; ++g1)
+            for (size_t i = qg[g1].i0; i < qg[g1].i1; ++i) {
+                rows += (int64_t)rs.len[pr[idx[i]]] * qg[g1].nstr;
+                max_lr = std::max(max_lr, rs.len[pr[idx[i]]]);
+            }
+        const Variant &v = *qg[g0].v;
+        Launch L{&v, items.size(), 0, 0, 0};
+        GSW_TRY(occupancy_grid(ctx, v.stripe_fn, CROSS_WAVES * 64, cross_lds_bytes(v, s.ns), &L.grid));
+        const int64_t cap = stripe_item_rows(rs.n_nonempty ? (double)rs.total / rs.n_nonempty : 1.0, rs.max_len);
+        const int64_t target = std::min<int64_t>(cap, std::max<int64_t>(1, rows / (2 * (int64_t)std::max(L.grid, 1))));
+        for (size_t gi = g0; gi < g1; ++gi) {
+            const QGroup &G = qg[gi];
+            const int q = pq[idx[G.i0]];
+            int64_t item_rows = 0;
+            int32_t rb = (int32_t)vorder.size();
+            for (size_t i = G.i0; i < G.i1; ++i) {
+                const int lr = rs.len[pr[idx[i]]];
+                vorder.push_back(pr[idx[i]]);
+                voidx.push_back(po[idx[i]]);
+                vbpos.push_back(vbpos.back() + lr);
+                item_rows += lr;
+                if (i + 1 == G.i1 || item_rows * G.nstr >= target || item_rows + max_lr > cap) {
+                    const int32_t re = (int32_t)vorder.size();
+                    items.push_back(CrossItem{qs.h_off[q], qs.len[q], 0, -1, 0, -1, rb, re, G.shift, G.nstr});
+                    L.widest = std::max(L.widest, item_rows);
+                    rb = re;
+                    item_rows = 0;
+                }
+            }
+        }
+        L.it1 = items.size();
+        std::stable_sort(items.begin() + L.it0, items.end(), [&](const CrossItem &x, const CrossItem &y) {
+            return (vbpos[x.rend] - vbpos[x.rbeg]) * x.nstr > (vbpos[y.rend] - vbpos[y.rbeg]) * y.nstr;
+        });
+        L.grid = std::min<int>(L.grid, (int)(L.it1 - L.it0));
+        bnd_words = std::max(bnd_words, 2 * L.widest * L.grid);
+        launches.push_back(L);
+        g0 = g1;
+    }
+    if (launches.size() > 60) return fail(GSW_ERR_LIMIT, "too many tile groups (%zu)", launches.size());
+    GSW_TRY(ensure_buffer(&ctx->bnd, &ctx->bnd_bytes, sizeof(uint32_t) * (size_t)bnd_words));
+
+    // one device block for the items and the position arrays, freed after the launches
+    const size_t npos = vorder.size();
+    auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t o_items = 0, o_order = up(sizeof(CrossItem) * items.size());
+    const size_t o_bpos = o_order + up(sizeof(int32_t) * npos), o_oidx = o_bpos + up(sizeof(int64_t) * (npos + 1));
+    const size_t bytes = o_oidx + up(sizeof(int64_t) * npos);
+    char *d_mem = nullptr;
+    GSW_HIP_TRY(hipMalloc(&d_mem, bytes));
+    struct Free { char *p; ~Free() { (void)hipFree(p); } } free_mem{d_mem};
+    GSW_HIP_TRY(hipMemcpyAsync(d_mem + o_items, items.data(), sizeof(CrossItem) * items.size(), hipMemcpyHostToDevice,
+                               ctx->stream));
+    GSW_HIP_TRY(hipMemcpyAsync(d_mem + o_order, vorder.data(), sizeof(int32_t) * npos, hipMemcpyHostToDevice, ctx->stream));
+    GSW_HIP_TRY(hipMemcpyAsync(d_mem + o_bpos, vbpos.data(), sizeof(int64_t) * (npos + 1), hipMemcpyHostToDevice,
+                               ctx->stream));
+    GSW_HIP_TRY(hipMemcpyAsync(d_mem + o_oidx, voidx.data(), sizeof(int64_t) * npos, hipMemcpyHostToDevice, ctx->stream));
+    GSW_HIP_TRY(hipMemsetAsync(ctx->counters + 2, 0, sizeof(int32_t) * launches.size(), ctx->stream));
+    GSW_HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+    for (size_t li = 0; li < launches.size(); ++li) {
+        const Launch &L = launches[li];
+        KArgs a = base_args(s, qs, rs, d_out);
+        a.items = reinterpret_cast<const CrossItem *>(d_mem + o_items) + L.it0;
+        a.nitems = (int32_t)(L.it1 - L.it0);
+        a.counter = ctx->counters + 2 + li;
+        a.rorder = reinterpret_cast<const int32_t *>(d_mem + o_order);
+        a.bpos = reinterpret_cast<const int64_t *>(d_mem + o_bpos);
+        a.oidx = reinterpret_cast<const int64_t *>(d_mem + o_oidx);
+        a.bnd = static_cast<uint32_t *>(ctx->bnd);
+        a.bnd_stride = L.widest;
+        a.sat_check = 0;
+        L.v->stripe(a, L.grid, cross_lds_bytes(*L.v, s.ns), ctx->stream);
+        GSW_HIP_TRY(hipGetLastError());
+        res->launches++;
+    }
+    GSW_HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
+    GSW_HIP_TRY(hipEventSynchronize(ctx->ev1));
+    float ms = 0.f;
+    GSW_HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    res->kernel_ms += ms;
+    return GSW_OK;
+}
+
 // ---------------------------------------------------------------- pairs
 // Explicit pairs run in batches of PAIR_BATCH pairs.  Each batch is planned on the host
 // in O(n) (counting sorts by reference length rank, then by last-column register) into a
@@ -1437,7 +1582,8 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
     // Pairs without a tile have a query wider than every pairs tile.  Where the reference
     // fits one, the pair is scored with query and reference exchanged under the transposed
     // scheme (same score in every mode); pairs longer than every tile on both sides run on
-    // the one-thread-per-pair comparator.
+    // the int32 stripe kernels (batch_long_pairs), and only what no stripe tile takes on the
+    // one-thread-per-pair comparator.
     if (!p_qi.empty()) {
         std::vector<int32_t> tq, tr, cq, cr;
         std::vector<int64_t> to, co;
@@ -1457,6 +1603,14 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
             res->launches += r.launches;
             res->plain_pairs += r.plain_pairs;
             res->rescored += r.rescored;
+        }
+        if (!cq.empty()) {   // the int32 stripe kernels, as lists of references per query
+            std::vector<int32_t> lq, lr;
+            std::vector<int64_t> lo;
+            GSW_TRY(batch_long_pairs(ctx, s, qs, rs, cq, cr, co, d_out, res, &lq, &lr, &lo));
+            cq.swap(lq);
+            cr.swap(lr);
+            co.swap(lo);
         }
         if (!cq.empty()) {
             int maxq = 1;

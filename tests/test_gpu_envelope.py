@@ -175,7 +175,7 @@ def test_long_query_pairs_run_transposed(mode, fq, fr):
     st = al._b.last_stats
     want = native.score_encoded_pairs(al._queries, al._refs, qi, rj, scheme)
     assert np.array_equal(got.astype(np.int64), want)
-    assert st.plain_pairs == 2
+    assert st.plain_pairs == 0   # the two long pairs run on the int32 stripe kernels
 
 
 def test_long_query_saturation_rescored_transposed(monkeypatch):

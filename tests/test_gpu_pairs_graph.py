@@ -70,7 +70,7 @@ def test_head_and_rest_segments_mixed(mode):
     """600,000 pairs (a head segment of 344,064 pairs planned before the rest is classified)
     mixing empty sequences, int16 and int32 pairs of several tiles, queries longer than any
     pairs tile against shorter references (run transposed) and pairs longer than every tile
-    on both sides (comparator): every pair equals the comparator, a sample equals the oracle,
+    on both sides (int32 stripe kernels): every pair equals the comparator, a sample equals the oracle,
     and a shuffled order (other pairs in the head) gives the same score per pair."""
     rng = np.random.default_rng(23)
     qlen = rng.integers(1, 600, 2000)
@@ -90,7 +90,7 @@ def test_head_and_rest_segments_mixed(mode):
     got = al.score_pairs(qi, rj)
     st = al._b.last_stats
     both_long = int(((qlen[qi] > 2048) & (rlen[rj] > 2048)).sum())
-    assert st.plain_pairs == both_long > 0 and st.launches >= 8
+    assert both_long > 0 and st.plain_pairs == 0 and st.launches >= 8
     assert st.cells == int((qlen[qi].astype(np.int64) * rlen[rj].astype(np.int64)).sum())
     eager = Aligner(scheme, backend="hip-plain").index(synth.pack(refs)).set_queries(
         synth.pack(queries)).score_pairs(qi, rj)

@@ -6,7 +6,7 @@ import ctypes
 import numpy as np
 import pytest
 
-from gpusw_amd import Aligner, Scheme, _hip
+from gpusw_amd import Aligner, Matrix, Scheme, _hip
 from gpusw_amd.backends.hip import scheme_struct_cached
 from gpusw_amd.encode import funnel
 from gpusw_amd.serve import Server
@@ -70,8 +70,8 @@ def test_forced_half2_local_stripe_widths(tile, preset):
     bad = np.argwhere(got.astype(np.int64) != want)
     assert bad.size == 0, f"{len(bad)} mismatches, first {bad[:5].tolist()}"
     # the protein query against its own 500-residue stretch scores above 2048: that one
-    # listed result is rescored in int32 as an explicit pair, and explicit pairs do not
-    # stripe, so a pair of 713 x 500 residues wider than the forced tile takes the comparator
+    # listed result is rescored in int32 as an explicit pair of 713 x 500 residues, wider
+    # than the forced tile, whose int32 rows have no stripe kernel: the comparator takes it
     listed = 1 if preset == "blosum62" else 0
     assert st.plain_pairs == listed
 
@@ -118,8 +118,8 @@ def test_saturation_in_late_stripe_rescored(monkeypatch):
               len(queries), rset.handle, out.ptr, ctypes.byref(st))
     assert np.array_equal(out.to_host().astype(np.int64), want)
     # both saturated pairs are longer than every explicit-pairs tile (2,048 columns) on both
-    # sides: their int32 rescoring is the comparator's
-    assert st.rescored == int((want >= 32767).sum()) == 2 and st.plain_pairs == 2
+    # sides: they are rescored as explicit pairs on the int32 stripe kernels
+    assert st.rescored == int((want >= 32767).sum()) == 2 and st.plain_pairs == 0
 
 
 def test_served_long_queries_equal_aligner():
@@ -135,3 +135,33 @@ def test_served_long_queries_equal_aligner():
     assert np.array_equal(res[0].result.topk_idx, want.topk_idx)
     assert np.array_equal(res[0].result.topk_scores, want.topk_scores)
     assert np.array_equal(res[1].result.topk_scores, want.topk_scores[:3])
+
+
+@pytest.mark.parametrize("mode,fq,fr", [("local", False, False), ("global", False, False),
+                                        ("semiglobal", False, False), ("semiglobal", True, False),
+                                        ("semiglobal", False, True), ("semiglobal", True, True)])
+def test_long_explicit_pairs_on_int32_stripes(mode, fq, fr):
+    """Explicit pairs longer than every pairs tile (2,048 columns) on both sides run on the
+    int32 stripe kernels as per-query lists of references: one query with many references
+    (several items), queries of several stripe counts, a repeated pair, shuffled order and
+    shorter pairs in the same call; no comparator, every pair equal to the oracle."""
+    m = np.array([[2, -3, -1, -4], [-2, 3, -3, 0], [-1, -2, 4, -3], [-4, 1, -2, 2]])   # asymmetric
+    scheme = Scheme(mode=mode, matrix=Matrix.from_array("ACGT", m), gap_open=-4, gap_extend=-1,
+                    free_end_gaps_query=fq, free_end_gaps_ref=fr)
+    rng = np.random.default_rng(stable_seed(f"longpairs/{mode}/{fq}/{fr}"))
+    queries = rand_seqs(rng, 5, 2049, 3600, "ACGT") + rand_seqs(rng, 3, 20, 300, "ACGT")
+    refs = rand_seqs(rng, 30, 2049, 2500, "ACGT") + rand_seqs(rng, 10, 1, 400, "ACGT")
+    refs[3] = queries[1][300:2700]                           # a high-scoring long pair
+    qi = np.concatenate([np.zeros(30, np.int32), rng.integers(1, 5, 40).astype(np.int32),
+                         np.array([1, 1], np.int32), rng.integers(0, 8, 60).astype(np.int32)])
+    rj = np.concatenate([np.arange(30, dtype=np.int32), rng.integers(0, 30, 40).astype(np.int32),
+                         np.array([3, 3], np.int32), rng.integers(0, 40, 60).astype(np.int32)])
+    perm = rng.permutation(len(qi))
+    qi, rj = qi[perm], rj[perm]
+    al = Aligner(scheme, backend="hip").index(refs).set_queries(queries)
+    got = al.score_pairs(qi, rj)
+    st = al._b.last_stats
+    want = native.score_encoded_pairs(al._queries, al._refs, qi, rj, scheme)
+    bad = np.flatnonzero(got.astype(np.int64) != want)
+    assert bad.size == 0, f"{bad.size} mismatches, first {[(int(qi[i]), int(rj[i])) for i in bad[:5]]}"
+    assert st.plain_pairs == 0
