@@ -69,6 +69,7 @@ struct Variant {
     LaunchFn cross, pairs, stripe;
     const void *pairs_tab_fn;                       // pairs with the substitution table in LDS
     int kind;                                       // KIND_*
+    LaunchFn stripe1 = nullptr;                     // the stripe kernel in one-wave workgroups
 };
 // The half2 and int32 rows of the variant table (gsw_sweep_f16.hip, gsw_sweep_i32.hip).
 std::vector<Variant> half_variants();

@@ -349,6 +349,12 @@ template <typename V, int G, int R, bool L>
 static void launch_stripe(const KArgs &a, int grid, size_t lds, hipStream_t st) {
     hipLaunchKernelGGL((stripe_kernel<V, G, R, L, CROSS_WAVES>), dim3(grid), dim3(CROSS_WAVES * 64), lds, st, a);
 }
+// One wave per workgroup (and per item): explicit long pairs, whose items hold the few
+// references paired with one query, so that a workgroup's other waves do not sit idle.
+template <typename V, int G, int R, bool L>
+static void launch_stripe1(const KArgs &a, int grid, size_t lds, hipStream_t st) {
+    hipLaunchKernelGGL((stripe_kernel<V, G, R, L, CROSS_WAVES>), dim3(grid), dim3(64), lds, st, a);
+}
 
 #define GSW_VARIANT(V, KIND, G, R, L)                                                             \
     Variant{0, G, R, KIND != KIND_I32, L,                                                         \
@@ -365,7 +371,8 @@ static void launch_stripe(const KArgs &a, int grid, size_t lds, hipStream_t st) 
             reinterpret_cast<const void *>(&pairs_kernel<V, G, R, L, PAIRS_WAVES>),               \
             reinterpret_cast<const void *>(&stripe_kernel<V, G, R, L, CROSS_WAVES>),              \
             &launch_cross<V, G, R, L>, &launch_pairs<V, G, R, L>, &launch_stripe<V, G, R, L>,     \
-            reinterpret_cast<const void *>(&pairs_tab_kernel<V, G, R, L, PAIRS_TAB_WAVES>), KIND}
+            reinterpret_cast<const void *>(&pairs_tab_kernel<V, G, R, L, PAIRS_TAB_WAVES>), KIND,  \
+            &launch_stripe1<V, G, R, L>}
 #define GSW_SVARIANT2(V, KIND, G, R) GSW_SVARIANT(V, KIND, G, R, true), GSW_SVARIANT(V, KIND, G, R, false)
 // a tile for cross products only (no explicit-pairs kernels), and one that also runs stripes
 #define GSW_XVARIANT(V, KIND, G, R, L)                                                            \

@@ -900,28 +900,34 @@ static int batch_long_pairs(DeviceCtx *ctx, const SchemeDev &s, const SeqSet &qs
     if (qg.empty()) return GSW_OK;
     std::stable_sort(qg.begin(), qg.end(), [](const QGroup &x, const QGroup &y) { return x.v->id < y.v->id; });
 
-    // positions in group order; items per launch, each of whole references and about an
-    // equal share of the launch's rows (two items per workgroup), longest first
+    // positions in group order; items per launch, each of whole references: at least a
+    // wave's worth (64/G references) of its query's rows, else an equal share of the
+    // launch's rows (two items per workgroup), longest first.  Workgroups of one wave where
+    // the tile has that launch (a query's references are often few), their boundary
+    // regions within 512 MiB.
     std::vector<int32_t> vorder;
     std::vector<int64_t> vbpos{0}, voidx;
     std::vector<CrossItem> items;
-    struct Launch { const Variant *v; size_t it0, it1; int grid; int64_t widest; };
+    struct Launch { const Variant *v; size_t it0, it1; int grid, threads; int64_t widest; };
     std::vector<Launch> launches;
     int64_t bnd_words = 0;
     for (size_t g0 = 0; g0 < qg.size();) {
         size_t g1 = g0;
-        int64_t rows = 0;
+        int64_t rows = 0, npairs = 0;
         int max_lr = 0;
         for (; g1 < qg.size() && qg[g1].v == qg[g0].v; ++g1)
             for (size_t i = qg[g1].i0; i < qg[g1].i1; ++i) {
                 rows += (int64_t)rs.len[pr[idx[i]]] * qg[g1].nstr;
                 max_lr = std::max(max_lr, rs.len[pr[idx[i]]]);
+                ++npairs;
             }
         const Variant &v = *qg[g0].v;
-        Launch L{&v, items.size(), 0, 0, 0};
-        GSW_TRY(occupancy_grid(ctx, v.stripe_fn, CROSS_WAVES * 64, cross_lds_bytes(v, s.ns), &L.grid));
+        Launch L{&v, items.size(), 0, 0, v.stripe1 ? 64 : CROSS_WAVES * 64, 0};
+        GSW_TRY(occupancy_grid(ctx, v.stripe_fn, L.threads, cross_lds_bytes(v, s.ns), &L.grid));
         const int64_t cap = stripe_item_rows(rs.n_nonempty ? (double)rs.total / rs.n_nonempty : 1.0, rs.max_len);
-        const int64_t target = std::min<int64_t>(cap, std::max<int64_t>(1, rows / (2 * (int64_t)std::max(L.grid, 1))));
+        const int64_t wave_rows = (int64_t)(64 / v.G) * (rows / std::max<int64_t>(npairs, 1));
+        const int64_t target = std::min<int64_t>(
+            cap, std::max<int64_t>({1, wave_rows, rows / (2 * (int64_t)std::max(L.grid, 1))}));
         for (size_t gi = g0; gi < g1; ++gi) {
             const QGroup &G = qg[gi];
             const int q = pq[idx[G.i0]];
@@ -947,6 +953,7 @@ static int batch_long_pairs(DeviceCtx *ctx, const SchemeDev &s, const SeqSet &qs
             return (vbpos[x.rend] - vbpos[x.rbeg]) * x.nstr > (vbpos[y.rend] - vbpos[y.rbeg]) * y.nstr;
         });
         L.grid = std::min<int>(L.grid, (int)(L.it1 - L.it0));
+        L.grid = (int)std::max<int64_t>(1, std::min<int64_t>(L.grid, ((int64_t)1 << 26) / std::max<int64_t>(L.widest, 1)));
         bnd_words = std::max(bnd_words, 2 * L.widest * L.grid);
         launches.push_back(L);
         g0 = g1;
@@ -983,7 +990,7 @@ static int batch_long_pairs(DeviceCtx *ctx, const SchemeDev &s, const SeqSet &qs
         a.bnd = static_cast<uint32_t *>(ctx->bnd);
         a.bnd_stride = L.widest;
         a.sat_check = 0;
-        L.v->stripe(a, L.grid, cross_lds_bytes(*L.v, s.ns), ctx->stream);
+        (L.threads == 64 ? L.v->stripe1 : L.v->stripe)(a, L.grid, cross_lds_bytes(*L.v, s.ns), ctx->stream);
         GSW_HIP_TRY(hipGetLastError());
         res->launches++;
     }
