@@ -71,9 +71,13 @@ struct Variant {
     int kind;                                       // KIND_*
     LaunchFn stripe1 = nullptr;                     // the stripe kernel in one-wave workgroups
 };
-// The half2 and int32 rows of the variant table (gsw_sweep_f16.hip, gsw_sweep_i32.hip).
+// The half2 and int32 rows of the variant table (gsw_sweep_f16.hip, gsw_sweep_i32.hip), and
+// the further local half2 tiles (gsw_sweep_f16x.hip), which also attaches stripe kernels to
+// local half2 rows of the full tile set.
 std::vector<Variant> half_variants();
 std::vector<Variant> int32_variants();
+std::vector<Variant> half_extra_variants();
+void attach_half_stripes(std::vector<Variant> &rows);
 // LDS of the pairs table kernel: ns rows of (ns+2)^2 packed words (int32 tiles: ns+2),
 // times pairs_tab_copies interleaved copies (32 when that stays within 48 KiB, else 1).
 int pairs_tab_copies(const Variant &v, int ns);

@@ -38,7 +38,13 @@ std::vector<Variant> &variant_table() {
         // int16x2 rows here, int32 and half2 rows from their own units; the table is ordered
         // by tile (G, then R), each tile's rows int16x2, int32, half2
         const std::vector<Variant> v = {GSW_TILE_ROWS(s2, KIND_I16)};
-        const std::vector<Variant> w = int32_variants(), h = half_variants();
+        const std::vector<Variant> w = int32_variants();
+        const std::vector<Variant> h = [] {
+            std::vector<Variant> rows = half_variants();
+            for (const Variant &x : half_extra_variants()) rows.push_back(x);
+            attach_half_stripes(rows);
+            return rows;
+        }();
         std::set<std::pair<int, int>> tiles;
         for (const std::vector<Variant> *o : {&v, &w, &h})
             for (const Variant &x : *o) tiles.insert({x.G, x.R});

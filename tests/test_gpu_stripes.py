@@ -53,11 +53,11 @@ def test_forced_stripes_every_scheme(name, dtype):
     assert st.plain_pairs == 0
 
 
-@pytest.mark.parametrize("tile", [(8, 28), (8, 30), (8, 36)])
+@pytest.mark.parametrize("tile", [(8, 26), (8, 28), (8, 30), (8, 34), (8, 36), (8, 40), (8, 44)])
 @pytest.mark.parametrize("preset", ["dna", "blosum62"])
 def test_forced_half2_local_stripe_widths(tile, preset):
-    """The local stripe widths compiled for half2 only (224, 240 and 288 columns): queries
-    at, one past and well beyond the stripe edges, paired with short unstriped queries."""
+    """The local stripe widths compiled for half2 only (208 to 352 columns): queries at, one
+    past and well beyond the stripe edges, paired with short unstriped queries."""
     scheme = Scheme.preset(preset).replace(mode="local")
     alpha = PROT if preset == "blosum62" else "ACGT"
     W = tile[0] * tile[1]
