@@ -879,6 +879,30 @@ static int batch_long_pairs(DeviceCtx *ctx, const SchemeDev &s, const SeqSet &qs
     });
     struct QGroup { size_t i0, i1; const Variant *v; int nstr, shift; };
     std::vector<QGroup> qg;
+    int force_g, force_r;
+    forced_tile(&force_g, &force_r);
+    // A pair runs on one lane group: with fewer pairs than the GPU has lanes for (two waves
+    // per SIMD), a wider group sweeps each pair with more lanes, so the stripe cost of a
+    // G-lane tile is scaled by how much of the GPU its groups leave idle.
+    const double lanes = 2.0 * 4 * 64 * std::max(ctx->num_cus, 1);
+    auto pick = [&](int lq, double lr, int *nstr) -> const Variant * {
+        if (force_g) return pick_cross_tile(lq, lr, KIND_I32, local, s.ns, (size_t)ctx->max_smem, track_last_col, nstr);
+        const Variant *best = nullptr;
+        double best_cost = 1e30;
+        for (const Variant &v : variant_table()) {
+            if (!v.stripe_fn || v.kind != KIND_I32 || v.local != local) continue;
+            if (cross_lds_bytes(v, s.ns) > (size_t)ctx->max_smem) continue;
+            const int ns_ = stripes_for(v, lq, track_last_col);
+            const double idle = std::max(1.0, lanes / ((double)n * v.G));
+            const double c = stripe_cost(v.G, v.R, lq, lr, ns_, KIND_I32) * tile_occupancy_cost(v, s.ns) * idle;
+            if (c < best_cost) {
+                best_cost = c;
+                best = &v;
+                *nstr = ns_;
+            }
+        }
+        return best;
+    };
     for (size_t i0 = 0; i0 < n;) {
         const int q = pq[idx[i0]];
         size_t i1 = i0;
@@ -889,9 +913,7 @@ static int batch_long_pairs(DeviceCtx *ctx, const SchemeDev &s, const SeqSet &qs
             ok = ok && fits_int32(s, qs.len[q], rs.len[pr[idx[i1]]]);
         }
         int nstr = 1;
-        const Variant *v = ok ? pick_cross_tile(qs.len[q], (double)rows / (double)(i1 - i0), KIND_I32, local, s.ns,
-                                                (size_t)ctx->max_smem, track_last_col, &nstr)
-                              : nullptr;
+        const Variant *v = ok ? pick(qs.len[q], (double)rows / (double)(i1 - i0), &nstr) : nullptr;
         if (v && v->stripe_fn) {
             qg.push_back(QGroup{i0, i1, v, nstr, track_last_col ? (v->R - qs.len[q] % v->R) % v->R : 0});
         } else {

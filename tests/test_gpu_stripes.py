@@ -76,6 +76,25 @@ def test_forced_half2_local_stripe_widths(tile, preset):
     assert st.plain_pairs == listed
 
 
+@pytest.mark.parametrize("tile", [(16, 32), (32, 32)])
+@pytest.mark.parametrize("mode,fq,fr", [("local", False, False), ("global", False, False),
+                                        ("semiglobal", True, False), ("semiglobal", True, True)])
+def test_forced_int32_wide_stripes(tile, mode, fq, fr):
+    """The int32 stripe kernels of the 16- and 32-lane tiles (512 and 1,024 columns):
+    queries at, one past and beyond the stripe edges beside unstriped ones."""
+    scheme = Scheme(mode=mode, dtype="int32", free_end_gaps_query=fq, free_end_gaps_ref=fr)
+    W = tile[0] * tile[1]
+    rng = np.random.default_rng(stable_seed(f"wide/{W}/{mode}/{fq}/{fr}"))
+    qlens = [W - 1, W, W + 1, 2 * W + 5, 300, 40]
+    queries = ["".join(rng.choice(list("ACGT"), L)) for L in qlens]
+    refs = rand_seqs(rng, 30, 1, 400, "ACGT") + ["", queries[3][:350]]
+    got, st = _score(queries, refs, scheme, tile)
+    want = _oracle(queries, refs, scheme)
+    bad = np.argwhere(got.astype(np.int64) != want)
+    assert bad.size == 0, f"{len(bad)} mismatches, first {bad[:5].tolist()}"
+    assert st.plain_pairs == 0
+
+
 @pytest.mark.parametrize("mode", ["local", "global", "semiglobal"])
 def test_protein_beyond_widest_tile(mode):
     """No comparator fallback: 1500-3300 aa protein queries are striped by the cost model."""
