@@ -264,9 +264,14 @@ def _call_roofline(history, cells: int, kernel_s: float, steps: int, workload: s
     calls = max(1, len(history) // max(steps, 1))
     traffic, src = _call_traffic(workload)
     out = _valu_roof(cells, kernel_s, elem, local)
+    largest = _roofline(history, "local", workload)
+    if largest.get("launch_ms") is not None:
+        largest["note"] = ("launch_ms is the launch's own interval on its stream (it agrees with the "
+                           "rocprofv3 kernel-trace average), which it shares with the other stream's "
+                           "launches, so its frac is below the call's")
     out.update({"traffic": None if traffic is None else round(traffic * calls, 1), "kernel": label,
                 "kernels_ms_per_step": round(kernel_s * 1e3, 2), "calls_per_step": calls,
-                "largest_launch": _roofline(history, "local", workload)})
+                "largest_launch": largest})
     if traffic is not None:
         out["traffic_unit"] = "bytes per step (sum of the per-dispatch rocprofv3 rows of every batched kernel)"
         out["traffic_source"] = src
