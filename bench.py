@@ -27,7 +27,10 @@ before the timed region.  Also reported:
 * ``serving``: Poisson arrivals at one sub-saturation rate (>= 400 requests of 1-512
   queries, 70 % DNA / 30 % protein against both resident databases) through the
   continuous-batching server: p50/p99 latency, GCUPS, and every served top-k checked
-  against the batched top-k of the same queries, plus an oracle sample of served hits.
+  against the batched top-k of the same queries, plus an oracle sample of served hits;
+* ``read_pairs``: the 1,000,000 explicit read/window pairs of ``--workload read_pairs``,
+  semiglobal and global, each with its value, per-batch roofline, plain speedup and
+  oracle sample.
 
 ``--workload`` selects another configuration for extra lines: ``plain_dna`` (the 64 x 256
 int32 case, oracle-checked in full), ``protein_topk`` (BLOSUM62
@@ -401,7 +404,27 @@ def run_dna_cross(args):
             line["protein_topk"] = prot
         if not args.no_serving:
             line["serving"] = serving_leg(args, ((qc, qo), (rc, ro), idx, sc), prot_pool)
+    if not args.no_read_pairs:
+        line["read_pairs"] = read_pairs_leg(args)
     return line
+
+
+def read_pairs_leg(args) -> dict:
+    """The default run's read-pairs key: the 1,000,000-pair workload of `--workload
+    read_pairs` in both modes, each with its value, per-batch roofline, plain speedup and
+    oracle check (the full lines come from that workload)."""
+    out = {}
+    for ln in run_read_pairs(args):
+        mode = "semiglobal" if "semiglobal" in ln["config"]["workload"] else "global"
+        r = ln["roofline"]
+        out[mode] = {"value": ln["value"], "unit": "GCUPS", "ms_per_step": ln["ms_per_step"],
+                     "steps": ln["steps"], "host_call": ln["host_call"], "dtype": ln["dtype"],
+                     "config": ln["config"],
+                     "roofline": {k: r.get(k) for k in ("bound", "achieved", "peak", "unit", "frac", "elem",
+                                                        "slots_per_cell", "traffic", "kernel", "launch_ms",
+                                                        "launch_cells", "all_batches_ms", "launches")},
+                     "plain": ln["plain"], "checks": ln["checks"]}
+    return out
 
 
 def run_plain_dna(args):
@@ -764,6 +787,7 @@ def main() -> int:
                          "Aligner's, a 2 GiB score block, here all 1,000; the reference uses 512)")
     ap.add_argument("--no-protein", action="store_true", help="default run: skip the protein top-k leg")
     ap.add_argument("--no-serving", action="store_true", help="default run: skip the serving leg")
+    ap.add_argument("--no-read-pairs", action="store_true", help="default run: skip the read-pairs leg")
     ap.add_argument("--budget", type=int, default=64, help="serving: device memory budget (GiB)")
     ap.add_argument("--launch-cells", type=float, default=2e11, help="serving: cells per launch")
     ap.add_argument("--plain-queries", type=int, default=32)
