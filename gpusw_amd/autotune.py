@@ -6,8 +6,9 @@ stripes of W columns.  Its built-in cost model picks a tile (or stripes) per que
 from padding, per-step overhead and pipeline fill; this module replaces the model with
 measurements.  For every query-length bin it times each tile
 that covers the bin on a synthetic workload shaped like the caller's database (reference
-lengths drawn from the real ones), for the int16 and int32 element types the scheme can
-use, keeps the fastest, and installs the rules with ``gsw_set_tile_table``.  Results are
+lengths drawn from the real ones), for the packed element type the envelope picks (half2
+or int16x2, dtype "auto"; rules marked ``packed``) and for int32, keeps the fastest, and
+installs the rules with ``gsw_set_tile_table``.  Results are
 cached as JSON keyed by (mode, element type, alphabet size, bin, device).
 """
 from __future__ import annotations
