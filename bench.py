@@ -13,8 +13,8 @@ before the timed region.  Also reported:
   8), times cells, over the device time of the call's batched launches, which overlap on
   two streams (HIP events from the first launch's start to the last one's end), against
   the gfx950 peak of 78.6 T lane-slots/s; ``recurrence_ops_rate`` counts add/sub/max
-  operations instead (local 9, the clamp max(H, 0) folded; the survey's 10 in
-  ``survey_ops_per_cell``), which a three-input max retires two at a time; the largest
+  operations instead (local 9, the clamp max(H, 0) folded; 10 with the clamp counted, in
+  ``clamp_counted_ops_per_cell``), which a three-input max retires two at a time; the largest
   launch is reported beside it;
 * ``plain``: the one-thread-per-pair comparator on the same GPU (a query subsample);
 * ``cpu_baseline``: the oracle's C restatement on the host cores over a bounded sample
@@ -63,10 +63,10 @@ PEAK_SLOTS = 78.6   # T packed-16-bit VALU lane-slots/s: 256 CU x 64 lanes x 2 h
 # int32: one alignment per lane, twice the slots.
 SLOTS_PER_CELL = {("half2", True): 7.5, ("half2", False): 7.0, ("int16x2", True): 9.0,
                   ("int16x2", False): 8.0, ("int32", True): 18.0, ("int32", False): 16.0}
-# add/sub/max operations of the recurrence per cell (local: clamp folded), and the survey's
-# count, which prices max(H, 0) as an operation of its own
+# add/sub/max operations of the recurrence per cell (local: clamp folded), and the count
+# that prices max(H, 0) as an operation of its own
 RECURRENCE_OPS = {True: 9, False: 8}
-SURVEY_OPS = {True: 10, False: 8}
+CLAMP_COUNTED_OPS = {True: 10, False: 8}
 ROOF_UNIT = "T VALU lane-slots/s (packed 16-bit)"
 # arithmetic type of a tile kind, and what the binary16 one means for the scores
 DTYPE_OF = {"half2": "f16 (exact integers)", "int16x2": "int16", "int32": "int32"}
@@ -82,7 +82,7 @@ def _valu_roof(cells: float, seconds: float, elem: str, local: bool) -> dict:
     achieved = cells * slots / seconds / 1e12
     return {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_SLOTS, "unit": ROOF_UNIT,
             "frac": round(achieved / PEAK_SLOTS, 4), "elem": elem, "slots_per_cell": slots,
-            "recurrence_ops_per_cell": RECURRENCE_OPS[local], "survey_ops_per_cell": SURVEY_OPS[local],
+            "recurrence_ops_per_cell": RECURRENCE_OPS[local], "clamp_counted_ops_per_cell": CLAMP_COUNTED_OPS[local],
             "recurrence_ops_rate": round(cells * RECURRENCE_OPS[local] / seconds / 1e12, 3)}
 
 
