@@ -28,6 +28,8 @@ before the timed region.  Also reported:
   queries, 70 % DNA / 30 % protein against both resident databases) through the
   continuous-batching server: p50/p99 latency, GCUPS, and every served top-k checked
   against the batched top-k of the same queries, plus an oracle sample of served hits;
+* ``int16_tiles``: the headline step again with half2 off (``GSW_HALF=0``), every packed
+  pair on the int16x2 tiles: its GCUPS, roofline and equality with the headline scores;
 * ``read_pairs``: the 1,000,000 explicit read/window pairs of ``--workload read_pairs``,
   semiglobal and global, each with its value, per-batch roofline, plain speedup and
   oracle sample.
@@ -348,6 +350,36 @@ def run_dna_cross(args):
     plain_match = bool(np.array_equal(phost, host[psel]))
     log(f"plain: {plain_gcups:.1f} GCUPS over {len(psel)} queries, equal to batched: {plain_match}")
 
+    # ---- the same step on the int16x2 tiles alone (half2 off through GSW_HALF, which the
+    #      library reads per call), outside the headline: its rate, roofline and equality
+    int16 = None
+    if not args.no_int16:
+        prev = os.environ.get("GSW_HALF")
+        os.environ["GSW_HALF"] = "0"
+        try:
+            step()
+            gpusw_amd._hip.call("gsw_synchronize")
+            n16 = max(1, min(args.steps, 2))
+            k16 = []
+            t1 = time.perf_counter()
+            for _ in range(n16):
+                s16, dt16, _i16, _c16 = step()
+                k16.append(dt16)
+            gpusw_amd._hip.call("gsw_synchronize")
+            w16 = (time.perf_counter() - t1) / n16
+            elem16, _loc = _dominant_kind([b.last_stats])
+        finally:
+            if prev is None:
+                os.environ.pop("GSW_HALF", None)
+            else:
+                os.environ["GSW_HALF"] = prev
+        int16 = {"value": round(cells / w16 / 1e9, 2), "unit": "GCUPS", "ms_per_step": round(w16 * 1e3, 2),
+                 "steps": n16, "dtype": DTYPE_OF[elem16],
+                 "roofline": _valu_roof(cells, float(np.mean(k16)), elem16, True),
+                 "scores_equal_half2": bool(np.array_equal(s16.to_host(), host)),
+                 "what": "the headline step with GSW_HALF=0: every packed pair on the int16x2 tiles"}
+        log(f"int16 tiles: {int16['value']} GCUPS, scores equal: {int16['scores_equal_half2']}")
+
     # ---- traceback of the top-5 hits of 512 queries (outside the timed region)
     tq = np.repeat(np.arange(512, dtype=np.int32), 5)
     tr = np.ascontiguousarray(idx[:512].reshape(-1), dtype=np.int32)
@@ -396,6 +428,7 @@ def run_dna_cross(args):
         "checks": {"topk_equals_stable_sort": topk_ok, "plain_equals_batched": plain_match,
                    "oracle_mismatches": mism, "rescored": int(stats.rescored)},
         "traceback": traceback,
+        "int16_tiles": int16,
         "cpu_baseline": cpu,
     }
     if not args.no_protein or not args.no_serving:
@@ -788,6 +821,7 @@ def main() -> int:
     ap.add_argument("--no-protein", action="store_true", help="default run: skip the protein top-k leg")
     ap.add_argument("--no-serving", action="store_true", help="default run: skip the serving leg")
     ap.add_argument("--no-read-pairs", action="store_true", help="default run: skip the read-pairs leg")
+    ap.add_argument("--no-int16", action="store_true", help="default run: skip the int16-tiles leg")
     ap.add_argument("--budget", type=int, default=64, help="serving: device memory budget (GiB)")
     ap.add_argument("--launch-cells", type=float, default=2e11, help="serving: cells per launch")
     ap.add_argument("--plain-queries", type=int, default=32)
