@@ -124,9 +124,13 @@ int gsw_seqset_destroy(gsw_seqset set);
 
 /* ---- batched wavefront path (backend="hip") ------------------------------
  * Same contract and outputs as the plain calls above (int32 scores, row-major
- * (nq, nr) for cross), computed by the packed-int16 wavefront kernels with int32
- * fallback.  dtype_policy is GSW_DTYPE_AUTO/INT16/INT32 (src/gpusw/scheme.py:116-141
- * decides it on the host; AUTO lets the library pick per pair). */
+ * (nq, nr) for cross), computed by the wavefront kernels: two alignments per lane on
+ * half2 tiles (binary16, exact for the integers in [-2048, 2048]; local results of 2048
+ * or more rescored in int32) or packed int16 tiles, int32 tiles where the score envelope
+ * needs them.  Queries wider than every tile run as stripes; explicit pairs longer than
+ * every tile on both sides run on the int32 stripe kernels.  dtype_policy is
+ * GSW_DTYPE_AUTO/INT16/INT32 (src/gpusw/scheme.py:116-141 decides it on the host; AUTO
+ * lets the library pick per pair). */
 int gsw_score_cross(const gsw_scheme *s, int32_t dtype_policy,
                     gsw_seqset queries, const int32_t *h_qlist, int32_t nq,
                     gsw_seqset refs, int32_t *d_out, gsw_stats *stats);
