@@ -32,6 +32,13 @@ int fail(int code, const char *fmt, ...);
     } while (0)
 
 // ---------------------------------------------------------------- device context
+// Device counters of a context: [1] the saturation count, [2, 2 + MAX_GROUPS) one work
+// counter per launch of a call, [NCOUNTERS - 1] the max-code reduction.  A call forms at
+// most one launch per (variant, striped), and the variant table holds fewer than
+// MAX_GROUPS / 2 rows of one mode, so the group limit is never what stops a call.
+constexpr int NCOUNTERS = 512;
+constexpr int MAX_GROUPS = NCOUNTERS - 3;
+
 // One library stream and a set of grow-only scratch buffers per device.
 struct DeviceCtx {
     int device = -1;
@@ -47,7 +54,7 @@ struct DeviceCtx {
     void *meta = nullptr;      size_t meta_bytes = 0;      // task lists, lists of ids
     void *meta2 = nullptr;     size_t meta2_bytes = 0;
     void *bnd = nullptr;       size_t bnd_bytes = 0;       // stripe boundaries
-    int32_t *counters = nullptr;                            // 64 int32 counters
+    int32_t *counters = nullptr;                            // NCOUNTERS int32 counters
     std::vector<hipEvent_t> marks;                          // per-launch timing events
     std::mutex mu;                                          // serialises calls on this device
 };

@@ -169,6 +169,8 @@ int run_pass(NoAllocDevice &dev, const SchemeDev &s, int dtype_policy, const Seq
         launches.push_back(Launch{&v, t0, (int64_t)tasks.size()});
     }
     if ((int64_t)tasks.size() > MAX_TASKS) return fail(GSW_ERR_LIMIT, "task list exceeds the scratch array");
+    if ((int)launches.size() > SAT_SLOT)   // header words [0, SAT_SLOT) are the launches' work counters
+        return fail(GSW_ERR_LIMIT, "%zu tiles exceed the scratch array's %d work counters", launches.size(), SAT_SLOT);
     // -- inputs into the array (its counters start at zero), then the launches
     static const int32_t zeros[HDR_BYTES / sizeof(int32_t)] = {};
     char *base = dev.ptr;

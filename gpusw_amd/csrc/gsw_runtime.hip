@@ -75,7 +75,7 @@ int ctx_get(DeviceCtx **out) {
     GSW_HIP_TRY(hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking));
     GSW_HIP_TRY(hipEventCreate(&ctx->ev0));
     GSW_HIP_TRY(hipEventCreate(&ctx->ev1));
-    GSW_HIP_TRY(hipMalloc(&ctx->counters, 64 * sizeof(int32_t)));
+    GSW_HIP_TRY(hipMalloc(&ctx->counters, NCOUNTERS * sizeof(int32_t)));
     *out = ctx.get();
     g_ctx[dev] = std::move(ctx);
     return GSW_OK;
@@ -371,13 +371,13 @@ int gsw_seqset_create(const uint8_t *d_codes, const int32_t *d_off, const int32_
     // the largest code: explicit-pairs profiles need rows only up to it (codes at or past the
     // table side read its last row)
     if (set->total > 0) {
-        GSW_HIP_TRY(hipMemsetAsync(ctx->counters + 63, 0, sizeof(int32_t), ctx->stream));
+        GSW_HIP_TRY(hipMemsetAsync(ctx->counters + (NCOUNTERS - 1), 0, sizeof(int32_t), ctx->stream));
         const int64_t first = h_off[0];
         hipLaunchKernelGGL(max_code_kernel, dim3((unsigned)std::min<int64_t>(1024, (set->total + 255) / 256)),
-                           dim3(256), 0, ctx->stream, d_codes + first, set->total, ctx->counters + 63);
+                           dim3(256), 0, ctx->stream, d_codes + first, set->total, ctx->counters + (NCOUNTERS - 1));
         GSW_HIP_TRY(hipGetLastError());
         int32_t mc = 0;
-        GSW_HIP_TRY(hipMemcpyAsync(&mc, ctx->counters + 63, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+        GSW_HIP_TRY(hipMemcpyAsync(&mc, ctx->counters + (NCOUNTERS - 1), sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
         GSW_HIP_TRY(hipStreamSynchronize(ctx->stream));
         set->max_code = mc;
     } else {

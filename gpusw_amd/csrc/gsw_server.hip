@@ -594,7 +594,7 @@ int gsw_server_create(size_t mem_budget_bytes, double launch_cells, gsw_server *
     GSW_HIP_TRY(hipEventCreate(&srv->ctx.ev0));
     GSW_HIP_TRY(hipEventCreate(&srv->ctx.ev1));
     GSW_HIP_TRY(hipEventCreateWithFlags(&srv->codes_ready, hipEventDisableTiming));
-    GSW_HIP_TRY(hipMalloc(&srv->ctx.counters, 64 * sizeof(int32_t)));
+    GSW_HIP_TRY(hipMalloc(&srv->ctx.counters, NCOUNTERS * sizeof(int32_t)));
     if (launch_cells > 0) srv->launch_cells = launch_cells;
     // query codes are addressed with 32-bit offsets: at most 1 GiB of them
     const size_t budget = std::max<size_t>(mem_budget_bytes, size_t(64) << 20);

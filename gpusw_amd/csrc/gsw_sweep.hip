@@ -693,7 +693,7 @@ int batch_cross(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
     }
     // two boundary regions: consecutive striped launches may run at the same time
     if (bnd_entries > 0) GSW_TRY(ensure_buffer(&ctx->bnd, &ctx->bnd_bytes, 4 * sizeof(uint32_t) * (size_t)bnd_entries));
-    if (groups.size() > 60) return fail(GSW_ERR_LIMIT, "too many tile groups (%zu)", groups.size());
+    if (groups.size() > (size_t)MAX_GROUPS) return fail(GSW_ERR_LIMIT, "too many tile groups (%zu)", groups.size());
     // Launches alternate between the library stream and the second stream, so each one's
     // workgroups take the CUs as the previous launch's last workgroups finish instead of
     // waiting for all of them.  Every launch is timed by events right before and after it
@@ -986,7 +986,7 @@ static int batch_long_pairs(DeviceCtx *ctx, const SchemeDev &s, const SeqSet &qs
         launches.push_back(L);
         g0 = g1;
     }
-    if (launches.size() > 60) return fail(GSW_ERR_LIMIT, "too many tile groups (%zu)", launches.size());
+    if (launches.size() > (size_t)MAX_GROUPS) return fail(GSW_ERR_LIMIT, "too many tile groups (%zu)", launches.size());
     GSW_TRY(ensure_buffer(&ctx->bnd, &ctx->bnd_bytes, sizeof(uint32_t) * (size_t)bnd_words));
 
     // one device block for the items and the position arrays, freed after the launches
