@@ -9,8 +9,9 @@ before the timed region.  Also reported:
 
 * ``roofline``: packed 16-bit VALU roofline of the cross kernels -- the VALU lane-slots the
   recurrence needs per cell on the tiles' element type (half2: 7.5 in local mode = 4 adds,
-  3 three-input maxima and half a maximum for the running best, 7 otherwise; int16x2: 9 and
-  8), times cells, over the device time of the call's batched launches, which overlap on
+  3 three-input maxima and half a maximum for the running best, 5 otherwise in the
+  diagonal-offset form, whose plain-form 7 is ``plain_form_frac``; int16x2: 9 and 6), times
+  cells, over the device time of the call's batched launches, which overlap on
   two streams (HIP events from the first launch's start to the last one's end), against
   the gfx950 peak of 78.6 T lane-slots/s; ``recurrence_ops_rate`` counts add/sub/max
   operations instead (local 9, the clamp max(H, 0) folded; 10 with the clamp counted, in
@@ -59,15 +60,18 @@ METRIC = "bit-exact GCUPS on 1 MI355X (batched vs plain); p50/p99 latency when s
 PEAK_SLOTS = 78.6   # T packed-16-bit VALU lane-slots/s: 256 CU x 64 lanes x 2 halves x 2.4 GHz
 # VALU lane-slots per cell update, by tile element kind and local mode.  half2 (binary16
 # pairs): 4 adds + E, H, F each one v_pk_maximum3_f16 (local: clamps at 0 as third operand)
-# + the running best over two columns per three-input max (local) = 7.5; global/semiglobal
-# 4 adds + 2 two-input maxima + 1 three-input = 7.  int16x2: local 4 add/sub + 5 max = 9
-# (E, F and H-gap kept >= 0 by saturating subtraction, so max(H, 0) costs nothing), else 8.
-# int32: one alignment per lane, twice the slots.
-SLOTS_PER_CELL = {("half2", True): 7.5, ("half2", False): 7.0, ("int16x2", True): 9.0,
-                  ("int16x2", False): 8.0, ("int32", True): 18.0, ("int32", False): 16.0}
-# add/sub/max operations of the recurrence per cell (local: clamp folded), and the count
-# that prices max(H, 0) as an operation of its own
-RECURRENCE_OPS = {True: 9, False: 8}
+# + the running best over two columns per three-input max (local) = 7.5.  Global/semiglobal
+# packed tiles run the diagonal-offset form (values carried as X + (i+j)*D, D = -gap_extend:
+# extensions cost nothing), 2 adds + E, F and H maxima = 5.  int16x2: local 4 add/sub + 5
+# max = 9 (E, F and H-gap kept >= 0 by saturating subtraction, so max(H, 0) costs nothing),
+# offset form 2 adds + 4 max = 6.  int32 (plain form, one alignment per lane): twice 9 and 8.
+SLOTS_PER_CELL = {("half2", True): 7.5, ("half2", False): 5.0, ("int16x2", True): 9.0,
+                  ("int16x2", False): 6.0, ("int32", True): 18.0, ("int32", False): 16.0}
+# the plain (not offset) form's global/semiglobal slots, reported beside the roofline
+PLAIN_FORM_SLOTS = {"half2": 7.0, "int16x2": 8.0, "int32": 16.0}
+# add/sub/max operations of the recurrence per cell (local: clamp folded; otherwise the
+# offset form), and the plain Gotoh count (local: max(H, 0) as an operation of its own)
+RECURRENCE_OPS = {True: 9, False: 6}
 CLAMP_COUNTED_OPS = {True: 10, False: 8}
 ROOF_UNIT = "T VALU lane-slots/s (packed 16-bit)"
 # arithmetic type of a tile kind, and what the binary16 one means for the scores
@@ -85,7 +89,9 @@ def _valu_roof(cells: float, seconds: float, elem: str, local: bool) -> dict:
     return {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_SLOTS, "unit": ROOF_UNIT,
             "frac": round(achieved / PEAK_SLOTS, 4), "elem": elem, "slots_per_cell": slots,
             "recurrence_ops_per_cell": RECURRENCE_OPS[local], "clamp_counted_ops_per_cell": CLAMP_COUNTED_OPS[local],
-            "recurrence_ops_rate": round(cells * RECURRENCE_OPS[local] / seconds / 1e12, 3)}
+            "recurrence_ops_rate": round(cells * RECURRENCE_OPS[local] / seconds / 1e12, 3),
+            **({} if local else {"plain_form_slots_per_cell": PLAIN_FORM_SLOTS[elem],
+                                 "plain_form_frac": round(cells * PLAIN_FORM_SLOTS[elem] / seconds / 1e12 / PEAK_SLOTS, 4)})}
 
 
 def _dominant_kind(history) -> tuple[str, bool]:
@@ -454,7 +460,8 @@ def read_pairs_leg(args) -> dict:
                      "steps": ln["steps"], "host_call": ln["host_call"], "dtype": ln["dtype"],
                      "config": ln["config"],
                      "roofline": {k: r.get(k) for k in ("bound", "achieved", "peak", "unit", "frac", "elem",
-                                                        "slots_per_cell", "traffic", "kernel", "launch_ms",
+                                                        "slots_per_cell", "plain_form_slots_per_cell",
+                                                        "plain_form_frac", "traffic", "kernel", "launch_ms",
                                                         "launch_cells", "all_batches_ms", "launches")},
                      "plain": ln["plain"], "checks": ln["checks"]}
     return out

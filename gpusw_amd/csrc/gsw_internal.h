@@ -95,6 +95,12 @@ bool use_int16(const SchemeDev &s, int dtype_policy, int lq, int lr);
 bool half_entries(const SchemeDev &s);
 bool fits_half(const SchemeDev &s, int lq, int lr);
 bool half_enabled();
+// The envelopes of the packed batched sweeps, which carry global/semiglobal values in the
+// diagonal-offset form (gsw_sweep.h): local mode as fits_int16/fits_half; otherwise every
+// offset value of the pair's real cells fits the type.  (fits_int16 keeps the plain values'
+// envelope for the one-pair-per-thread comparator's int16 rows.)
+bool sweep_fits_int16(const SchemeDev &s, int lq, int lr);
+bool sweep_fits_half(const SchemeDev &s, int lq, int lr);
 // Internal dtype policy: packed int16 where GSW_DTYPE_INT16 would use it, never half2 (the
 // rerun of local work whose half2 results overflowed the saturation list).
 constexpr int GSW_DTYPE_PK16 = 16;

@@ -61,9 +61,9 @@ __attribute__((amdgpu_waves_per_eu(WavesPerEU<LOCAL, G, R>::value))) cross_kerne
             const int lt = rem / LW, kk = rem - lt * LW;
             const int col = lt * R + kk - item.shift;   // front padding (col < 0) scores 0
             const int va = kk >= R || col >= lqa ? O::NEG
-                           : col < 0 ? 0 : a.sub[c * a.ns + min((int)a.qcodes[qsa + col], a.ns - 1)];
+                           : prof_score<V, LOCAL>(a.k, col < 0 ? 0 : a.sub[c * a.ns + min((int)a.qcodes[qsa + col], a.ns - 1)]);
             const int vb = kk >= R || col >= lqb ? O::NEG
-                           : col < 0 ? 0 : a.sub[c * a.ns + min((int)a.qcodes[qsb + col], a.ns - 1)];
+                           : prof_score<V, LOCAL>(a.k, col < 0 ? 0 : a.sub[c * a.ns + min((int)a.qcodes[qsb + col], a.ns - 1)]);
             const uint32_t word = O::pack(va, vb);
             char *dst = lds + c * L::ROWB + (kk >> 2) * L::CH + lt * 16 + (kk & 3) * 4;
 #pragma unroll
@@ -136,9 +136,9 @@ __attribute__((amdgpu_waves_per_eu(WavesPerEU<LOCAL, G, R>::value))) stripe_kern
                 const int lt = rem / LW, kk = rem - lt * LW;
                 const int col = cbase + lt * R + kk;   // front padding (col < 0) scores 0
                 const int va = kk >= R || col >= lqa ? O::NEG
-                               : col < 0 ? 0 : a.sub[c * a.ns + min((int)a.qcodes[qsa + col], a.ns - 1)];
+                               : prof_score<V, LOCAL>(a.k, col < 0 ? 0 : a.sub[c * a.ns + min((int)a.qcodes[qsa + col], a.ns - 1)]);
                 const int vb = kk >= R || col >= lqb ? O::NEG
-                               : col < 0 ? 0 : a.sub[c * a.ns + min((int)a.qcodes[qsb + col], a.ns - 1)];
+                               : prof_score<V, LOCAL>(a.k, col < 0 ? 0 : a.sub[c * a.ns + min((int)a.qcodes[qsb + col], a.ns - 1)]);
                 const uint32_t word = O::pack(va, vb);
                 char *dst = lds + c * L::ROWB + (kk >> 2) * L::CH + lt * 16 + (kk & 3) * 4;
 #pragma unroll
@@ -243,7 +243,7 @@ __global__ void __launch_bounds__(WAVES * 64) pairs_kernel(KArgs a) {
         auto entry = [&](int c, int code) -> int {
             const int v = lane_table ? __builtin_amdgcn_ds_bpermute((c * a.ns + max(code, 0)) * 4, subv)
                                      : a.sub[c * a.ns + max(code, 0)];
-            return code == -1 ? O::NEG : code == -2 ? 0 : v;
+            return code == -1 ? O::NEG : prof_score<V, LOCAL>(a.k, code == -2 ? 0 : v);
         };
 #pragma unroll 1
         for (int c = 0; c < nrows; ++c) {
@@ -290,8 +290,8 @@ __global__ void __launch_bounds__(WAVES * 64) pairs_tab_kernel(KArgs a) {
         const int w = idx / copies;
         const int c = w / row_words, rem = w - c * row_words;
         const int qa = O::PACKED ? rem / nq : rem, qb = O::PACKED ? rem - (rem / nq) * nq : rem;
-        const int va = qa == npad ? O::NEG : qa == zpad ? 0 : a.sub[c * ns + qa];
-        const int vb = qb == npad ? O::NEG : qb == zpad ? 0 : a.sub[c * ns + qb];
+        const int va = qa == npad ? O::NEG : prof_score<V, LOCAL>(a.k, qa == zpad ? 0 : a.sub[c * ns + qa]);
+        const int vb = qb == npad ? O::NEG : prof_score<V, LOCAL>(a.k, qb == zpad ? 0 : a.sub[c * ns + qb]);
         reinterpret_cast<uint32_t *>(lds)[idx] = O::pack(va, vb);
     }
     __syncthreads();

@@ -123,7 +123,7 @@ int run_pass(NoAllocDevice &dev, const SchemeDev &s, int dtype_policy, const Seq
         if (lq > wmax)
             return fail(GSW_ERR_LIMIT, "pair %lld: query of %d columns; this entry point scores queries of up to %d",
                         (long long)i, lq, wmax);
-        const bool use16 = dtype_policy == GSW_DTYPE_INT32 ? false : local16 ? true : fits_int16(s, lq, lr);
+        const bool use16 = dtype_policy == GSW_DTYPE_INT32 ? false : local16 ? true : sweep_fits_int16(s, lq, lr);
         bool tab = false;
         const Variant *v = pick_pairs_tile(lq, mean_lr, use16 ? KIND_I16 : KIND_I32, local, s.ns, 0,
                                            (size_t)dev.info.max_smem, &tab);

@@ -205,6 +205,35 @@ bool fits_half(const SchemeDev &s, int lq, int lr) {
     return lo - margin >= -2048 && hi + std::max(s.sub_max, 0) + 16 <= 2047;
 }
 
+// The packed sweeps carry global/semiglobal values in the diagonal-offset form X + (i+j)*D,
+// D = -gap_extend (gsw_sweep.h).  Every value they compute for a real cell lies in [lo, hi]:
+//   H >= min(0, go + D, 2go + 2D, -64D): the two-gap path from the corner gives
+//        H(i,j) >= 2go + (i+j-2)*ge, the boundaries are go + D, i*D or j*D, and front
+//        padding (fewer than 64 columns) carries (i+j)*D with j > -64;
+//   H <= max(sub,0)*min(lq,lr) + (lq+lr+64)*D: at most min(lq,lr) substitutions score;
+//   the sums diag + s + 2D and H + go + D, and E and F (maxima of the latter), lie within
+//   H's range widened by min/max(0, sub + 2D, go + D).
+// The last-column maximum of semiglobal free query ends is taken over H(i, lq) + lq*D, which
+// is >= go + D (a gap from the free column-0 boundary) and <= hi, so it is exact too.
+static bool offset_fits(const SchemeDev &s, int lq, int lr, int64_t vmin, int64_t vmax) {
+    const int64_t D = -(int64_t)s.ge, go = s.go, smin = s.sub_min, smax = s.sub_max;
+    const int64_t hmin = std::min<int64_t>({0, go + D, 2 * go + 2 * D, -64 * D});
+    const int64_t hmax = (int64_t)std::min(lq, lr) * std::max<int64_t>(smax, 0) + ((int64_t)lq + lr + 64) * D;
+    const int64_t lo = hmin + std::min<int64_t>({0, smin + 2 * D, go + D}) - 16;
+    const int64_t hi = hmax + std::max<int64_t>({0, smax + 2 * D, go + D}) + 16;
+    return lo >= vmin && hi <= vmax;
+}
+
+bool sweep_fits_int16(const SchemeDev &s, int lq, int lr) {
+    if (s.mode == GSW_MODE_LOCAL) return fits_int16(s, lq, lr);
+    return int16_entries(s) && offset_fits(s, lq, lr, -32767, 32766);
+}
+
+bool sweep_fits_half(const SchemeDev &s, int lq, int lr) {
+    if (s.mode == GSW_MODE_LOCAL) return fits_half(s, lq, lr);
+    return half_entries(s) && offset_fits(s, lq, lr, -2047, 2047);
+}
+
 bool half_enabled() {
     const char *e = std::getenv("GSW_HALF");
     return !(e && e[0] == '0');

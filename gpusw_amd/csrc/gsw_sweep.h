@@ -207,6 +207,28 @@ __device__ __forceinline__ int col0_value(const SweepConst &k, int row) {
     return 0;
 }
 
+// Diagonal-offset form of the global/semiglobal recurrence, used by the packed types.
+// Every value of cell (i, j) -- reference row i and query column j, both 1-based, j <= 0
+// on front padding -- is carried as X + (i + j)*D with D = -gap_extend.  An extension
+// (+gap_extend) and a step right or down (+D) then cancel, and with P = H + gap_open + D:
+//   E(i,j) = max(E(i,j-1), P(i,j-1))          F(i,j) = max(F(i-1,j), P(i-1,j))
+//   H(i,j) = max(H(i-1,j-1) + s + 2D, E(i,j), F(i,j))
+// which is five instructions per cell instead of seven on half2 (six instead of eight on
+// int16): the profile holds s + 2D, and P is shared by the E of the next column and the F
+// of the next row.  Row 0 and the column-0 boundary become gap_open + D on every row or
+// column (the gap ladders) or (i + j)*D (free ends, front padding); results subtract
+// (i + j)*D in int32.  The host admits a pair to a packed type only when every offset value
+// of its real cells is exact in it (sweep_fits_half, sweep_fits_int16).
+template <typename V, bool LOCAL> struct DiagOffset {
+    static constexpr bool ON = !LOCAL && VOps<V>::PACKED;
+};
+
+// Profile entry of substitution score v (front padding: v = 0), s + 2D in the offset form.
+template <typename V, bool LOCAL>
+__device__ __forceinline__ int prof_score(const SweepConst &k, int v) {
+    return DiagOffset<V, LOCAL>::ON ? v - 2 * k.ge : v;
+}
+
 constexpr int RES_NONE = -0x40000000 - 7;   // below every reachable result
 
 // H(i, j-1) + gap_open for a row's first column.  The int16 local form clamps it at 0
@@ -244,6 +266,19 @@ __device__ __forceinline__ void cell_max3(int kk, V xd, V &e, V &h, V &hprev, V 
         hgo = O::add(h, GOV);
         f = O::mx(O::add(f, GEV), hgo);
     }
+}
+
+// One cell of the offset form (DiagOffset): hgo holds P of the cell to the left on entry
+// and this cell's P on return; f holds F of this cell on entry and of the cell below on
+// return.  Half2: the add of xd, the three-input max and three more instructions.
+template <typename V>
+__device__ __forceinline__ void cell_offset(V xd, V &e, V &h, V &hup, V &hgo, V &f, V GOD) {
+    using O = VOps<V>;
+    e = O::mx(e, hgo);
+    h = O::MAX3 ? O::mx3(xd, f, e) : O::mx(O::mx(xd, f), e);
+    hup = h;
+    hgo = O::add(h, GOD);
+    f = O::mx(f, hgo);
 }
 
 // Per-group results after a sweep.
@@ -295,20 +330,27 @@ __device__ __forceinline__ SweepOut sweep_group(const uint8_t *__restrict__ rseq
     const V GOV = O::splat(k.go), GEV = O::splat(k.ge);
     const V GOP = O::splat(-k.go), GEP = O::splat(-k.ge);
     const V ZERO = O::splat(0);
+    // the offset form's D (0: plain values); H(0, col) and H(row, cbase) as carried
+    constexpr bool OFS = DiagOffset<V, LOCAL>::ON;
+    const int D = OFS ? -k.ge : 0;
+    const V GOD = O::splat(k.go + D), NDV = O::splat(-D);
+    auto row0 = [&](int col) { return row0_value(k, col) + col * D; };
+    auto col0 = [&](int row) { return col0_value(k, row) + (row + cbase) * D; };
 #pragma unroll
     for (int kk = 0; kk < R; ++kk) {
         if (LOCAL) {
             hup[kk] = ZERO;
             fn[kk] = ZERO;
         } else {
-            const int h0 = row0_value(k, cbase + t * R + kk + 1);
+            const int h0 = row0(cbase + t * R + kk + 1);
             hup[kk] = O::splat(h0);
-            fn[kk] = O::splat(max(h0 + k.go, O::NEG + k.ge));   // F(1, col)
+            // F(1, col)
+            fn[kk] = O::splat(OFS ? max(h0 + k.go + D, O::NEG) : max(h0 + k.go, O::NEG + k.ge));
         }
     }
-    V hout = LOCAL ? ZERO : O::splat(row0_value(k, cbase + (t + 1) * R));
-    V hin = LOCAL ? ZERO : O::splat(t == 0 ? col0_value(k, 1) : row0_value(k, cbase + t * R));
-    V din = LOCAL ? ZERO : O::splat(row0_value(k, cbase));   // H(0, left column), lane 0
+    V hout = LOCAL ? ZERO : O::splat(row0(cbase + (t + 1) * R));
+    V hin = LOCAL ? ZERO : O::splat(t == 0 ? col0(1) : row0(cbase + t * R));
+    V din = LOCAL ? ZERO : O::splat(row0(cbase));   // H(0, left column), lane 0
     V ein = LOCAL ? ZERO : O::splat(O::NEG);
     V eout = ein;
     // boundary word i of a plane: uniform base + 32-bit byte offset (SGPR-base addressing)
@@ -321,6 +363,7 @@ __device__ __forceinline__ SweepOut sweep_group(const uint8_t *__restrict__ rseq
     }
     V best = ZERO;
     V endcol = O::splat(O::NEG);
+    V rowoff = O::splat((t - 1) * D);   // -i*D of the lane's row i (1-based) in this step
     int corner_a = RES_NONE, corner_b = RES_NONE, endrow_a = RES_NONE, endrow_b = RES_NONE;
 
     // out-of-range codes score as the sentinel
@@ -345,12 +388,17 @@ __device__ __forceinline__ SweepOut sweep_group(const uint8_t *__restrict__ rseq
     };
     // column-0 boundary H(i, 0) of lane 0's next row, kept as a running (saturating) value
     const bool ladder = !LOCAL && (k.mode == 1 || (k.mode == 2 && !k.free_q));
-    V c0v = ladder ? O::splat(col0_value(k, 2)) : ZERO;
+    // lane 0's boundary H(i, cbase) of its next row and its change per row: the plain gap
+    // ladder steps by gap_extend; in the offset form the ladder is constant and free query
+    // ends step by D
+    V c0v = OFS ? O::splat(col0(2)) : ladder ? O::splat(col0_value(k, 2)) : ZERO;
+    const bool c0_steps = OFS ? !ladder : ladder;
     // Row shifts of ZERO_FILL tiles give every lane t > 0 lane t-1's H and E of the row.
     // Lane 0 takes the column-0 boundary: local H = E = 0 (the zero fill itself); otherwise
     // E stays the sentinel it holds and H steps down the column-0 ladder by gap_extend (or
-    // stays 0 with free query ends), so each is the kept old value of one DPP move.
-    const V LADV = ladder ? GEV : ZERO;
+    // stays 0 with free query ends; in the offset form the ladder stays and free ends step
+    // by D), so each is the kept old value of one DPP move.
+    const V LADV = OFS ? (ladder ? ZERO : O::splat(D)) : ladder ? GEV : ZERO;
     auto shift_boundary = [&](V &hi, V &ei, V ho, V eo) {
         if constexpr (!GroupLanes<G>::ZERO_FILL) {
             (void)hi, (void)ei, (void)ho, (void)eo;
@@ -392,8 +440,9 @@ __device__ __forceinline__ SweepOut sweep_group(const uint8_t *__restrict__ rseq
 #pragma unroll
             for (int kk = 0; kk < R; ++kk) {
                 const int col = cbase + t * R + kk;
-                const int va = O::lo(hup[kk]);
-                const int vb = O::hi(hup[kk]);
+                const int off = (lr + col + 1) * D;   // offset form: H(lr, col+1) + off
+                const int va = O::lo(hup[kk]) - off;
+                const int vb = O::hi(hup[kk]) - off;
                 if (col == lqa - 1) corner_a = va;
                 if (col == lqb - 1) corner_b = vb;
                 if (col < lqa) endrow_a = max(endrow_a, va);
@@ -405,8 +454,8 @@ __device__ __forceinline__ SweepOut sweep_group(const uint8_t *__restrict__ rseq
         if (semi) {
             if (track_endcol) {
                 // the lane whose last register is column lq-1 (none in other stripes)
-                if (cbase + t * R + R - 1 == lqa - 1) ca = max(ca, O::lo(endcol));
-                if (cbase + t * R + R - 1 == lqb - 1) cb = max(cb, O::hi(endcol));
+                if (cbase + t * R + R - 1 == lqa - 1) ca = max(ca, O::lo(endcol) - lqa * D);
+                if (cbase + t * R + R - 1 == lqb - 1) cb = max(cb, O::hi(endcol) - lqb * D);
             }
             if (k.free_r) {
                 ca = max(ca, endrow_a);
@@ -426,7 +475,7 @@ __device__ __forceinline__ SweepOut sweep_group(const uint8_t *__restrict__ rseq
     // One row of the lane's R columns from its profile row pv.
     auto row_cells = [&](const V *pv) {
         V e = ein;
-        V hgo = row_hgo<V, LOCAL>(hin, GOV, GOP);
+        V hgo = OFS ? O::add(hin, GOD) : row_hgo<V, LOCAL>(hin, GOV, GOP);
         V h = hin, hprev = ZERO;
         // xd = H(i-1, j-1) + s(i, j).  The sum for column kk+1 is taken before
         // hup[kk] is overwritten, so H(i, j) can replace H(i-1, j) in place.
@@ -434,7 +483,9 @@ __device__ __forceinline__ SweepOut sweep_group(const uint8_t *__restrict__ rseq
 #pragma unroll
         for (int kk = 0; kk < R; ++kk) {
             const V xn = (kk + 1 < R) ? O::add(hup[kk], pv[kk + 1]) : xd;
-            if constexpr (O::MAX3) {
+            if constexpr (OFS) {
+                cell_offset<V>(xd, e, h, hup[kk], hgo, fn[kk], GOD);
+            } else if constexpr (O::MAX3) {
                 cell_max3<V, LOCAL, R>(kk, xd, e, h, hprev, best, hup[kk], hgo, fn[kk], GOV, GEV, ZERO);
             } else if (LOCAL) {
                 e = O::mx(O::subu(e, GEP), hgo);
@@ -454,7 +505,8 @@ __device__ __forceinline__ SweepOut sweep_group(const uint8_t *__restrict__ rseq
         }
         hout = h;
         eout = e;
-        if (!LOCAL && track_endcol) endcol = O::mx(endcol, h);   // lane's last column
+        // lane's last column (offset form: H(i, j) + j*D, the row's i*D taken off)
+        if (!LOCAL && track_endcol) endcol = O::mx(endcol, OFS ? O::add(h, rowoff) : h);
     };
     // One step of a tile (not a stripe): lane t on row s - t.  GUARD: rows outside [0, lr)
     // are skipped (the pipeline fill and drain); without it every lane of the wave is on a
@@ -477,6 +529,7 @@ __device__ __forceinline__ SweepOut sweep_group(const uint8_t *__restrict__ rseq
             c = (int)rseq[min(max(row + 2, 0), rlast)];
         }
         din = hin;
+        if constexpr (OFS) rowoff = O::add(rowoff, NDV);
         if constexpr (GroupLanes<G>::ZERO_FILL) {
             shift_boundary(hin, ein, hout, eout);
         } else {
@@ -486,7 +539,7 @@ __device__ __forceinline__ SweepOut sweep_group(const uint8_t *__restrict__ rseq
                 hin = c0v;
                 ein = LOCAL ? ZERO : O::splat(O::NEG);
             }
-            if (ladder) c0v = O::add(c0v, GEV);
+            if (c0_steps) c0v = O::add(c0v, LADV);
         }
     };
     // Steps from G-1 to the shortest active reference of the wave have every active lane on
@@ -521,6 +574,7 @@ __device__ __forceinline__ SweepOut sweep_group(const uint8_t *__restrict__ rseq
                 *bnd(be, boff + (uint32_t)row) = (uint32_t)O::bits(eout);
             }
             din = hin;
+            if constexpr (OFS) rowoff = O::add(rowoff, NDV);
             hin = O::from_bits(GroupLanes<G>::shift(O::bits(hout)));
             ein = O::from_bits(GroupLanes<G>::shift(O::bits(eout)));
             if (t == 0) {
@@ -532,7 +586,7 @@ __device__ __forceinline__ SweepOut sweep_group(const uint8_t *__restrict__ rseq
                     ein = LOCAL ? ZERO : O::splat(O::NEG);
                 }
             }
-            if (ladder) c0v = O::add(c0v, GEV);
+            if (c0_steps) c0v = O::add(c0v, LADV);
         }
         return finish();
     }
@@ -577,31 +631,44 @@ __device__ __forceinline__ SweepOut sweep_pairs_pipe(const uint8_t *__restrict__
     const V GOV = O::splat(k.go), GEV = O::splat(k.ge);
     const V GOP = O::splat(-k.go), GEP = O::splat(-k.ge);
     const V ZERO = O::splat(0);
+    // the offset form's D (0: plain values); H(0, col) and H(row, cbase) as carried
+    constexpr bool OFS = DiagOffset<V, LOCAL>::ON;
+    const int D = OFS ? -k.ge : 0;
+    const V GOD = O::splat(k.go + D), NDV = O::splat(-D);
+    auto row0 = [&](int col) { return row0_value(k, col) + col * D; };
+    auto col0 = [&](int row) { return col0_value(k, row) + (row + cbase) * D; };
 #pragma unroll
     for (int kk = 0; kk < R; ++kk) {
         if (LOCAL) {
             hup[kk] = ZERO;
             fn[kk] = ZERO;
         } else {
-            const int h0 = row0_value(k, cbase + t * R + kk + 1);
+            const int h0 = row0(cbase + t * R + kk + 1);
             hup[kk] = O::splat(h0);
-            fn[kk] = O::splat(max(h0 + k.go, O::NEG + k.ge));   // F(1, col)
+            // F(1, col)
+            fn[kk] = O::splat(OFS ? max(h0 + k.go + D, O::NEG) : max(h0 + k.go, O::NEG + k.ge));
         }
     }
-    V hout = LOCAL ? ZERO : O::splat(row0_value(k, cbase + (t + 1) * R));
-    V hin = LOCAL ? ZERO : O::splat(t == 0 ? col0_value(k, 1) : row0_value(k, cbase + t * R));
-    V din = LOCAL ? ZERO : O::splat(row0_value(k, cbase));
+    V hout = LOCAL ? ZERO : O::splat(row0(cbase + (t + 1) * R));
+    V hin = LOCAL ? ZERO : O::splat(t == 0 ? col0(1) : row0(cbase + t * R));
+    V din = LOCAL ? ZERO : O::splat(row0(cbase));
     V ein = LOCAL ? ZERO : O::splat(O::NEG);
     V eout = ein;
     V best = ZERO;
     V endcol = O::splat(O::NEG);
+    V rowoff = O::splat((t - 1) * D);   // -i*D of the lane's row i (1-based) in this step
     const bool ladder = !LOCAL && (k.mode == 1 || (k.mode == 2 && !k.free_q));
-    V c0v = ladder ? O::splat(col0_value(k, 2)) : ZERO;
+    // lane 0's boundary H(i, cbase) of its next row and its change per row: the plain gap
+    // ladder steps by gap_extend; in the offset form the ladder is constant and free query
+    // ends step by D
+    V c0v = OFS ? O::splat(col0(2)) : ladder ? O::splat(col0_value(k, 2)) : ZERO;
+    const bool c0_steps = OFS ? !ladder : ladder;
     // Row shifts of ZERO_FILL tiles give every lane t > 0 lane t-1's H and E of the row.
     // Lane 0 takes the column-0 boundary: local H = E = 0 (the zero fill itself); otherwise
     // E stays the sentinel it holds and H steps down the column-0 ladder by gap_extend (or
-    // stays 0 with free query ends), so each is the kept old value of one DPP move.
-    const V LADV = ladder ? GEV : ZERO;
+    // stays 0 with free query ends; in the offset form the ladder stays and free ends step
+    // by D), so each is the kept old value of one DPP move.
+    const V LADV = OFS ? (ladder ? ZERO : O::splat(D)) : ladder ? GEV : ZERO;
     auto shift_boundary = [&](V &hi, V &ei, V ho, V eo) {
         if constexpr (!GroupLanes<G>::ZERO_FILL) {
             (void)hi, (void)ei, (void)ho, (void)eo;
@@ -639,13 +706,15 @@ __device__ __forceinline__ SweepOut sweep_pairs_pipe(const uint8_t *__restrict__
         load_row(cin, nxt);
         if ((unsigned)row < (unsigned)lr) {
             V e = ein;
-            V hgo = row_hgo<V, LOCAL>(hin, GOV, GOP);
+            V hgo = OFS ? O::add(hin, GOD) : row_hgo<V, LOCAL>(hin, GOV, GOP);
             V h = hin, hprev = ZERO;
             V xd = O::add(din, use[0]);
 #pragma unroll
             for (int kk = 0; kk < R; ++kk) {
                 const V xn = (kk + 1 < R) ? O::add(hup[kk], use[kk + 1]) : xd;
-                if constexpr (O::MAX3) {
+                if constexpr (OFS) {
+                    cell_offset<V>(xd, e, h, hup[kk], hgo, fn[kk], GOD);
+                } else if constexpr (O::MAX3) {
                     cell_max3<V, LOCAL, R>(kk, xd, e, h, hprev, best, hup[kk], hgo, fn[kk], GOV, GEV, ZERO);
                 } else if (LOCAL) {
                     e = O::mx(O::subu(e, GEP), hgo);
@@ -668,9 +737,10 @@ __device__ __forceinline__ SweepOut sweep_pairs_pipe(const uint8_t *__restrict__
             // unconditional (read only when track_endcol): a branch here splits the step, and
             // the scheduler then sinks the F updates below it as dependent add/max pairs, each
             // needing a wait state between them (about 70 s_nop per step)
-            if (!LOCAL) endcol = O::mx(endcol, h);
+            if (!LOCAL) endcol = O::mx(endcol, OFS ? O::add(h, rowoff) : h);
         }
         din = hin;
+        if constexpr (OFS) rowoff = O::add(rowoff, NDV);
         if constexpr (GroupLanes<G>::ZERO_FILL) {
             shift_boundary(hin, ein, hout, eout);
         } else {
@@ -680,7 +750,7 @@ __device__ __forceinline__ SweepOut sweep_pairs_pipe(const uint8_t *__restrict__
                 hin = c0v;
                 ein = LOCAL ? ZERO : O::splat(O::NEG);
             }
-            if (ladder) c0v = O::add(c0v, GEV);
+            if (c0_steps) c0v = O::add(c0v, LADV);
         }
     };
     for (int s = 0; s < nsteps; s += 2) {   // a step past nsteps has no row in range: harmless
@@ -705,8 +775,9 @@ __device__ __forceinline__ SweepOut sweep_pairs_pipe(const uint8_t *__restrict__
 #pragma unroll
         for (int kk = 0; kk < R; ++kk) {
             const int col = cbase + t * R + kk;
-            const int va = O::lo(hup[kk]);
-            const int vb = O::hi(hup[kk]);
+            const int off = (lr + col + 1) * D;   // offset form: H(lr, col+1) + off
+            const int va = O::lo(hup[kk]) - off;
+            const int vb = O::hi(hup[kk]) - off;
             if (col == lqa - 1) corner_a = va;
             if (col == lqb - 1) corner_b = vb;
             if (col < lqa) endrow_a = max(endrow_a, va);
@@ -716,8 +787,8 @@ __device__ __forceinline__ SweepOut sweep_pairs_pipe(const uint8_t *__restrict__
     int ca = corner_a, cb = corner_b;
     if (semi) {
         if (track_endcol) {
-            if (cbase + t * R + R - 1 == lqa - 1) ca = max(ca, O::lo(endcol));
-            if (cbase + t * R + R - 1 == lqb - 1) cb = max(cb, O::hi(endcol));
+            if (cbase + t * R + R - 1 == lqa - 1) ca = max(ca, O::lo(endcol) - lqa * D);
+            if (cbase + t * R + R - 1 == lqb - 1) cb = max(cb, O::hi(endcol) - lqb * D);
         }
         if (k.free_r) {
             ca = max(ca, endrow_a);

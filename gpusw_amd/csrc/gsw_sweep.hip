@@ -399,7 +399,8 @@ int plan_cross_units(const SchemeDev &s, int dtype_policy, const std::vector<Cro
     for (const CrossQuery &q : queries) {
         if (!fits_int32(s, q.len, max_lr))
             return fail(GSW_ERR_OVERFLOW, "score range of query %d exceeds the int32 kernels", q.id);
-        const int kind = pair_kind(s, dtype_policy, fits_int16(s, q.len, max_lr), fits_half(s, q.len, max_lr), half);
+        const int kind = pair_kind(s, dtype_policy, sweep_fits_int16(s, q.len, max_lr), sweep_fits_half(s, q.len, max_lr),
+                                  half);
         if (kind != KIND_I32 && !track_last_col) {
             packed_free[kind].push_back(q);
             continue;
@@ -1231,16 +1232,16 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
             if (lq <= wmax) {
                 if (lim32[lq] < 0) {
                     lim32[lq] = lr_limit(fits_int32, lq);
-                    lim16[lq] = lr_limit(fits_int16, lq);
-                    limh[lq] = lr_limit(fits_half, lq);
+                    lim16[lq] = lr_limit(sweep_fits_int16, lq);
+                    limh[lq] = lr_limit(sweep_fits_half, lq);
                 }
                 ok32 = lr <= lim32[lq];
                 ok16 = lr <= lim16[lq];
                 okh = lr <= limh[lq];
             } else {
                 ok32 = fits_int32(s, lq, lr);
-                ok16 = fits_int16(s, lq, lr);
-                okh = fits_half(s, lq, lr);
+                ok16 = sweep_fits_int16(s, lq, lr);
+                okh = sweep_fits_half(s, lq, lr);
             }
             if (!ok32) {
                 P.bad = k;
