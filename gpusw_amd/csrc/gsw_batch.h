@@ -70,6 +70,7 @@ struct Variant {
     const void *pairs_tab_fn;                       // pairs with the substitution table in LDS
     int kind;                                       // KIND_*
     LaunchFn stripe1 = nullptr;                     // the stripe kernel in one-wave workgroups
+    const void *pairs_perm_fn = nullptr;            // half2 pairs on byte-select tables (or null)
 };
 // The half2 and int32 rows of the variant table (gsw_sweep_f16.hip, gsw_sweep_i32.hip), and
 // the further local half2 tiles (gsw_sweep_f16x.hip), which also attaches stripe kernels to
@@ -78,6 +79,12 @@ std::vector<Variant> half_variants();
 std::vector<Variant> int32_variants();
 std::vector<Variant> half_extra_variants();
 void attach_half_stripes(std::vector<Variant> &rows);
+// The byte-select pairs kernels of the 8-lane half2 tiles up to 192 columns.
+void attach_half_perm(std::vector<Variant> &rows);
+// Byte-select pairs tables (binary16 entries, pairs_tab_kernel PERM) fit the scheme: at most
+// 6 codes and every profile entry's low byte zero; their LDS bytes.
+bool half_perm_ok(const SchemeDev &s);
+inline size_t pairs_perm_lds_bytes(int ns) { return 8 * (size_t)ns; }
 // LDS of the pairs table kernel: ns rows of (ns+2)^2 packed words (int32 tiles: ns+2),
 // times pairs_tab_copies interleaved copies (32 when that stays within 48 KiB, else 1).
 int pairs_tab_copies(const Variant &v, int ns);

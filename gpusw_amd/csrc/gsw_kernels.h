@@ -276,7 +276,13 @@ __global__ void __launch_bounds__(WAVES * 64) pairs_kernel(KArgs a) {
 // Small tables are stored in pairs_tab_copies() interleaved copies (word w of copy c at
 // w*copies + c) and lane l reads copy l mod copies: the lanes of an LDS pass then hit
 // distinct banks whatever entries they look up.
-template <typename V, int G, int R, bool LOCAL, int WAVES>
+//
+// PERM (binary16, at most 6 codes, every entry's low byte zero -- half_perm_ok on the host):
+// the table is 8 bytes per reference code, byte k the high byte of the entry for query code
+// k (k = ns: past the end, k = ns + 1: front padding), and each lane keeps one v_perm_b32
+// selector per column instead of a table offset: a row costs one 8-byte LDS load and each
+// column one VALU op, with no per-column load to wait for.
+template <typename V, int G, int R, bool LOCAL, int WAVES, bool PERM = false>
 __global__ void __launch_bounds__(WAVES * 64) pairs_tab_kernel(KArgs a) {
     using O = VOps<V>;
     constexpr int GPW = 64 / G;
@@ -286,17 +292,27 @@ __global__ void __launch_bounds__(WAVES * 64) pairs_tab_kernel(KArgs a) {
     const int ns = a.ns, nq = ns + 2, npad = ns, zpad = ns + 1;
     const int row_words = O::PACKED ? nq * nq : nq;
     const int copies = a.tab_copies;
-    for (int idx = threadIdx.x; idx < ns * row_words * copies; idx += blockDim.x) {
-        const int w = idx / copies;
-        const int c = w / row_words, rem = w - c * row_words;
-        const int qa = O::PACKED ? rem / nq : rem, qb = O::PACKED ? rem - (rem / nq) * nq : rem;
-        const int va = qa == npad ? O::NEG : prof_score<V, LOCAL>(a.k, qa == zpad ? 0 : a.sub[c * ns + qa]);
-        const int vb = qb == npad ? O::NEG : prof_score<V, LOCAL>(a.k, qb == zpad ? 0 : a.sub[c * ns + qb]);
-        reinterpret_cast<uint32_t *>(lds)[idx] = O::pack(va, vb);
+    if constexpr (PERM) {
+        static_assert(O::MAX3, "byte-select tables hold binary16 entries");
+        for (int idx = threadIdx.x; idx < ns * 8; idx += blockDim.x) {
+            const int c = idx >> 3, kq = idx & 7;
+            const int v = kq < ns ? prof_score<V, LOCAL>(a.k, a.sub[c * ns + kq])
+                          : kq == npad ? PERM_NEG : prof_score<V, LOCAL>(a.k, 0);
+            lds[idx] = (char)(O::pack(v, v) >> 8);   // the high byte of v's binary16 bits
+        }
+    } else {
+        for (int idx = threadIdx.x; idx < ns * row_words * copies; idx += blockDim.x) {
+            const int w = idx / copies;
+            const int c = w / row_words, rem = w - c * row_words;
+            const int qa = O::PACKED ? rem / nq : rem, qb = O::PACKED ? rem - (rem / nq) * nq : rem;
+            const int va = qa == npad ? O::NEG : prof_score<V, LOCAL>(a.k, qa == zpad ? 0 : a.sub[c * ns + qa]);
+            const int vb = qb == npad ? O::NEG : prof_score<V, LOCAL>(a.k, qb == zpad ? 0 : a.sub[c * ns + qb]);
+            reinterpret_cast<uint32_t *>(lds)[idx] = O::pack(va, vb);
+        }
     }
     __syncthreads();
     const int trow = row_words * copies * 4;
-    const char *tab_lane = lds + (lane % copies) * 4;
+    const char *tab_lane = PERM ? lds : lds + (lane % copies) * 4;
     const PairTask *tasks = static_cast<const PairTask *>(a.items);
     const int nitems = a.nitems_ptr ? __builtin_amdgcn_readfirstlane(*a.nitems_ptr) : a.nitems;
     const int nblocks = (nitems + GPW - 1) / GPW;
@@ -322,10 +338,12 @@ __global__ void __launch_bounds__(WAVES * 64) pairs_tab_kernel(KArgs a) {
             const int col = t * R + kk - shift;
             const int ca = col >= lqa ? npad : col < 0 ? zpad : min((int)a.qcodes[qsa + col], ns - 1);
             const int cb = col >= lqb ? npad : col < 0 ? zpad : min((int)a.qcodes[qsb + col], ns - 1);
-            toff[kk] = (O::PACKED ? ca * nq + cb : ca) * copies * 4;
+            // byte-select: bytes 0 and 2 zero (selector 12), bytes 1 and 3 the table bytes of
+            // the two codes (selector k picks byte k of the row's 8)
+            toff[kk] = PERM ? 0x0C | ca << 8 | 0x0C << 16 | cb << 24 : (O::PACKED ? ca * nq + cb : ca) * copies * 4;
         }
         const int nsteps = __builtin_amdgcn_readfirstlane(wave_max_i32(lr)) + G - 1;
-        const SweepOut res = sweep_group<V, G, R, LOCAL, false, true, true>(
+        const SweepOut res = sweep_group<V, G, R, LOCAL, false, true, true, PERM>(
             a.rcodes + rs, lr, nsteps, tab_lane, t, a.k, lqa, lqb, -shift, nullptr, nullptr, false, false, toff, trow);
         if (t == 0 && valid) {
             emit<V>(a, task.oa, qa, task.ref, res.a);

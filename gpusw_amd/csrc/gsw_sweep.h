@@ -229,6 +229,10 @@ __device__ __forceinline__ int prof_score(const SweepConst &k, int v) {
     return DiagOffset<V, LOCAL>::ON ? v - 2 * k.ge : v;
 }
 
+// Past-the-end sentinel of the byte-select pairs kernel: binary16 0xF700, whose low byte is
+// zero like every entry that kernel selects (see pairs_tab_kernel).
+constexpr int PERM_NEG = -28672;
+
 constexpr int RES_NONE = -0x40000000 - 7;   // below every reachable result
 
 // H(i, j-1) + gap_open for a row's first column.  The int16 local form clamps it at 0
@@ -312,7 +316,11 @@ struct SweepOut {
 // TAB: no profile; prof_lane is a substitution table of trow bytes per reference code
 // and toff[kk] the byte offset of column kk's (query A code, query B code) entry in a row,
 // so each column costs one LDS load (see the pairs table kernel).
-template <typename V, int G, int R, bool LOCAL, bool STRIPE = false, bool PF_ = false, bool TAB = false>
+// PERM (with TAB): prof_lane is a table of 8 bytes per reference code, byte k the high byte
+// of the binary16 entry for query code k, and toff[kk] a v_perm_b32 selector that builds
+// column kk's (A, B) pair from them: one 8-byte LDS load per row, one VALU op per column.
+template <typename V, int G, int R, bool LOCAL, bool STRIPE = false, bool PF_ = false, bool TAB = false,
+          bool PERM = false>
 __device__ __forceinline__ SweepOut sweep_group(const uint8_t *__restrict__ rseq, int lr, int nsteps,
                                                 const char *prof_lane, int t, const SweepConst &k,
                                                 int lqa, int lqb, int cbase, uint32_t *bh = nullptr,
@@ -369,7 +377,12 @@ __device__ __forceinline__ SweepOut sweep_group(const uint8_t *__restrict__ rseq
     // out-of-range codes score as the sentinel
     auto code_at = [&](int r) -> int { return min((int)rseq[r], k.ns - 1); };
     auto load_row = [&](int c, V *dst) {
-        if constexpr (TAB) {
+        if constexpr (PERM) {
+            const uint2 w = *reinterpret_cast<const uint2 *>(prof_lane + c * 8);
+#pragma unroll
+            for (int kk = 0; kk < R; ++kk)
+                dst[kk] = O::from_bits((int)__builtin_amdgcn_perm(w.y, w.x, (uint32_t)toff[kk]));
+        } else if constexpr (TAB) {
             // a code is below 256 and a row fits LDS: a 24-bit multiply (full rate) is exact
             const char *pc = prof_lane + __umul24((unsigned)c, (unsigned)trow);
 #pragma unroll

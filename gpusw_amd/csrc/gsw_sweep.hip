@@ -43,6 +43,7 @@ std::vector<Variant> &variant_table() {
             std::vector<Variant> rows = half_variants();
             for (const Variant &x : half_extra_variants()) rows.push_back(x);
             attach_half_stripes(rows);
+            attach_half_perm(rows);
             return rows;
         }();
         std::set<std::pair<int, int>> tiles;
@@ -223,6 +224,35 @@ int pairs_tab_copies(const Variant &v, int ns) {
 size_t pairs_tab_lds_bytes(const Variant &v, int ns) {
     const size_t nq = (size_t)ns + 2;
     return 4 * (size_t)ns * (v.packed ? nq * nq : nq) * pairs_tab_copies(v, ns);
+}
+
+// The scheme admits byte-select tables: at most 6 codes (8 bytes per row with the two
+// padding codes), and every binary16 profile entry -- s, or s + 2D in the diagonal-offset
+// form, and the front padding's 0 or 2D -- an integer of at most 2048 whose binary16 low
+// byte is zero (at most three significant bits: 0-8, 10, 12, 14, 16, 20, 24, ...).
+bool half_perm_ok(const SchemeDev &s) {
+    if (s.ns > 6 || !half_entries(s)) return false;
+    const int64_t d2 = s.mode == GSW_MODE_LOCAL ? 0 : -2 * (int64_t)s.ge;
+    auto high_byte_only = [](int64_t v) {
+        const int64_t a = v < 0 ? -v : v;
+        if (a > 2048) return false;
+        if (a < 8) return true;
+        const int e = 63 - __builtin_clzll((unsigned long long)a);
+        return (a & ((INT64_C(1) << (e - 2)) - 1)) == 0;
+    };
+    if (!high_byte_only(d2)) return false;
+    for (int32_t v : s.h_sub)
+        if (!high_byte_only(v + d2)) return false;
+    return true;
+}
+
+// A table-kernel batch runs the byte-select kernel where the tile has one and the scheme
+// allows it (perm), else the LDS-table kernel.
+static const void *tab_kernel_fn(const Variant &v, bool perm) {
+    return perm && v.pairs_perm_fn ? v.pairs_perm_fn : v.pairs_tab_fn;
+}
+static size_t tab_kernel_lds(const Variant &v, bool perm, int ns) {
+    return perm && v.pairs_perm_fn ? pairs_perm_lds_bytes(ns) : pairs_tab_lds_bytes(v, ns);
 }
 
 // Kernel of explicit pairs: by default the substitution-table kernel (three waves per SIMD,
@@ -1057,13 +1087,15 @@ struct PairGraph {
 static std::mutex g_graph_mu;
 static std::map<std::tuple<int, int, int>, std::unique_ptr<PairGraph>> g_pair_graphs;
 
-static int pair_graph(DeviceCtx *ctx, const Variant &v, bool tab, int slot, int ns, int rows, PairGraph **out) {
+static int pair_graph(DeviceCtx *ctx, const Variant &v, bool tab, bool perm, int slot, int ns, int rows,
+                      PairGraph **out) {
     std::lock_guard<std::mutex> lk(g_graph_mu);
-    auto key = std::make_tuple(ctx->device, 2 * v.id + (tab ? 1 : 0), slot);
+    perm = perm && tab && v.pairs_perm_fn;
+    auto key = std::make_tuple(ctx->device, 4 * v.id + (tab ? 1 : 0) + (perm ? 2 : 0), slot);
     auto &pg = g_pair_graphs[key];
     const int waves = tab ? PAIRS_TAB_WAVES : pairs_waves(v, rows, (size_t)ctx->max_smem);
     if (waves == 0) return fail(GSW_ERR_LIMIT, "pairs tile (%d, %d) does not fit in LDS", v.G, v.R);
-    const size_t lds = tab ? pairs_tab_lds_bytes(v, ns) : (size_t)waves * pairs_wave_bytes(v, rows);
+    const size_t lds = tab ? tab_kernel_lds(v, perm, ns) : (size_t)waves * pairs_wave_bytes(v, rows);
     if (pg && pg->lds == lds && pg->waves == waves) {
         *out = pg.get();
         return GSW_OK;
@@ -1083,7 +1115,7 @@ static int pair_graph(DeviceCtx *ctx, const Variant &v, bool tab, int slot, int 
     }
     pg->lds = lds;
     pg->waves = waves;
-    const void *fn = tab ? v.pairs_tab_fn : v.pairs_fn;
+    const void *fn = tab ? tab_kernel_fn(v, perm) : v.pairs_fn;
     GSW_TRY(occupancy_grid(ctx, fn, waves * 64, lds, &pg->grid));
     GSW_HIP_TRY(hipGraphCreate(&pg->graph, 0));
     KArgs a{};
@@ -1146,6 +1178,11 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
     const double mean_lr = rs.n_nonempty ? (double)rs.total / rs.n_nonempty : 1.0;
     const int T = n >= 65536 ? host_threads() : 1;
     const bool half = half_enabled();
+    // half2 pairs on byte-select tables where the scheme allows (GSW_PAIRS_PERM=0: LDS tables)
+    const bool perm = half && half_perm_ok(s) && [] {
+        const char *e = std::getenv("GSW_PAIRS_PERM");
+        return !(e && e[0] == '0');
+    }();
     // profile rows of the profile kernel: codes past the set's largest one never occur
     const int prow = std::min(s.ns - 1, rs.max_code);
     const size_t nv = 2 * variant_table().size();   // kinds: 2 * variant id + (table kernel)
@@ -1356,7 +1393,7 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
             const Batch &B = batches[bb];
             if (slot_of.count({B.kind, B.slot})) continue;
             PairGraph *pg = nullptr;
-            GSW_TRY(pair_graph(ctx, *B.v, B.tab, B.slot, s.ns, prow + 1, &pg));
+            GSW_TRY(pair_graph(ctx, *B.v, B.tab, perm, B.slot, s.ns, prow + 1, &pg));
             slot_of[{B.kind, B.slot}] = pg;
             KArgs a = base_args(s, qs, rs, d_out);
             a.sat = sb.d_sat;
@@ -1370,7 +1407,7 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
             hipKernelNodeParams kp{};
             a.tab_copies = B.tab ? pairs_tab_copies(*B.v, s.ns) : 1;
             a.prow = prow;
-            kp.func = const_cast<void *>(B.tab ? B.v->pairs_tab_fn : B.v->pairs_fn);
+            kp.func = const_cast<void *>(B.tab ? tab_kernel_fn(*B.v, perm) : B.v->pairs_fn);
             kp.gridDim = dim3(pg->grid);
             kp.blockDim = dim3(pg->waves * 64);
             kp.sharedMemBytes = (unsigned)pg->lds;

@@ -36,4 +36,23 @@ void attach_half_stripes(std::vector<Variant> &rows) {
     }
 }
 
+// Byte-select pairs kernels (pairs_tab_kernel PERM) of the 8-lane half2 tiles that explicit
+// pairs of up to 192 columns run on (150-nt reads take the (8, 19) one).
+void attach_half_perm(std::vector<Variant> &rows) {
+#define GSW_PERM_CASE(RR)                                                                             \
+    if (v.R == RR)                                                                                    \
+        v.pairs_perm_fn = v.local ? reinterpret_cast<const void *>(&pairs_tab_kernel<h2, 8, RR, true, PAIRS_TAB_WAVES, true>) \
+                                  : reinterpret_cast<const void *>(&pairs_tab_kernel<h2, 8, RR, false, PAIRS_TAB_WAVES, true>);
+    for (Variant &v : rows) {
+        if (v.kind != KIND_F16 || v.G != 8 || !v.pairs_tab_fn) continue;
+        GSW_PERM_CASE(8)
+        GSW_PERM_CASE(12)
+        GSW_PERM_CASE(16)
+        GSW_PERM_CASE(19)
+        GSW_PERM_CASE(20)
+        GSW_PERM_CASE(24)
+    }
+#undef GSW_PERM_CASE
+}
+
 }  // namespace gsw

@@ -57,3 +57,27 @@ def test_offset_form_gap_extend_range(ge):
     got = al.score_cross()
     assert al._b.last_stats.plain_pairs == 0
     assert np.array_equal(got.astype(np.int64), native.score_encoded_cross(al._queries, al._refs, scheme))
+
+
+@pytest.mark.parametrize("mode,fq,fr", [("local", True, True), ("global", True, True),
+                                        ("semiglobal", True, False), ("semiglobal", False, True)])
+@pytest.mark.parametrize("match", [2, 9])
+def test_pairs_byte_select_and_lds_tables_agree(monkeypatch, mode, fq, fr, match):
+    """half2 read pairs on byte-select tables (every binary16 entry with a zero low byte:
+    match 2) and on LDS tables (GSW_PAIRS_PERM=0, or match 9, whose entry 9 + 4 = 13 needs
+    more than three significant bits): the same scores, equal to the oracle."""
+    scheme = Scheme(mode=mode, match=match, free_end_gaps_query=fq, free_end_gaps_ref=fr)
+    rng = np.random.default_rng(stable_seed(f"perm/{mode}/{fq}/{fr}/{match}"))
+    reads = rand_seqs(rng, 300, 140, 161, "ACGTN")
+    windows = rand_seqs(rng, 60, 180, 330, "ACGTN")
+    al = Aligner(scheme, backend="hip").index(windows).set_queries(reads)
+    qi = rng.integers(0, 300, 9000).astype(np.int32)
+    rj = rng.integers(0, 60, 9000).astype(np.int32)
+    got = al.score_pairs(qi, rj)
+    if match == 2 or mode == "local":   # global match 9 leaves the half2 envelope: int16
+        assert _kind(al._b.last_stats) == "half2"
+    monkeypatch.setenv("GSW_PAIRS_PERM", "0")
+    lds = al.score_pairs(qi, rj)
+    assert np.array_equal(got, lds)
+    want = native.score_encoded_pairs(al._queries, al._refs, qi, rj, scheme)
+    assert np.array_equal(got.astype(np.int64), want)
