@@ -205,8 +205,9 @@ def _kernel_name(variant: int, striped: int) -> str | None:
     if variant < 0:
         return None
     g, r, elem, local = _hip.variant_info(variant)
-    kind, waves = {0: ("cross", 8), 1: ("stripe", 8), 2: ("pairs", 8), 3: ("pairs_tab", 4)}[int(striped)]
-    return f"void gsw::{kind}_kernel<{_ELEM_NAME[elem]}, {g}, {r}, {str(local).lower()}, {waves}>(gsw::KArgs)"
+    kind, waves, extra = {0: ("cross", 8, ""), 1: ("stripe", 8, ""), 2: ("pairs", 8, ""), 3: ("pairs_tab", 4, ""),
+                          4: ("pairs_tab", 4, ", true")}[int(striped)]
+    return f"void gsw::{kind}_kernel<{_ELEM_NAME[elem]}, {g}, {r}, {str(local).lower()}, {waves}{extra}>(gsw::KArgs)"
 
 
 def _measured_traffic(workload: str, kernel: str | None):

@@ -1616,7 +1616,8 @@ int batch_pairs(DeviceCtx *ctx, const SchemeDev &s, int dtype_policy, const SeqS
             res->top_ms = (t_hi - t_lo) / cnt;
             res->top_cells = (int64_t)(sum_cells / cnt);
             res->top_variant = dominant / 2;
-            res->top_striped = (dominant & 1) ? 3 : 2;   // pairs kernel (3: table kernel)
+            // pairs kernel (3: table kernel, 4: its byte-select form)
+            res->top_striped = !(dominant & 1) ? 2 : perm && variant_table()[dominant / 2].pairs_perm_fn ? 4 : 3;
         }
     }
 

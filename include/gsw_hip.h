@@ -75,7 +75,7 @@ typedef struct gsw_stats {
                           /*   batches (first start to last end) per batch      */
     int64_t top_cells;    /*   cells of that launch (pairs: mean per batch)      */
     int32_t top_variant;  /*   its variant id (-1: none)                         */
-    int32_t top_striped;  /*   kernel: 0 cross, 1 stripe, 2 pairs, 3 pairs table */
+    int32_t top_striped;  /*   kernel: 0 cross, 1 stripe, 2 pairs, 3 pairs table, 4 byte-select table */
 } gsw_stats;
 
 /* ---- runtime (src/gpusw/_compile.py:30-46; backends/cuda.py:29-41) ----- */

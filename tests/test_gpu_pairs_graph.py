@@ -141,7 +141,8 @@ def test_dna_pairs_both_kernels(mode, tab, ref_codes, monkeypatch):
     rj = rng.integers(0, 80, 5000).astype(np.int32)
     al = Aligner(scheme, backend="hip").index(synth.pack(refs)).set_queries(synth.pack(queries))
     got = al.score_pairs(qi, rj)
-    assert al._b.last_stats.top_striped == (2 if tab == "0" else 3)   # default: the table kernel
+    # default: the table kernel (4: its byte-select form, on half2 tiles up to 192 columns)
+    assert al._b.last_stats.top_striped in ((2,) if tab == "0" else (3, 4))
     want = native.score_encoded_pairs(al._queries, al._refs, qi, rj, scheme)
     assert np.array_equal(got.astype(np.int64), want)
 
