@@ -701,14 +701,17 @@ def run_read_pairs(args):
     # both modes are timed before any check runs (the CPU oracle's threads and the comparator
     # stay out of the timed calls)
     timed = []
-    # a call is ~10 ms: at least three untimed calls per mode (the first calls of a new
-    # aligner still settle host buffers and planning threads)
+    # a call is ~6-10 ms: at least three untimed calls per mode (the first calls of a new
+    # aligner still settle host buffers and planning threads) and at least 0.25 s of them
+    # (a kernel trace showed a mode's first call with 1.5x longer batches than its later ones)
     warmup = max(args.warmup, 3)
     for mode in ("semiglobal", "global"):
         scheme = Scheme.preset("dna").replace(mode=mode, dtype="int16")
         al = Aligner(scheme, backend="hip").index((wc, wo)).set_queries((rc, ro))
-        for _ in range(warmup):
+        t_w, n_w = time.perf_counter(), 0
+        while n_w < warmup or time.perf_counter() - t_w < 0.25:
             al.score_pairs(qi, rj)
+            n_w += 1
         al._b.history = []
         t0 = time.perf_counter()
         for _ in range(args.steps):
